@@ -1,0 +1,486 @@
+"""gopf_amd -- MI355X spectral time step of gopf behind its own interfaces.
+
+Python view of the C ABI in ``include/gpf.h`` (ctypes; no torch types).  The
+classes keep the reference's names so code reads like gopf's own tests:
+
+  Model            pf.Model        (pf/model.go:116-260)
+  Solver           pf.Solver + TimeStepper (pf/solver.go:29-134, euler.go, rk4.go)
+  FourierTransform pfutil.FFTWWrapper (pfutil/fftWrap.go:8-95)
+  NewVandeven      pf.NewVandeven  (pf/vandeven.go:13-40)
+
+All compute runs in ``libgopf_amd.so`` (hand-written HIP for gfx950); loading
+fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple, Union
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgopf_amd.so")
+
+GPF_FP64, GPF_FP32 = 0, 1
+GPF_EULER, GPF_RK4 = 0, 1
+KIND_CONCRETE, KIND_SQGRAD, KIND_ELAST = 0, 1, 2
+
+
+class GpfError(RuntimeError):
+    """A failed library call; the reference panics in the same situations."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[gpf error {code}] {msg}")
+        self.code = code
+
+
+class _Options(C.Structure):
+    _fields_ = [("precision", C.c_int), ("stepper", C.c_int), ("filter_order", C.c_int),
+                ("fused", C.c_int), ("device", C.c_int)]
+
+
+class _Term(C.Structure):
+    _fields_ = [("kind", C.c_int), ("coef_re", C.c_double), ("coef_im", C.c_double), ("brick", C.c_int),
+                ("has_lap", C.c_int), ("lap", C.c_int), ("user", C.c_int)]
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lp = C.POINTER(C.c_long)
+_vp = C.c_void_p
+
+
+def lib():
+    """Load libgopf_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m gopf_amd.build` (the HIP library is required)")
+        L = C.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L) -> None:
+    sig = {
+        "gpf_last_error": (C.c_char_p, []),
+        "gpf_model_create": (C.c_int, [C.POINTER(_vp)]),
+        "gpf_model_destroy": (None, [_vp]),
+        "gpf_model_add_field": (C.c_int, [_vp, C.c_char_p, _dp, C.c_long]),
+        "gpf_model_add_scalar": (C.c_int, [_vp, C.c_char_p, C.c_double, C.c_double]),
+        "gpf_model_add_equation": (C.c_int, [_vp, C.c_char_p]),
+        "gpf_model_register_square_gradient": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.c_double, C.c_int]),
+        "gpf_model_register_homogeneous_modulus": (C.c_int, [_vp, C.c_char_p, C.c_char_p, _dp, _dp]),
+        "gpf_model_register_polynomial": (C.c_int, [_vp, C.c_char_p, C.c_int, _dp, _dp, C.c_int]),
+        "gpf_model_init": (C.c_int, [_vp]),
+        "gpf_model_num_bricks": (C.c_int, [_vp, _ip, _ip]),
+        "gpf_model_brick_name": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int]),
+        "gpf_model_num_terms": (C.c_int, [_vp, C.c_int, _ip, _ip]),
+        "gpf_model_get_term": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.POINTER(_Term)]),
+        "gpf_options_default": (None, [C.POINTER(_Options)]),
+        "gpf_solver_create": (C.c_int, [_vp, _ip, C.c_int, C.c_double, C.POINTER(_Options), C.POINTER(_vp)]),
+        "gpf_solver_destroy": (None, [_vp]),
+        "gpf_solver_propagate": (C.c_int, [_vp, C.c_int]),
+        "gpf_solver_stepper_propagate": (C.c_int, [_vp, C.c_int]),
+        "gpf_solver_get_time": (C.c_int, [_vp, _dp]),
+        "gpf_solver_get_field": (C.c_int, [_vp, C.c_int, _dp, C.c_long]),
+        "gpf_solver_set_field": (C.c_int, [_vp, C.c_int, _dp, C.c_long]),
+        "gpf_solver_energy": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_double, _dp, _dp]),
+        "gpf_solver_monitor_energy": (C.c_int, [_vp, C.c_int, _dp, C.c_double]),
+        "gpf_solver_energy_series": (C.c_int, [_vp, _dp, _dp, C.c_long, _lp]),
+        "gpf_solver_synchronize": (C.c_int, [_vp]),
+        "gpf_solver_is_fused": (C.c_int, [_vp]),
+        "gpf_solver_enable_timing": (C.c_int, [_vp, C.c_int]),
+        "gpf_solver_pass_times": (C.c_int, [_vp, C.c_int, _dp, _lp, C.c_char_p, C.c_int]),
+        "gpf_solver_stream": (_vp, [_vp]),
+        "gpf_fft_create": (C.c_int, [_ip, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+        "gpf_fft_destroy": (None, [_vp]),
+        "gpf_fft_forward": (C.c_int, [_vp, _dp, C.c_long]),
+        "gpf_fft_inverse": (C.c_int, [_vp, _dp, C.c_long]),
+        "gpf_fft_freq": (C.c_int, [_ip, C.c_int, C.c_long, _dp]),
+        "gpf_fft_conjugate_node": (C.c_long, [_ip, C.c_int, C.c_long]),
+        "gpf_vandeven_table": (C.c_int, [C.c_int, _dp]),
+        "gpf_vandeven_eval": (C.c_double, [_dp, C.c_double]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise GpfError(rc, lib().gpf_last_error().decode())
+
+
+def _ints(v: Sequence[int]):
+    arr = (C.c_int * len(v))(*[int(x) for x in v])
+    return arr
+
+
+def _cbuf(data, n: Optional[int] = None) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(data, dtype=np.complex128).reshape(-1))
+    if n is not None and a.size != n:
+        raise ValueError("length mismatch")
+    return a
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+# ---------------------------------------------------------------------------
+# registered terms
+# ---------------------------------------------------------------------------
+
+@dataclass
+class SquaredGradient:
+    """pf.SquaredGradient (squareGradientTerm.go:15-37): Factor * |grad Field|^2."""
+    Field: str
+    Factor: float = 1.0
+
+
+@dataclass
+class HomogeneousModulusLinElast:
+    """pf.HomogeneousModulusLinElast (homoLinElast.go:33-43, 137-150)."""
+    FieldName: str
+    MatProp: np.ndarray          # 81 values C_ijkl
+    Misfit: np.ndarray           # 3x3
+
+
+def CubicMaterial(c11: float, c12: float, c44: float) -> np.ndarray:
+    """elasticity.CubicMaterial (rank4.go:113-128)."""
+    t = np.zeros((3, 3, 3, 3))
+    for i in range(3):
+        t[i, i, i, i] = c11
+    for i in range(3):
+        for j in range(i + 1, 3):
+            t[i, i, j, j] = c12
+            t[j, j, i, i] = c12
+            t[i, j, i, j] = c44
+            t[j, i, j, i] = c44
+    return t.reshape(81)
+
+
+def Isotropic(bulk: float, poisson: float) -> np.ndarray:
+    """elasticity.Isotropic (rank4.go:84-110)."""
+    shear = 3.0 * bulk * (1.0 - 2.0 * poisson) / (2.0 * (1.0 + poisson))
+    t = np.zeros((3, 3, 3, 3))
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                for l in range(3):
+                    v = 0.0
+                    if i == j and k == l:
+                        v += bulk - 2.0 * shear / 3.0
+                    if i == k and j == l:
+                        v += shear
+                    if i == l and j == k:
+                        v += shear
+                    t[i, j, k, l] = v
+    return t.reshape(81)
+
+
+# ---------------------------------------------------------------------------
+# pf.Model
+# ---------------------------------------------------------------------------
+
+class Model:
+    """pf.Model held by the library (model.go:116-260)."""
+
+    def __init__(self) -> None:
+        h = _vp()
+        _check(lib().gpf_model_create(C.byref(h)))
+        self._h = h
+        self._fields: List[str] = []
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().gpf_model_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    # -- building --------------------------------------------------------------
+    def AddField(self, name: str, data=None, n: Optional[int] = None) -> None:
+        """NewField + AddField (model.go:44-57, 141-144)."""
+        if data is None:
+            if n is None:
+                raise ValueError("need data or n")
+            _check(lib().gpf_model_add_field(self._h, name.encode(), None, n))
+        else:
+            buf = _cbuf(data)
+            _check(lib().gpf_model_add_field(self._h, name.encode(), _dptr(buf), buf.size))
+        self._fields.append(name)
+
+    def AddScalar(self, name: str, value: complex) -> None:
+        v = complex(value)
+        _check(lib().gpf_model_add_scalar(self._h, name.encode(), v.real, v.imag))
+
+    def AddEquation(self, eq: str) -> None:
+        _check(lib().gpf_model_add_equation(self._h, eq.encode()))
+
+    def RegisterExplicitTerm(self, name: str, term) -> None:
+        self._register(name, term, implicit=False)
+
+    def RegisterImplicitTerm(self, name: str, term) -> None:
+        self._register(name, term, implicit=True)
+
+    def _register(self, name: str, term, implicit: bool) -> None:
+        if isinstance(term, SquaredGradient):
+            _check(lib().gpf_model_register_square_gradient(self._h, name.encode(), term.Field.encode(),
+                                                            float(term.Factor), 1 if implicit else 0))
+        elif isinstance(term, HomogeneousModulusLinElast):
+            if implicit:
+                raise GpfError(1, "HomogeneousModulusLinElast is an explicit term")
+            Cm = np.ascontiguousarray(np.asarray(term.MatProp, dtype=np.float64).reshape(81))
+            mis = np.ascontiguousarray(np.asarray(term.Misfit, dtype=np.float64).reshape(9))
+            _check(lib().gpf_model_register_homogeneous_modulus(self._h, name.encode(), term.FieldName.encode(),
+                                                                _dptr(Cm), _dptr(mis)))
+        else:
+            raise TypeError(f"term type {type(term).__name__} cannot run on the device")
+
+    def RegisterPolynomial(self, name: str, terms: List[Tuple[float, Dict[str, float]]]) -> None:
+        """RegisterFunction (model.go:400-412) for a polynomial of the fields' real parts."""
+        nf = len(self._fields)
+        coefs = np.array([float(c) for c, _ in terms], dtype=np.float64)
+        pw = np.zeros((len(terms), nf), dtype=np.float64)
+        for t, (_, powers) in enumerate(terms):
+            for f, p in powers.items():
+                pw[t, self._fields.index(f)] = p
+        pw = np.ascontiguousarray(pw.reshape(-1))
+        _check(lib().gpf_model_register_polynomial(self._h, name.encode(), len(terms), _dptr(coefs),
+                                                   _dptr(pw) if pw.size else None, nf))
+
+    def Init(self) -> None:
+        _check(lib().gpf_model_init(self._h))
+
+    # -- introspection -----------------------------------------------------------
+    def AllFieldNames(self) -> List[str]:
+        nf, nd = C.c_int(), C.c_int()
+        _check(lib().gpf_model_num_bricks(self._h, C.byref(nf), C.byref(nd)))
+        out = []
+        buf = C.create_string_buffer(256)
+        for i in range(nf.value + nd.value):
+            _check(lib().gpf_model_brick_name(self._h, i, buf, 256))
+            out.append(buf.value.decode())
+        return out
+
+    def RHS(self, eq: int) -> Tuple[List[_Term], List[_Term]]:
+        nt, nd = C.c_int(), C.c_int()
+        _check(lib().gpf_model_num_terms(self._h, eq, C.byref(nt), C.byref(nd)))
+        res = ([], [])
+        for den, count in ((0, nt.value), (1, nd.value)):
+            for i in range(count):
+                t = _Term()
+                _check(lib().gpf_model_get_term(self._h, eq, den, i, C.byref(t)))
+                res[den].append(t)
+        return res
+
+    def NumFields(self) -> int:
+        return len(self._fields)
+
+    def FieldIndex(self, name: str) -> int:
+        return self._fields.index(name)
+
+
+# ---------------------------------------------------------------------------
+# pf.Solver
+# ---------------------------------------------------------------------------
+
+class Solver:
+    """pf.Solver with a device TimeStepper (solver.go:29-134)."""
+
+    def __init__(self, model: Model, domainSize: Sequence[int], dt: float, stepper: str = "euler",
+                 precision: str = "fp64", filter_order: int = 0, fused: bool = True, device: int = 0):
+        o = _Options()
+        lib().gpf_options_default(C.byref(o))
+        o.precision = GPF_FP32 if precision in ("fp32", "float32") else GPF_FP64
+        if stepper not in ("euler", "rk4"):
+            raise GpfError(1, "Unknown stepper scheme")
+        o.stepper = GPF_RK4 if stepper == "rk4" else GPF_EULER
+        o.filter_order = int(filter_order)
+        o.fused = 1 if fused else 0
+        o.device = int(device)
+        h = _vp()
+        self.dims = [int(d) for d in domainSize]
+        self.n = int(np.prod(self.dims))
+        _check(lib().gpf_solver_create(model._h, _ints(self.dims), len(self.dims), float(dt), C.byref(o), C.byref(h)))
+        self._h = h
+        self.model = model
+        self.Dt = dt
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().gpf_solver_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def Propagate(self, nsteps: int) -> None:
+        _check(lib().gpf_solver_propagate(self._h, int(nsteps)))
+
+    def StepperPropagate(self, nsteps: int) -> None:
+        _check(lib().gpf_solver_stepper_propagate(self._h, int(nsteps)))
+
+    def Solve(self, nepochs: int, nsteps: int, callbacks=()) -> None:
+        """Solver.Solve (solver.go:106-120) with host callbacks cb(solver, epoch)."""
+        for epoch in range(nepochs):
+            self.Propagate(nsteps)
+            for cb in callbacks:
+                cb(self, epoch)
+
+    def GetTime(self) -> float:
+        t = C.c_double()
+        _check(lib().gpf_solver_get_time(self._h, C.byref(t)))
+        return t.value
+
+    def _idx(self, f: Union[int, str]) -> int:
+        return self.model.FieldIndex(f) if isinstance(f, str) else int(f)
+
+    def Field(self, f: Union[int, str]) -> np.ndarray:
+        out = np.zeros(self.n, dtype=np.complex128)
+        _check(lib().gpf_solver_get_field(self._h, self._idx(f), _dptr(out), self.n))
+        return out
+
+    def SetField(self, f: Union[int, str], data) -> None:
+        buf = _cbuf(data, self.n)
+        _check(lib().gpf_solver_set_field(self._h, self._idx(f), _dptr(buf), self.n))
+
+    def Energy(self, f: Union[int, str], poly: Sequence[float], kappa: float) -> Tuple[float, float]:
+        p = np.ascontiguousarray(np.asarray(poly, dtype=np.float64))
+        e, m = C.c_double(), C.c_double()
+        _check(lib().gpf_solver_energy(self._h, self._idx(f), p.size, _dptr(p), float(kappa), C.byref(e), C.byref(m)))
+        return e.value, m.value
+
+    def MonitorEnergy(self, poly: Sequence[float], kappa: float) -> None:
+        p = np.ascontiguousarray(np.asarray(poly, dtype=np.float64))
+        _check(lib().gpf_solver_monitor_energy(self._h, p.size, _dptr(p), float(kappa)))
+
+    def EnergySeries(self) -> Tuple[np.ndarray, np.ndarray]:
+        n = C.c_long()
+        _check(lib().gpf_solver_energy_series(self._h, None, None, 0, C.byref(n)))
+        e = np.zeros(n.value)
+        m = np.zeros(n.value)
+        _check(lib().gpf_solver_energy_series(self._h, _dptr(e), _dptr(m), n.value, C.byref(n)))
+        return e, m
+
+    def Synchronize(self) -> None:
+        _check(lib().gpf_solver_synchronize(self._h))
+
+    @property
+    def fused(self) -> bool:
+        return bool(lib().gpf_solver_is_fused(self._h))
+
+    def EnableTiming(self, on: bool = True) -> None:
+        _check(lib().gpf_solver_enable_timing(self._h, 1 if on else 0))
+
+    def PassTimes(self) -> Dict[str, Tuple[float, int]]:
+        ms = np.zeros(16)
+        cnt = np.zeros(16, dtype=np.int64)
+        names = C.create_string_buffer(16 * 64)
+        np_ = lib().gpf_solver_pass_times(self._h, 16, _dptr(ms), cnt.ctypes.data_as(_lp), names, 64)
+        if np_ < 0:
+            _check(-np_)
+        out = {}
+        for i in range(np_):
+            nm = names.raw[i * 64:(i + 1) * 64].split(b"\0", 1)[0].decode()
+            out[nm] = (float(ms[i]), int(cnt[i]))
+        return out
+
+    def Stream(self) -> int:
+        return int(lib().gpf_solver_stream(self._h) or 0)
+
+
+# ---------------------------------------------------------------------------
+# pfutil.FFTWWrapper
+# ---------------------------------------------------------------------------
+
+class FourierTransform:
+    """FourierTransform implemented on the device (fftWrap.go:8-95)."""
+
+    def __init__(self, dims: Sequence[int], precision: str = "fp64", device: int = 0):
+        self.Dimensions = [int(d) for d in dims]
+        self.n = int(np.prod(self.Dimensions))
+        h = _vp()
+        prec = GPF_FP32 if precision in ("fp32", "float32") else GPF_FP64
+        _check(lib().gpf_fft_create(_ints(self.Dimensions), len(self.Dimensions), prec, int(device), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().gpf_fft_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def FFT(self, data: np.ndarray) -> np.ndarray:
+        """In place on a complex128 numpy array, like FFTWWrapper.FFT; also returns it."""
+        if data.dtype != np.complex128 or not data.flags.c_contiguous:
+            raise TypeError("data must be a contiguous complex128 array")
+        _check(lib().gpf_fft_forward(self._h, data.ctypes.data_as(_dp), data.size))
+        return data
+
+    def IFFT(self, data: np.ndarray) -> np.ndarray:
+        if data.dtype != np.complex128 or not data.flags.c_contiguous:
+            raise TypeError("data must be a contiguous complex128 array")
+        _check(lib().gpf_fft_inverse(self._h, data.ctypes.data_as(_dp), data.size))
+        return data
+
+    def Freq(self, i: int) -> List[float]:
+        return Freq(self.Dimensions, i)
+
+    def ConjugateNode(self, i: int) -> int:
+        return int(lib().gpf_fft_conjugate_node(_ints(self.Dimensions), len(self.Dimensions), int(i)))
+
+
+def Freq(dims: Sequence[int], i: int) -> List[float]:
+    """FFTWWrapper.Freq (fftWrap.go:57-74) without constructing a plan."""
+    out = (C.c_double * 3)()
+    _check(lib().gpf_fft_freq(_ints(dims), len(dims), int(i), out))
+    return [out[a] for a in range(len(dims))]
+
+
+def ConjugateNode(dims: Sequence[int], i: int) -> int:
+    return int(lib().gpf_fft_conjugate_node(_ints(dims), len(dims), int(i)))
+
+
+class Vandeven:
+    """pf.Vandeven (vandeven.go:8-40)."""
+
+    def __init__(self, order: int):
+        self.Data = np.zeros(1000)
+        _check(lib().gpf_vandeven_table(int(order), _dptr(self.Data)))
+        self.order = order
+
+    def Eval(self, x: float) -> float:
+        return float(lib().gpf_vandeven_eval(_dptr(self.Data), float(x)))
+
+
+def NewVandeven(order: int) -> Vandeven:
+    return Vandeven(order)
+
+
+def cahn_hilliard_model(n: int, M: float = 1.0, kappa: float = 1.0, data=None) -> Model:
+    """The Cahn-Hilliard model with f = c^2 (1 - c)^2 as gopf equation strings.
+
+    f'(c) = 2c - 6c^2 + 4c^3; dc/dt = M LAP f'(c) - M kappa LAP^2 c, written in the
+    form of examples/cahnHilliard/main.go:33 with scalars m2 = 2M, m6 = 6M, m4 = 4M, mk = M kappa.
+    """
+    m = Model()
+    m.AddField("conc", data if data is not None else None, n=n)
+    for name, v in (("m2", 2 * M), ("m6", 6 * M), ("m4", 4 * M), ("mk", M * kappa)):
+        m.AddScalar(name, v)
+    m.AddEquation("dconc/dt = m2*LAP conc - m6*LAP conc^2 + m4*LAP conc^3 - mk*LAP^2 conc")
+    m.Init()
+    return m
+
+
+CH_BULK_POLY = [0.0, 0.0, 1.0, -2.0, 1.0]   # f(c) = c^2 - 2 c^3 + c^4 = c^2 (1 - c)^2

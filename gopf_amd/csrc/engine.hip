@@ -1,0 +1,1594 @@
+// engine.hip -- device plans of gopf's spectral time step and the C ABI (include/gpf.h).
+//
+// Two stepping plans compile from a parsed pf.Model:
+//
+//  * fused single-field path (Euler, one field, explicit terms that are
+//    LAP^p of polynomials of the field, implicit LAP^p terms, optional
+//    Vandeven filter) -- the Cahn-Hilliard / Allen-Cahn class.  One step is
+//      2D: rows_fused (x) -> pencil_update (y)                          2 HBM passes
+//      3D: rows_fused (x) -> pencils fwd (y) -> pencil_update (z)
+//          -> pencils inv (y)                                           4 HBM passes
+//    The state between steps is the x-half-spectrum of the field scaled by
+//    1/nx, so the x-inverse of one step and the x-forward of the next share
+//    one pass.
+//
+//  * general path (any number of fields, Euler or RK4, ConcreteTerms on
+//    fields / derived fields / constants, SquaredGradient,
+//    HomogeneousModulusLinElast, Vandeven filter): derived fields in real
+//    space, R2C / C2R transforms, and one k-space update kernel per field in
+//    the reference's sequential order (euler.go:27-39, rk4.go:29-127).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/gpf.h"
+#include "fft_core.h"
+#include "model.h"
+#include "pointwise.h"
+#include "spectral.h"
+
+namespace gpf {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) throw Error(GPF_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static thread_local std::string g_last_error;
+
+template <typename T>
+class Buf {
+public:
+    Buf() = default;
+    explicit Buf(size_t count) { alloc(count); }
+    Buf(const Buf&) = delete;
+    Buf& operator=(const Buf&) = delete;
+    Buf(Buf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    Buf& operator=(Buf&& o) noexcept {
+        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        return *this;
+    }
+    ~Buf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&p_), count * sizeof(T)));
+        n_ = count;
+    }
+    T* get() const { return p_; }
+    size_t size() const { return n_; }
+    void swap(Buf& o) { std::swap(p_, o.p_); std::swap(n_, o.n_); }
+
+private:
+    void release() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// exp(-2 pi i q / n), q in [0, n), computed in extended precision
+template <typename T>
+static std::vector<cpx<T>> twiddle_host(int n) {
+    std::vector<cpx<T>> w(n);
+    for (int q = 0; q < n; ++q) {
+        long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)q / (long double)n;
+        w[q].x = (T)cosl(a);
+        w[q].y = (T)sinl(a);
+    }
+    return w;
+}
+
+// Vandeven table (pf/vandeven.go:13-27)
+static void vandeven_table(int order, double* data) {
+    data[0] = 1.0;
+    const double prefactor = std::tgamma((double)(2 * order)) / (std::tgamma((double)order) * std::tgamma((double)order));
+    const double dx = 1.0 / 999.0;
+    for (int i = 1; i < 1000; ++i) {
+        const double x = (double)i * dx;
+        const double i2 = std::pow(x * (1 - x), (double)(order - 1));
+        const double x1 = x - dx;
+        const double i1 = std::pow(x1 * (1.0 - x1), (double)(order - 1));
+        data[i] = data[i - 1] - prefactor * 0.5 * (i1 + i2) * dx;
+    }
+}
+
+static double energy_density(const double* C, const double* eps) {
+    double r = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k)
+                for (int l = 0; l < 3; ++l) r += C[i * 27 + j * 9 + k * 3 + l] * eps[i * 3 + j] * eps[k * 3 + l];
+    return 0.5 * r;
+}
+
+static bool real_coef(std::complex<double> c, double* out) {
+    *out = c.real();
+    return std::fabs(c.imag()) <= 1e-12 * std::max(1.0, std::fabs(c.real()));
+}
+
+// ---------------------------------------------------------------------------
+// geometry
+// ---------------------------------------------------------------------------
+struct Geometry {
+    int ndim = 2;
+    int dims[3] = {1, 1, 1};
+    int nx = 1, ny = 1, nz = 1, nxh = 1;
+    long n = 1, nmain = 0, nnyq = 0, nk = 0, rows = 0;
+    KGrid kg;
+
+    void init(const int* d, int nd) {
+        if (nd != 2 && nd != 3) throw Error(GPF_EINVAL, "domain must be 2D or 3D");
+        ndim = nd;
+        for (int a = 0; a < nd; ++a) {
+            if (d[a] < 2) throw Error(GPF_EINVAL, "domain extents must be >= 2");
+            dims[a] = d[a];
+        }
+        if (nd == 2) {
+            ny = d[0];
+            nx = d[1];
+            nz = 1;
+        } else {
+            // pfutil/fftWrap.go:42-74 indexes Freq with D1 as the fastest extent while FFTW
+            // plans n = [D0, D1, D2]; the two agree only for cubic domains.
+            if (d[0] != d[1] || d[1] != d[2])
+                throw Error(GPF_EINVAL, "3D domains must be cubic (the reference's Freq convention, fftWrap.go:42-74, "
+                                        "is only consistent for D0 = D1 = D2)");
+            nz = d[0];
+            ny = d[1];
+            nx = d[2];
+        }
+        if (nx % 2) throw Error(GPF_EINVAL, "the fastest extent must be even (real-to-half-spectrum layout)");
+        for (int v : {nx, ny, nz})
+            if (v > 1 && !fft_size_supported(v))
+                throw Error(GPF_EINVAL, "unsupported transform length " + std::to_string(v));
+        nxh = nx / 2;
+        n = (long)nx * ny * nz;
+        rows = (long)ny * nz;
+        nmain = rows * nxh;
+        nnyq = rows;
+        nk = nmain + nnyq;
+        kg.nx = nx;
+        kg.ny = ny;
+        kg.nz = nz;
+        kg.ndim = nd;
+    }
+};
+
+static int choose_tw(int n, int narr, size_t elem, int min_tw) {
+    const int T = fft_threads_per_line(n);
+    int tw = 256 / T;
+    if (tw < 1) tw = 1;
+    if (tw > 64) tw = 64;
+    while (tw < min_tw && tw * 2 * T <= 1024) tw *= 2;
+    while (tw * T > 1024 && tw > 1) tw /= 2;
+    while (tw > 1 && (size_t)narr * n * (tw + 1) * elem > 80 * 1024) tw /= 2;
+    return tw;
+}
+
+static Seg seg_y_main(const Geometry& g) {
+    Seg s;
+    s.na = g.nz; s.nb = g.nxh; s.sa = (long)g.ny * g.nxh; s.sb = 1; s.es = g.nxh;
+    s.ksrc[0] = 1; s.ksrc[1] = 2; s.ksrc[2] = 0;
+    return s;
+}
+static Seg seg_y_nyq(const Geometry& g) {
+    Seg s;
+    s.na = 1; s.nb = g.nz; s.sa = 0; s.sb = g.ny; s.es = 1;
+    s.ksrc[0] = 3; s.kfix[0] = g.nxh; s.ksrc[1] = 2; s.ksrc[2] = 1;
+    return s;
+}
+static Seg seg_z_main(const Geometry& g) {
+    Seg s;
+    s.na = g.ny; s.nb = g.nxh; s.sa = g.nxh; s.sb = 1; s.es = (long)g.ny * g.nxh;
+    s.ksrc[0] = 1; s.ksrc[1] = 0; s.ksrc[2] = 2;
+    return s;
+}
+static Seg seg_z_nyq(const Geometry& g) {
+    Seg s;
+    s.na = 1; s.nb = g.ny; s.sa = 0; s.sb = 1; s.es = g.ny;
+    s.ksrc[0] = 3; s.kfix[0] = g.nxh; s.ksrc[1] = 1; s.ksrc[2] = 2;
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// compiled equation of one field (general path)
+// ---------------------------------------------------------------------------
+struct TermRef {
+    int kind = kConcrete;
+    double coef = 1.0;
+    int pw = 0;
+    int brick = -1;  // concrete: brick index
+    int user = -1;
+};
+
+struct FieldEq {
+    std::vector<TermRef> terms;
+    KPoly denom;
+};
+
+// fused single-field plan
+struct FusedPlan {
+    bool ok = false;
+    std::string why;
+    std::vector<double> gpoly;
+    int src_lap = 0;
+    bool has_src = false;
+    KPoly denom, lin;
+};
+
+static void kpoly_add(KPoly& p, double c, int pw) {
+    for (int t = 0; t < p.n; ++t)
+        if (p.pow[t] == pw) { p.coef[t] += c; return; }
+    if (p.n >= kMaxKTerms) throw Error(GPF_EINVAL, "too many distinct LAP powers in one equation");
+    p.coef[p.n] = c;
+    p.pow[p.n] = pw;
+    ++p.n;
+}
+
+static FusedPlan compile_fused(const Model& m, int stepper) {
+    FusedPlan fp;
+    auto fail = [&](const std::string& w) { fp.ok = false; fp.why = w; return fp; };
+    if (m.field_names.size() != 1) return fail("more than one field");
+    if (stepper != GPF_EULER) return fail("stepper is not Euler");
+    const auto& terms = m.rhs[0].first;
+    const auto& denum = m.rhs[0].second;
+    std::vector<double> g(kMaxPoly, 0.0);
+    int src_lap = -1;
+    for (const auto& t : terms) {
+        if (t.kind != kConcrete) return fail("registered term");
+        double c;
+        if (!real_coef(t.coef, &c)) return fail("complex coefficient");
+        const int pw = t.has_lap ? t.lap : 0;
+        if (pw < 0) return fail("negative LAP power");
+        if (t.brick < 0) return fail("constant term");
+        if (t.brick == 0) {
+            kpoly_add(fp.lin, c, pw);
+            continue;
+        }
+        const DerivedDesc& d = m.derived[t.brick - 1];
+        if (src_lap >= 0 && src_lap != pw) return fail("explicit sources with different LAP powers");
+        src_lap = pw;
+        for (const auto& dt : d.terms) {
+            double power = 0.0;
+            for (const auto& f : dt.second) {
+                if (f.first != 0) return fail("derived field of another field");
+                power += f.second;
+            }
+            if (power != std::floor(power) || power < 0 || power >= kMaxPoly) return fail("non-integer power");
+            g[(int)power] += c * dt.first;
+        }
+    }
+    for (const auto& t : denum) {
+        if (t.kind != kConcrete || t.brick >= 0) return fail("field-dependent implicit term");
+        double c;
+        if (!real_coef(t.coef, &c)) return fail("complex coefficient");
+        kpoly_add(fp.denom, c, t.has_lap ? t.lap : 0);
+    }
+    fp.has_src = src_lap >= 0;
+    fp.src_lap = fp.has_src ? src_lap : 0;
+    int deg = 0;
+    for (int q = 0; q < kMaxPoly; ++q)
+        if (g[q] != 0.0) deg = q + 1;
+    fp.gpoly.assign(g.begin(), g.begin() + deg);
+    fp.ok = true;
+    return fp;
+}
+
+// ---------------------------------------------------------------------------
+// engine
+// ---------------------------------------------------------------------------
+class EngineBase {
+public:
+    virtual ~EngineBase() = default;
+    virtual void step() = 0;
+    virtual void on_step_finished() = 0;
+    virtual void get_field(int idx, double* out) = 0;
+    virtual void set_field(int idx, const double* in) = 0;
+    virtual void energy(int idx, int npoly, const double* poly, double kappa, double* e, double* mass) = 0;
+    virtual void monitor(int npoly, const double* poly, double kappa) = 0;
+    virtual long series(double* e, double* m, long cap) = 0;
+    virtual bool fused() const = 0;
+    virtual void enable_timing(bool on) = 0;
+    virtual int pass_times(int maxpass, double* ms, long* launches, char* names, int namelen) = 0;
+    virtual hipStream_t stream() const = 0;
+    virtual long size() const = 0;
+};
+
+template <typename T>
+class Engine : public EngineBase {
+public:
+    Engine(const Model& m, const int* dims, int ndim, double dt, const gpf_options& o);
+    ~Engine() override;
+
+    void step() override;
+    void on_step_finished() override;
+    void get_field(int idx, double* out) override;
+    void set_field(int idx, const double* in) override;
+    void energy(int idx, int npoly, const double* poly, double kappa, double* e, double* mass) override;
+    void monitor(int npoly, const double* poly, double kappa) override;
+    long series(double* e, double* m, long cap) override;
+    bool fused() const override { return fp_.ok; }
+    void enable_timing(bool on) override { timing_ = on; }
+    int pass_times(int maxpass, double* ms, long* launches, char* names, int namelen) override;
+    hipStream_t stream() const override { return st_; }
+    long size() const override { return g_.n; }
+
+private:
+    // transforms
+    const cpx<T>* tw(int n);
+    void pencils(const std::vector<cpx<T>*>& dst, const std::vector<const cpx<T>*>& src, int axis, int dir);
+    void r2c(const std::vector<const T*>& in, const std::vector<cpx<T>*>& out, T in_scale);
+    void c2r(const std::vector<const cpx<T>*>& in, const std::vector<T*>& out, T scale);
+    cpx<T>* kscratch(int i);
+    T* rscratch(int i);
+    // timing
+    void tbegin(int pass);
+    void tend(int pass);
+    // fused path
+    void fused_step();
+    // general path
+    void general_step_euler();
+    void general_step_rk4();
+    void ensure_k_fields();
+    void sync_derived();
+    void eval_user_terms_for(int eq);
+    void eval_elastic();
+    void rhs_update(int eq, cpx<T>* dst, const cpx<T>* cur, double alpha, double beta, double gamma, bool filt,
+                    const cpx<T>* extra);
+
+    Geometry g_;
+    hipStream_t st_ = nullptr;
+    double dt_;
+    int stepper_;
+    int nf_ = 0, nd_ = 0;
+    Model mcopy_;
+    FusedPlan fp_;
+    std::vector<FieldEq> eqs_;
+    Buf<double> filter_;
+    bool has_filter_ = false;
+    std::vector<std::pair<int, Buf<cpx<T>>>> tw_;
+    // general-path storage
+    std::vector<Buf<T>> F_, Dr_;
+    std::vector<Buf<cpx<T>>> K_, Tsq_, Tel_, INIT_, FINAL_, KF_;
+    std::vector<Buf<T>> lag_;
+    std::vector<DerivedArgs<T>> dargs_;
+    bool kvalid_ = false;
+    std::vector<Buf<cpx<T>>> ks_;
+    std::vector<Buf<T>> rs_;
+    // fused-path storage
+    Buf<cpx<T>> X_, Y_, G_;
+    Buf<double> mass_;
+    // monitors
+    bool mon_ = false;
+    std::vector<double> mon_poly_;
+    double mon_kappa_ = 0.0;
+    Buf<double> pa_bulk_, pa_mass_, pc_eg_, series_, dpoly_;
+    long nblk_a_ = 0, nblk_c_ = 0, series_cap_ = 0, series_n_ = 0;
+    // timing
+    bool timing_ = false;
+    std::vector<std::string> pass_names_;
+    std::vector<double> pass_ms_;
+    std::vector<long> pass_cnt_;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending_;
+    hipEvent_t cur_ev_ = nullptr;
+};
+
+template <typename T>
+Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gpf_options& o)
+    : dt_(dt), stepper_(o.stepper), mcopy_(m) {
+    g_.init(dims, ndim);
+    HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    nf_ = (int)m.field_names.size();
+    nd_ = (int)m.derived.size();
+    if (nf_ == 0) throw Error(GPF_EINVAL, "Model: No fields added");
+    if (nf_ > kMaxFields) throw Error(GPF_EINVAL, "too many fields");
+    if (m.rhs.size() != m.equations.size() || m.rhs.empty())
+        throw Error(GPF_ESTATE, "model is not initialised (call gpf_model_init)");
+    if ((int)m.rhs.size() < nf_)
+        throw Error(GPF_EINVAL, "every field needs an equation (RHS[i] updates Fields[i], euler.go:27-39)");
+    for (int i = 0; i < nf_; ++i)
+        if ((long)m.field_data[i].size() != g_.n)
+            throw Error(GPF_EINVAL, "solver: Inconsistent domain size and number of grid points");
+    if (o.filter_order > 0) {
+        std::vector<double> tab(1000);
+        vandeven_table(o.filter_order, tab.data());
+        filter_.alloc(1000);
+        HIPCHK(hipMemcpy(filter_.get(), tab.data(), 1000 * sizeof(double), hipMemcpyHostToDevice));
+        has_filter_ = true;
+    }
+    if (o.fused) fp_ = compile_fused(m, stepper_);
+    if (fp_.ok) {
+        X_.alloc(g_.nk);
+        Y_.alloc(g_.nk);
+        G_.alloc(g_.nk);
+        mass_.alloc(1);
+        pass_names_ = {"rows_fused", "pencils_fwd", "pencil_update", "pencils_inv"};
+    } else {
+        // compile every equation for the general path
+        for (int i = 0; i < nf_; ++i) {
+            FieldEq fe;
+            for (const auto& t : m.rhs[i].first) {
+                TermRef r;
+                r.kind = t.kind;
+                if (!real_coef(t.coef, &r.coef)) throw Error(GPF_EINVAL, "complex scalar coefficients are not supported (real fields)");
+                r.pw = (t.kind == kConcrete) ? (t.has_lap ? t.lap : 0) : t.lap;
+                if (r.pw < 0) throw Error(GPF_EINVAL, "negative LAP power");
+                r.brick = t.brick;
+                r.user = t.user;
+                fe.terms.push_back(r);
+            }
+            if (fe.terms.size() > (size_t)kMaxUTerms) throw Error(GPF_EINVAL, "too many terms in one equation");
+            for (const auto& t : m.rhs[i].second) {
+                if (t.kind != kConcrete || t.brick >= 0)
+                    throw Error(GPF_EINVAL, "implicit terms that depend on fields cannot be evaluated on the device "
+                                            "(only scalar * LAP^p terms are implicit)");
+                double c;
+                if (!real_coef(t.coef, &c)) throw Error(GPF_EINVAL, "complex scalar coefficients are not supported");
+                kpoly_add(fe.denom, c, t.has_lap ? t.lap : 0);
+            }
+            eqs_.push_back(fe);
+        }
+        for (int i = 0; i < nf_; ++i) F_.emplace_back(g_.n);
+        for (int d = 0; d < nd_; ++d) Dr_.emplace_back(g_.n);
+        for (int b = 0; b < nf_ + nd_; ++b) K_.emplace_back(g_.nk);
+        for (size_t s = 0; s < m.user_table[kSquaredGradient].size(); ++s) Tsq_.emplace_back(g_.nk);
+        for (size_t e = 0; e < m.user_table[kHomogeneousModulus].size(); ++e) {
+            Tel_.emplace_back(g_.nk);
+            lag_.emplace_back(g_.n);
+            HIPCHK(hipMemsetAsync(lag_.back().get(), 0, g_.n * sizeof(T), st_));
+            bool found = false;
+            for (int f = 0; f < nf_; ++f) found |= (m.field_names[f] == m.user_table[kHomogeneousModulus][e].field);
+            if (!found) throw Error(GPF_EINVAL, "HomogeneousModulusLinElast field is not a field");
+        }
+        for (const auto& s : m.user_table[kSquaredGradient]) {
+            if (!m.is_field_name(s.field)) throw Error(GPF_EINVAL, "SquaredGradient field is not a field");
+        }
+        if (stepper_ == GPF_RK4)
+            for (int i = 0; i < nf_; ++i) {
+                INIT_.emplace_back(g_.nk);
+                FINAL_.emplace_back(g_.nk);
+                KF_.emplace_back(g_.nk);
+            }
+        // derived-field programs
+        for (int d = 0; d < nd_; ++d) {
+            const DerivedDesc& dd = m.derived[d];
+            DerivedArgs<T> a;
+            a.nf = nf_;
+            a.n = g_.n;
+            a.out = Dr_[d].get();
+            if ((int)dd.terms.size() > kMaxDTerms) throw Error(GPF_EINVAL, "derived field with too many terms");
+            a.nterm = (int)dd.terms.size();
+            for (int t = 0; t < a.nterm; ++t) {
+                a.coef[t] = dd.terms[t].first;
+                for (int f = 0; f < kMaxFields; ++f) a.pw[t][f] = 0.0;
+                for (const auto& fac : dd.terms[t].second) a.pw[t][fac.first] += fac.second;
+            }
+            for (int f = 0; f < nf_; ++f) a.f[f] = F_[f].get();
+            dargs_.push_back(a);
+        }
+        pass_names_ = {"general_step"};
+    }
+    pass_ms_.assign(pass_names_.size(), 0.0);
+    pass_cnt_.assign(pass_names_.size(), 0);
+    for (int i = 0; i < nf_; ++i) {
+        std::vector<double> buf(2 * g_.n);
+        for (long j = 0; j < g_.n; ++j) {
+            buf[2 * j] = m.field_data[i][j].real();
+            buf[2 * j + 1] = m.field_data[i][j].imag();
+        }
+        set_field(i, buf.data());
+    }
+}
+
+template <typename T>
+Engine<T>::~Engine() {
+    if (st_) {
+        (void)hipStreamSynchronize(st_);
+        for (auto& p : pending_) {
+            (void)hipEventDestroy(p.second.first);
+            (void)hipEventDestroy(p.second.second);
+        }
+        (void)hipStreamDestroy(st_);
+    }
+}
+
+template <typename T>
+const cpx<T>* Engine<T>::tw(int n) {
+    for (auto& p : tw_)
+        if (p.first == n) return p.second.get();
+    auto h = twiddle_host<T>(n);
+    Buf<cpx<T>> b(n);
+    HIPCHK(hipMemcpy(b.get(), h.data(), n * sizeof(cpx<T>), hipMemcpyHostToDevice));
+    tw_.emplace_back(n, std::move(b));
+    return tw_.back().second.get();
+}
+
+template <typename T>
+cpx<T>* Engine<T>::kscratch(int i) {
+    while ((int)ks_.size() <= i) ks_.emplace_back(g_.nk);
+    return ks_[i].get();
+}
+
+template <typename T>
+T* Engine<T>::rscratch(int i) {
+    while ((int)rs_.size() <= i) rs_.emplace_back(g_.n);
+    return rs_[i].get();
+}
+
+template <typename T>
+void Engine<T>::tbegin(int pass) {
+    if (!timing_) return;
+    hipEvent_t a;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventRecord(a, st_));
+    cur_ev_ = a;
+    (void)pass;
+}
+
+template <typename T>
+void Engine<T>::tend(int pass) {
+    if (!timing_) return;
+    hipEvent_t b;
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(b, st_));
+    pending_.push_back({pass, {cur_ev_, b}});
+}
+
+template <typename T>
+int Engine<T>::pass_times(int maxpass, double* ms, long* launches, char* names, int namelen) {
+    HIPCHK(hipStreamSynchronize(st_));
+    for (auto& p : pending_) {
+        float e = 0.f;
+        HIPCHK(hipEventElapsedTime(&e, p.second.first, p.second.second));
+        pass_ms_[p.first] += e;
+        pass_cnt_[p.first] += 1;
+        (void)hipEventDestroy(p.second.first);
+        (void)hipEventDestroy(p.second.second);
+    }
+    pending_.clear();
+    const int np = (int)pass_names_.size();
+    for (int i = 0; i < np && i < maxpass; ++i) {
+        if (ms) ms[i] = pass_ms_[i];
+        if (launches) launches[i] = pass_cnt_[i];
+        if (names && namelen > 0) {
+            std::strncpy(names + (size_t)i * namelen, pass_names_[i].c_str(), namelen - 1);
+            names[(size_t)i * namelen + namelen - 1] = 0;
+        }
+        pass_ms_[i] = 0.0;
+        pass_cnt_[i] = 0;
+    }
+    return np;
+}
+
+template <typename T>
+void Engine<T>::pencils(const std::vector<cpx<T>*>& dst, const std::vector<const cpx<T>*>& src, int axis, int dir) {
+    const int n = (axis == 1) ? g_.ny : g_.nz;
+    for (size_t q0 = 0; q0 < dst.size(); q0 += kMaxArr) {
+        PencilArgs<T> a;
+        a.tw = tw(n);
+        a.narr = (int)std::min(dst.size() - q0, (size_t)kMaxArr);
+        a.seg[0] = (axis == 1) ? seg_y_main(g_) : seg_z_main(g_);
+        a.seg[1] = (axis == 1) ? seg_y_nyq(g_) : seg_z_nyq(g_);
+        a.nseg = 2;
+        a.TW = choose_tw(n, a.narr, sizeof(cpx<T>), 8);
+        for (int q = 0; q < a.narr; ++q) {
+            a.arr[q][0] = dst[q0 + q];
+            a.arr[q][1] = dst[q0 + q] + g_.nmain;
+            if (!src.empty() && src[q0 + q]) {
+                a.src[q][0] = src[q0 + q];
+                a.src[q][1] = src[q0 + q] + g_.nmain;
+            }
+        }
+        HIPCHK(launch_pencils<T>(a, n, dir, st_));
+    }
+}
+
+template <typename T>
+void Engine<T>::r2c(const std::vector<const T*>& in, const std::vector<cpx<T>*>& out, T in_scale) {
+    for (size_t q0 = 0; q0 < in.size(); q0 += kMaxArr) {
+        RowR2CArgs<T> a;
+        a.tw = tw(g_.nx);
+        a.narr = (int)std::min(in.size() - q0, (size_t)kMaxArr);
+        for (int q = 0; q < a.narr; ++q) {
+            a.in[q] = in[q0 + q];
+            a.main[q] = out[q0 + q];
+            a.nyq[q] = out[q0 + q] + g_.nmain;
+        }
+        a.rows = g_.rows;
+        a.nx = g_.nx;
+        a.TW = choose_tw(g_.nx, (a.narr + 1) / 2, sizeof(cpx<T>), 1);
+        a.in_scale = in_scale;
+        HIPCHK(launch_rows_r2c<T>(a, st_));
+    }
+    pencils(out, {}, 1, -1);
+    if (g_.ndim == 3) pencils(out, {}, 2, -1);
+}
+
+template <typename T>
+void Engine<T>::c2r(const std::vector<const cpx<T>*>& in, const std::vector<T*>& out, T scale) {
+    for (size_t q0 = 0; q0 < in.size(); q0 += kMaxArr) {
+        const int na = (int)std::min(in.size() - q0, (size_t)kMaxArr);
+        std::vector<cpx<T>*> scr;
+        std::vector<const cpx<T>*> src;
+        for (int q = 0; q < na; ++q) {
+            scr.push_back(kscratch(q));
+            src.push_back(in[q0 + q]);
+        }
+        if (g_.ndim == 3) {
+            pencils(scr, src, 2, +1);
+            pencils(scr, {}, 1, +1);
+        } else {
+            pencils(scr, src, 1, +1);
+        }
+        RowC2RArgs<T> a;
+        a.tw = tw(g_.nx);
+        a.narr = na;
+        for (int q = 0; q < na; ++q) {
+            a.main[q] = scr[q];
+            a.nyq[q] = scr[q] + g_.nmain;
+            a.out[q] = out[q0 + q];
+        }
+        a.rows = g_.rows;
+        a.nx = g_.nx;
+        a.TW = choose_tw(g_.nx, (na + 1) / 2, sizeof(cpx<T>), 1);
+        a.scale = scale;
+        HIPCHK(launch_rows_c2r<T>(a, st_));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// field I/O
+// ---------------------------------------------------------------------------
+template <typename T>
+void Engine<T>::set_field(int idx, const double* in) {
+    if (idx < 0 || idx >= nf_) throw Error(GPF_EINVAL, "field index out of range");
+    std::vector<T> h(g_.n);
+    double s = 0.0, c = 0.0;
+    for (long j = 0; j < g_.n; ++j) {
+        const double v = in[2 * j];
+        h[j] = (T)v;
+        const double t = s + v;
+        c += (std::fabs(s) >= std::fabs(v)) ? ((s - t) + v) : ((v - t) + s);
+        s = t;
+    }
+    if (fp_.ok) {
+        T* tmp = rscratch(0);
+        HIPCHK(hipMemcpyAsync(tmp, h.data(), g_.n * sizeof(T), hipMemcpyHostToDevice, st_));
+        RowR2CArgs<T> a;
+        a.tw = tw(g_.nx);
+        a.narr = 1;
+        a.in[0] = tmp;
+        a.main[0] = X_.get();
+        a.nyq[0] = X_.get() + g_.nmain;
+        a.rows = g_.rows;
+        a.nx = g_.nx;
+        a.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
+        a.in_scale = T(1) / (T)g_.nx;
+        HIPCHK(launch_rows_r2c<T>(a, st_));
+        const double mass = s + c;
+        HIPCHK(hipMemcpyAsync(mass_.get(), &mass, sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+    } else {
+        HIPCHK(hipMemcpyAsync(F_[idx].get(), h.data(), g_.n * sizeof(T), hipMemcpyHostToDevice, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+        kvalid_ = false;
+    }
+}
+
+template <typename T>
+void Engine<T>::get_field(int idx, double* out) {
+    if (idx < 0 || idx >= nf_) throw Error(GPF_EINVAL, "field index out of range");
+    std::vector<T> h(g_.n);
+    if (fp_.ok) {
+        T* tmp = rscratch(0);
+        RowC2RArgs<T> a;
+        a.tw = tw(g_.nx);
+        a.narr = 1;
+        a.main[0] = X_.get();
+        a.nyq[0] = X_.get() + g_.nmain;
+        a.out[0] = tmp;
+        a.rows = g_.rows;
+        a.nx = g_.nx;
+        a.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
+        a.scale = T(1);
+        HIPCHK(launch_rows_c2r<T>(a, st_));
+        HIPCHK(hipMemcpyAsync(h.data(), tmp, g_.n * sizeof(T), hipMemcpyDeviceToHost, st_));
+    } else {
+        HIPCHK(hipMemcpyAsync(h.data(), F_[idx].get(), g_.n * sizeof(T), hipMemcpyDeviceToHost, st_));
+    }
+    HIPCHK(hipStreamSynchronize(st_));
+    for (long j = 0; j < g_.n; ++j) {
+        out[2 * j] = (double)h[j];
+        out[2 * j + 1] = 0.0;
+    }
+}
+
+template <typename T>
+void Engine<T>::energy(int idx, int npoly, const double* poly, double kappa, double* e, double* mass) {
+    if (idx < 0 || idx >= nf_) throw Error(GPF_EINVAL, "field index out of range");
+    if (npoly > kMaxPoly || npoly < 0) throw Error(GPF_EINVAL, "energy polynomial too long");
+    const T* c;
+    if (fp_.ok) {
+        T* tmp = rscratch(0);
+        RowC2RArgs<T> a;
+        a.tw = tw(g_.nx);
+        a.narr = 1;
+        a.main[0] = X_.get();
+        a.nyq[0] = X_.get() + g_.nmain;
+        a.out[0] = tmp;
+        a.rows = g_.rows;
+        a.nx = g_.nx;
+        a.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
+        a.scale = T(1);
+        HIPCHK(launch_rows_c2r<T>(a, st_));
+        c = tmp;
+    } else {
+        c = F_[idx].get();
+    }
+    const long nb = 512;
+    Buf<double> pf(nb), px(nb), pg(nb), res(2), dp(kMaxPoly);
+    std::vector<double> hp(kMaxPoly, 0.0);
+    for (int q = 0; q < npoly; ++q) hp[q] = poly[q];
+    HIPCHK(hipMemcpyAsync(dp.get(), hp.data(), kMaxPoly * sizeof(double), hipMemcpyHostToDevice, st_));
+    HIPCHK(launch_real_sums<T>(c, g_.n, npoly, dp.get(), pf.get(), px.get(), nb, st_));
+    cpx<T>* kc = kscratch(kMaxArr);
+    r2c({c}, {kc}, T(1));
+    HIPCHK(launch_kgrad_energy<T>(kc, g_.kg, g_.nmain, g_.nk, pg.get(), nb, st_));
+    HIPCHK(launch_reduce2(pf.get(), nb, 1.0, pg.get(), nb, 0.5 * kappa / (double)g_.n, res.get(), st_));
+    HIPCHK(launch_reduce2(px.get(), nb, 1.0, px.get(), 0, 0.0, res.get() + 1, st_));
+    double h[2];
+    HIPCHK(hipMemcpyAsync(h, res.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+    *e = h[0];
+    *mass = h[1];
+}
+
+template <typename T>
+void Engine<T>::monitor(int npoly, const double* poly, double kappa) {
+    if (!fp_.ok) throw Error(GPF_EINVAL, "the per-step energy series is recorded by the fused path only");
+    if (npoly > kMaxPoly || npoly < 0) throw Error(GPF_EINVAL, "energy polynomial too long");
+    mon_ = true;
+    mon_poly_.assign(poly, poly + npoly);
+    mon_kappa_ = kappa;
+    series_n_ = 0;
+}
+
+template <typename T>
+long Engine<T>::series(double* e, double* m, long cap) {
+    HIPCHK(hipStreamSynchronize(st_));
+    const long n = std::min(cap, series_n_);
+    if (n > 0) {
+        std::vector<double> h(2 * series_n_);
+        HIPCHK(hipMemcpy(h.data(), series_.get(), 2 * series_n_ * sizeof(double), hipMemcpyDeviceToHost));
+        for (long i = 0; i < n; ++i) {
+            if (e) e[i] = h[2 * i];
+            if (m) m[i] = h[2 * i + 1];
+        }
+    }
+    return series_n_;
+}
+
+// ---------------------------------------------------------------------------
+// fused single-field step
+// ---------------------------------------------------------------------------
+template <typename T>
+void Engine<T>::fused_step() {
+    RowFusedArgs<T> ra;
+    ra.tw = tw(g_.nx);
+    ra.s_main = X_.get();
+    ra.s_nyq = X_.get() + g_.nmain;
+    ra.c_main = Y_.get();
+    ra.c_nyq = Y_.get() + g_.nmain;
+    ra.g_main = G_.get();
+    ra.g_nyq = G_.get() + g_.nmain;
+    ra.rows = g_.rows;
+    ra.nx = g_.nx;
+    ra.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
+    ra.npoly = (int)fp_.gpoly.size();
+    for (int q = 0; q < ra.npoly; ++q) ra.gpoly[q] = fp_.gpoly[q];
+    const long nblk_a = rows_blocks(g_.rows, ra.TW);
+    PencilUpdateArgs<T> u;
+    const int axis = (g_.ndim == 3) ? 2 : 1;
+    const int nu = (axis == 2) ? g_.nz : g_.ny;
+    u.p.tw = tw(nu);
+    u.p.narr = fp_.has_src ? 2 : 1;
+    u.p.arr[0][0] = Y_.get();
+    u.p.arr[0][1] = Y_.get() + g_.nmain;
+    u.p.arr[1][0] = G_.get();
+    u.p.arr[1][1] = G_.get() + g_.nmain;
+    u.p.seg[0] = (axis == 2) ? seg_z_main(g_) : seg_y_main(g_);
+    u.p.seg[1] = (axis == 2) ? seg_z_nyq(g_) : seg_y_nyq(g_);
+    u.p.nseg = 2;
+    u.p.TW = choose_tw(nu, u.p.narr, sizeof(cpx<T>), 8);
+    u.g = g_.kg;
+    u.dt = dt_;
+    u.scale = 1.0 / (double)g_.n;
+    u.denom = fp_.denom;
+    u.lin = fp_.lin;
+    u.src_lap = fp_.src_lap;
+    u.has_src = fp_.has_src ? 1 : 0;
+    u.filter = has_filter_ ? filter_.get() : nullptr;
+    u.mass = (sizeof(T) == 4) ? mass_.get() : nullptr;
+    int tiles0 = 0;
+    const long nblk_c = pencil_blocks(u.p.seg, u.p.nseg, u.p.TW, &tiles0);
+    if (mon_) {
+        if (nblk_a_ != nblk_a) { pa_bulk_.alloc(nblk_a); pa_mass_.alloc(nblk_a); nblk_a_ = nblk_a; }
+        if (nblk_c_ != nblk_c) { pc_eg_.alloc(nblk_c); nblk_c_ = nblk_c; }
+        if (series_n_ >= series_cap_) {
+            const long cap = std::max(1024L, 2 * series_cap_);
+            Buf<double> nb(2 * cap);
+            if (series_n_ > 0)
+                HIPCHK(hipMemcpyAsync(nb.get(), series_.get(), 2 * series_n_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
+            series_.swap(nb);
+            HIPCHK(hipStreamSynchronize(st_));
+            series_cap_ = cap;
+        }
+        ra.nbulk = (int)mon_poly_.size();
+        for (int q = 0; q < ra.nbulk; ++q) ra.bulk[q] = mon_poly_[q];
+        ra.bulk_partial = pa_bulk_.get();
+        ra.mass_partial = pa_mass_.get();
+        u.egrad_partial = pc_eg_.get();
+    }
+    tbegin(0);
+    HIPCHK(launch_rows_fused<T>(ra, st_));
+    tend(0);
+    if (g_.ndim == 3) {
+        tbegin(1);
+        if (fp_.has_src) pencils({Y_.get(), G_.get()}, {}, 1, -1);
+        else pencils({Y_.get()}, {}, 1, -1);
+        tend(1);
+    }
+    tbegin(2);
+    HIPCHK(launch_pencil_update<T>(u, nu, st_));
+    tend(2);
+    if (g_.ndim == 3) {
+        tbegin(3);
+        pencils({Y_.get()}, {}, 1, +1);
+        tend(3);
+    }
+    if (mon_) {
+        HIPCHK(launch_reduce2(pa_bulk_.get(), nblk_a, 1.0, pc_eg_.get(), nblk_c, 0.5 * mon_kappa_ / (double)g_.n,
+                              series_.get() + 2 * series_n_, st_));
+        HIPCHK(launch_reduce2(pa_mass_.get(), nblk_a, 1.0, pa_mass_.get(), 0, 0.0, series_.get() + 2 * series_n_ + 1, st_));
+        ++series_n_;
+    }
+    X_.swap(Y_);
+}
+
+// ---------------------------------------------------------------------------
+// general path
+// ---------------------------------------------------------------------------
+template <typename T>
+void Engine<T>::ensure_k_fields() {
+    if (kvalid_) return;
+    std::vector<const T*> in;
+    std::vector<cpx<T>*> out;
+    for (int i = 0; i < nf_; ++i) {
+        in.push_back(F_[i].get());
+        out.push_back(K_[i].get());
+    }
+    r2c(in, out, T(1));
+    kvalid_ = true;
+}
+
+template <typename T>
+void Engine<T>::sync_derived() {
+    if (nd_ == 0) return;
+    std::vector<const T*> in;
+    std::vector<cpx<T>*> out;
+    for (int d = 0; d < nd_; ++d) {
+        HIPCHK(launch_derived<T>(dargs_[d], st_));
+        in.push_back(Dr_[d].get());
+        out.push_back(K_[nf_ + d].get());
+    }
+    r2c(in, out, T(1));
+}
+
+// SquaredGradient terms of equation eq, evaluated on the current k-space bricks
+template <typename T>
+void Engine<T>::eval_user_terms_for(int eq) {
+    for (const auto& t : eqs_[eq].terms) {
+        if (t.kind != kSquaredGradient) continue;
+        const UserTerm& ut = mcopy_.user_table[kSquaredGradient][t.user];
+        const int b = mcopy_.brick_index(ut.field);
+        const int nd = g_.ndim;
+        KGradArgs<T> a;
+        a.src = K_[b].get();
+        for (int d = 0; d < nd; ++d) a.out[d] = kscratch(kMaxArr + 1 + d);
+        a.g = g_.kg;
+        a.nmain = g_.nmain;
+        a.nk = g_.nk;
+        HIPCHK(launch_kgrad<T>(a, nd, st_));
+        std::vector<const cpx<T>*> in;
+        std::vector<T*> out;
+        for (int d = 0; d < nd; ++d) {
+            in.push_back(a.out[d]);
+            out.push_back(rscratch(1 + d));
+        }
+        c2r(in, out, T(1) / (T)g_.n);
+        const T* w[3] = {out[0], out[nd > 1 ? 1 : 0], out[nd > 2 ? 2 : 0]};
+        T* sq = rscratch(4);
+        HIPCHK(launch_sqsum<T>(w, nd, (T)ut.factor, sq, g_.n, st_));
+        r2c({sq}, {Tsq_[t.user].get()}, T(1));
+    }
+}
+
+// HomogeneousModulusLinElast: FFT(H'(phi) (IFFT(K H^) / N - 2 e* H(phi))), once per step
+template <typename T>
+void Engine<T>::eval_elastic() {
+    for (size_t e = 0; e < Tel_.size(); ++e) {
+        const UserTerm& ut = mcopy_.user_table[kHomogeneousModulus][e];
+        T* H = rscratch(1);
+        HIPCHK(launch_indicator<T>(lag_[e].get(), H, g_.n, st_));
+        cpx<T>* Hk = kscratch(kMaxArr + 1);
+        r2c({H}, {Hk}, T(1));
+        KElasticArgs<T> a;
+        a.src = Hk;
+        a.dst = Hk;
+        std::memcpy(a.ep.C, ut.C, sizeof(a.ep.C));
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s = 0.0;
+                for (int k = 0; k < 3; ++k)
+                    for (int l = 0; l < 3; ++l) s = s + ut.C[i * 27 + j * 9 + k * 3 + l] * ut.misfit[k * 3 + l];
+                a.ep.sig[i * 3 + j] = s;
+            }
+        a.ep.dim = g_.ndim;
+        a.g = g_.kg;
+        a.nmain = g_.nmain;
+        a.nk = g_.nk;
+        HIPCHK(launch_kelastic<T>(a, st_));
+        T* S = rscratch(2);
+        c2r({Hk}, {S}, T(1) / (T)g_.n);
+        T* P = rscratch(3);
+        HIPCHK(launch_elastic_real<T>(lag_[e].get(), S, 2.0 * energy_density(ut.C, ut.misfit), P, g_.n, st_));
+        r2c({P}, {Tel_[e].get()}, T(1));
+    }
+}
+
+// dst = filt * (alpha * cur + beta * (sum_t terms_t [+ extra])) / (1 - gamma * D)
+template <typename T>
+void Engine<T>::rhs_update(int eq, cpx<T>* dst, const cpx<T>* cur, double alpha, double beta, double gamma,
+                           bool filt, const cpx<T>* extra) {
+    KUpdateArgs<T> a;
+    a.dst = dst;
+    a.cur = cur;
+    a.g = g_.kg;
+    a.nmain = g_.nmain;
+    a.nk = g_.nk;
+    a.alpha = alpha;
+    a.beta = beta;
+    a.gamma = gamma;
+    a.denom = eqs_[eq].denom;
+    a.filter = (filt && has_filter_) ? filter_.get() : nullptr;
+    if (extra) {
+        a.brick[0] = extra;
+        a.coef[0] = 1.0;
+        a.pw[0] = 0;
+        a.nterm = 1;
+    } else {
+        for (const auto& t : eqs_[eq].terms) {
+            const int q = a.nterm++;
+            a.coef[q] = t.coef;
+            a.pw[q] = t.pw;
+            if (t.kind == kConcrete) a.brick[q] = (t.brick >= 0) ? K_[t.brick].get() : nullptr;
+            else if (t.kind == kSquaredGradient) a.brick[q] = Tsq_[t.user].get();
+            else a.brick[q] = Tel_[t.user].get();
+        }
+    }
+    HIPCHK(launch_kupdate<T>(a, st_));
+}
+
+template <typename T>
+void Engine<T>::general_step_euler() {
+    ensure_k_fields();
+    sync_derived();
+    eval_elastic();
+    for (int i = 0; i < nf_; ++i) {
+        eval_user_terms_for(i);
+        rhs_update(i, K_[i].get(), K_[i].get(), 1.0, dt_, dt_, true, nullptr);
+    }
+    std::vector<const cpx<T>*> in;
+    std::vector<T*> out;
+    for (int i = 0; i < nf_; ++i) {
+        in.push_back(K_[i].get());
+        out.push_back(F_[i].get());
+    }
+    c2r(in, out, T(1) / (T)g_.n);
+}
+
+template <typename T>
+void Engine<T>::general_step_rk4() {
+    ensure_k_fields();
+    const size_t kb = g_.nk * sizeof(cpx<T>);
+    for (int i = 0; i < nf_; ++i) {
+        HIPCHK(hipMemcpyAsync(INIT_[i].get(), K_[i].get(), kb, hipMemcpyDeviceToDevice, st_));
+        HIPCHK(hipMemcpyAsync(FINAL_[i].get(), K_[i].get(), kb, hipMemcpyDeviceToDevice, st_));
+    }
+    sync_derived();
+    eval_elastic();  // the lagged field is constant within a step (homoLinElast.go:130-134)
+    for (int i = 0; i < nf_; ++i) {
+        eval_user_terms_for(i);
+        rhs_update(i, KF_[i].get(), nullptr, 0.0, 1.0, 0.0, false, nullptr);
+    }
+    const double fac_prep[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    const double fac_corr[3] = {0.5, 0.5, 1.0};
+    for (int stage = 0; stage < 4; ++stage) {
+        for (int i = 0; i < nf_; ++i) {
+            // final += fac dt kFactor; field = initial  (rk4.go:77-84)
+            KUpdateArgs<T> a;
+            a.dst = FINAL_[i].get();
+            a.cur = FINAL_[i].get();
+            a.brick[0] = KF_[i].get();
+            a.coef[0] = 1.0;
+            a.pw[0] = 0;
+            a.nterm = 1;
+            a.alpha = 1.0;
+            a.beta = fac_prep[stage] * dt_;
+            a.gamma = 0.0;
+            a.g = g_.kg;
+            a.nmain = g_.nmain;
+            a.nk = g_.nk;
+            HIPCHK(launch_kupdate<T>(a, st_));
+            HIPCHK(hipMemcpyAsync(K_[i].get(), INIT_[i].get(), kb, hipMemcpyDeviceToDevice, st_));
+        }
+        if (stage == 3) break;
+        // correction(factor) (rk4.go:101-127)
+        const double f = fac_corr[stage];
+        std::vector<const cpx<T>*> in;
+        std::vector<T*> out;
+        for (int i = 0; i < nf_; ++i) {
+            rhs_update(i, K_[i].get(), K_[i].get(), 1.0, f * dt_, f * dt_, false, KF_[i].get());
+            in.push_back(K_[i].get());
+            out.push_back(F_[i].get());
+        }
+        c2r(in, out, T(1) / (T)g_.n);
+        sync_derived();
+        for (int i = 0; i < nf_; ++i) {
+            eval_user_terms_for(i);
+            rhs_update(i, KF_[i].get(), nullptr, 0.0, 1.0, 0.0, false, nullptr);
+        }
+    }
+    std::vector<const cpx<T>*> in;
+    std::vector<T*> out;
+    for (int i = 0; i < nf_; ++i) {
+        // final / (1 - dt D), filter (rk4.go:57-67)
+        KUpdateArgs<T> a;
+        a.dst = K_[i].get();
+        a.cur = FINAL_[i].get();
+        a.alpha = 1.0;
+        a.beta = 0.0;
+        a.gamma = dt_;
+        a.denom = eqs_[i].denom;
+        a.filter = has_filter_ ? filter_.get() : nullptr;
+        a.g = g_.kg;
+        a.nmain = g_.nmain;
+        a.nk = g_.nk;
+        HIPCHK(launch_kupdate<T>(a, st_));
+        in.push_back(K_[i].get());
+        out.push_back(F_[i].get());
+    }
+    c2r(in, out, T(1) / (T)g_.n);
+}
+
+template <typename T>
+void Engine<T>::step() {
+    if (fp_.ok) {
+        fused_step();
+        return;
+    }
+    tbegin(0);
+    if (stepper_ == GPF_EULER) general_step_euler();
+    else general_step_rk4();
+    tend(0);
+}
+
+// OnStepFinished of every registered term (solver.go:73-83): only
+// HomogeneousModulusLinElast keeps state, the real part of its field (homoLinElast.go:130-134)
+template <typename T>
+void Engine<T>::on_step_finished() {
+    if (fp_.ok) return;
+    for (size_t e = 0; e < lag_.size(); ++e) {
+        const UserTerm& ut = mcopy_.user_table[kHomogeneousModulus][e];
+        int f = 0;
+        for (int i = 0; i < nf_; ++i)
+            if (mcopy_.field_names[i] == ut.field) f = i;
+        HIPCHK(launch_copy<T>(F_[f].get(), lag_[e].get(), g_.n, st_));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// full complex FourierTransform (pfutil.FFTWWrapper)
+// ---------------------------------------------------------------------------
+class FFTBase {
+public:
+    virtual ~FFTBase() = default;
+    virtual void exec(double* data, int dir) = 0;
+};
+
+template <typename T>
+class FullFFT : public FFTBase {
+public:
+    FullFFT(const int* dims, int ndim) : ndim_(ndim) {
+        if (ndim < 1 || ndim > 3) throw Error(GPF_EINVAL, "FourierTransform needs 1 to 3 dimensions");
+        nx_ = dims[ndim - 1];
+        ny_ = (ndim >= 2) ? dims[ndim - 2] : 1;
+        nz_ = (ndim == 3) ? dims[0] : 1;
+        for (int v : {nx_, ny_, nz_})
+            if (v > 1 && !fft_size_supported(v)) throw Error(GPF_EINVAL, "unsupported transform length " + std::to_string(v));
+        n_ = (long)nx_ * ny_ * nz_;
+        HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+        d_.alloc(n_);
+        for (int v : {nx_, ny_, nz_}) {
+            if (v <= 1) continue;
+            bool have = false;
+            for (auto& p : tw_) have |= (p.first == v);
+            if (have) continue;
+            auto h = twiddle_host<T>(v);
+            Buf<cpx<T>> b(v);
+            HIPCHK(hipMemcpy(b.get(), h.data(), v * sizeof(cpx<T>), hipMemcpyHostToDevice));
+            tw_.emplace_back(v, std::move(b));
+        }
+    }
+    ~FullFFT() override {
+        if (st_) (void)hipStreamDestroy(st_);
+    }
+    void exec(double* data, int dir) override {
+        std::vector<cpx<T>> h(n_);
+        for (long i = 0; i < n_; ++i) {
+            h[i].x = (T)data[2 * i];
+            h[i].y = (T)data[2 * i + 1];
+        }
+        HIPCHK(hipMemcpyAsync(d_.get(), h.data(), n_ * sizeof(cpx<T>), hipMemcpyHostToDevice, st_));
+        if (nx_ > 1) {
+            Seg s;
+            s.na = 1; s.nb = (int)(ny_ * nz_); s.sb = nx_; s.es = 1;
+            pass(s, nx_, dir);
+        }
+        if (ny_ > 1) {
+            Seg s;
+            s.na = nz_; s.nb = nx_; s.sa = (long)ny_ * nx_; s.sb = 1; s.es = nx_;
+            pass(s, ny_, dir);
+        }
+        if (nz_ > 1) {
+            Seg s;
+            s.na = ny_; s.nb = nx_; s.sa = nx_; s.sb = 1; s.es = (long)nx_ * ny_;
+            pass(s, nz_, dir);
+        }
+        HIPCHK(hipMemcpyAsync(h.data(), d_.get(), n_ * sizeof(cpx<T>), hipMemcpyDeviceToHost, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+        for (long i = 0; i < n_; ++i) {
+            data[2 * i] = (double)h[i].x;
+            data[2 * i + 1] = (double)h[i].y;
+        }
+    }
+
+private:
+    void pass(const Seg& s, int n, int dir) {
+        PencilArgs<T> a;
+        for (auto& p : tw_)
+            if (p.first == n) a.tw = p.second.get();
+        a.narr = 1;
+        a.arr[0][0] = d_.get();
+        a.seg[0] = s;
+        a.nseg = 1;
+        a.TW = choose_tw(n, 1, sizeof(cpx<T>), s.sb == 1 ? 8 : 1);
+        HIPCHK(launch_pencils<T>(a, n, dir, st_));
+    }
+    int ndim_;
+    int nx_, ny_, nz_;
+    long n_;
+    hipStream_t st_ = nullptr;
+    Buf<cpx<T>> d_;
+    std::vector<std::pair<int, Buf<cpx<T>>>> tw_;
+};
+
+}  // namespace gpf
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace gpf;
+
+struct gpf_model {
+    Model m;
+    bool initialised = false;
+};
+
+struct gpf_solver {
+    std::unique_ptr<EngineBase> eng;
+    int stepper = GPF_EULER;
+    double dt = 0.0;
+    long current_step = 0;
+    int device = 0;
+};
+
+struct gpf_fft {
+    std::unique_ptr<FFTBase> impl;
+    int device = 0;
+};
+
+template <typename F>
+static int guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return GPF_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const ParseError& e) {
+        g_last_error = e.what();
+        return GPF_EPARSE;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return GPF_EINVAL;
+    }
+}
+
+static void set_device(int dev) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) throw Error(GPF_EDEVICE, "no HIP device available");
+    if (dev < 0 || dev >= n) throw Error(GPF_EINVAL, "device ordinal out of range");
+    HIPCHK(hipSetDevice(dev));
+}
+
+extern "C" {
+
+const char* gpf_last_error(void) { return g_last_error.c_str(); }
+
+int gpf_model_create(gpf_model** out) {
+    return guarded([&] {
+        if (!out) throw Error(GPF_EINVAL, "null output");
+        *out = new gpf_model();
+    });
+}
+
+void gpf_model_destroy(gpf_model* m) { delete m; }
+
+int gpf_model_add_field(gpf_model* m, const char* name, const double* data, long n) {
+    return guarded([&] {
+        if (!m || !name || n <= 0) throw Error(GPF_EINVAL, "bad argument");
+        std::vector<std::complex<double>> v(n);
+        if (data)
+            for (long i = 0; i < n; ++i) v[i] = {data[2 * i], data[2 * i + 1]};
+        m->m.add_field(name, std::move(v));
+    });
+}
+
+int gpf_model_add_scalar(gpf_model* m, const char* name, double re, double im) {
+    return guarded([&] {
+        if (!m || !name) throw Error(GPF_EINVAL, "bad argument");
+        m->m.add_scalar(name, {re, im});
+    });
+}
+
+int gpf_model_add_equation(gpf_model* m, const char* eq) {
+    return guarded([&] {
+        if (!m || !eq) throw Error(GPF_EINVAL, "bad argument");
+        m->m.add_equation(eq);
+    });
+}
+
+int gpf_model_register_square_gradient(gpf_model* m, const char* term, const char* field, double factor,
+                                       int implicit) {
+    return guarded([&] {
+        if (!m || !term || !field) throw Error(GPF_EINVAL, "bad argument");
+        UserTerm u;
+        u.kind = kSquaredGradient;
+        u.field = field;
+        u.factor = factor;
+        if (implicit) m->m.register_implicit_term(term, u);
+        else m->m.register_explicit_term(term, u);
+    });
+}
+
+int gpf_model_register_homogeneous_modulus(gpf_model* m, const char* term, const char* field, const double* C,
+                                           const double* misfit) {
+    return guarded([&] {
+        if (!m || !term || !field || !C || !misfit) throw Error(GPF_EINVAL, "bad argument");
+        UserTerm u;
+        u.kind = kHomogeneousModulus;
+        u.field = field;
+        std::memcpy(u.C, C, sizeof(u.C));
+        std::memcpy(u.misfit, misfit, sizeof(u.misfit));
+        m->m.register_explicit_term(term, u);
+    });
+}
+
+int gpf_model_register_polynomial(gpf_model* m, const char* name, int nterms, const double* coefs,
+                                  const double* powers, int nfields) {
+    return guarded([&] {
+        if (!m || !name || nterms < 0 || (nterms && (!coefs || !powers))) throw Error(GPF_EINVAL, "bad argument");
+        if (nfields != (int)m->m.field_names.size())
+            throw Error(GPF_EINVAL, "powers must list one exponent per field added so far");
+        std::vector<std::pair<double, std::vector<double>>> t;
+        for (int i = 0; i < nterms; ++i)
+            t.push_back({coefs[i], std::vector<double>(powers + (size_t)i * nfields, powers + (size_t)(i + 1) * nfields)});
+        m->m.register_polynomial(name, t);
+    });
+}
+
+int gpf_model_init(gpf_model* m) {
+    return guarded([&] {
+        if (!m) throw Error(GPF_EINVAL, "bad argument");
+        m->m.init();
+        m->initialised = true;
+    });
+}
+
+int gpf_model_num_bricks(const gpf_model* m, int* nfields, int* nderived) {
+    return guarded([&] {
+        if (!m) throw Error(GPF_EINVAL, "bad argument");
+        if (nfields) *nfields = (int)m->m.field_names.size();
+        if (nderived) *nderived = (int)m->m.derived.size();
+    });
+}
+
+int gpf_model_brick_name(const gpf_model* m, int idx, char* buf, int buflen) {
+    return guarded([&] {
+        if (!m || !buf || buflen <= 0) throw Error(GPF_EINVAL, "bad argument");
+        auto names = m->m.all_field_names();
+        if (idx < 0 || idx >= (int)names.size()) throw Error(GPF_EINVAL, "brick index out of range");
+        std::strncpy(buf, names[idx].c_str(), buflen - 1);
+        buf[buflen - 1] = 0;
+    });
+}
+
+int gpf_model_num_terms(const gpf_model* m, int eq, int* nterms, int* ndenum) {
+    return guarded([&] {
+        if (!m || !m->initialised) throw Error(GPF_ESTATE, "model is not initialised");
+        if (eq < 0 || eq >= (int)m->m.rhs.size()) throw Error(GPF_EINVAL, "equation index out of range");
+        if (nterms) *nterms = (int)m->m.rhs[eq].first.size();
+        if (ndenum) *ndenum = (int)m->m.rhs[eq].second.size();
+    });
+}
+
+int gpf_model_get_term(const gpf_model* m, int eq, int denum, int idx, gpf_term* out) {
+    return guarded([&] {
+        if (!m || !m->initialised || !out) throw Error(GPF_ESTATE, "model is not initialised");
+        if (eq < 0 || eq >= (int)m->m.rhs.size()) throw Error(GPF_EINVAL, "equation index out of range");
+        const auto& L = denum ? m->m.rhs[eq].second : m->m.rhs[eq].first;
+        if (idx < 0 || idx >= (int)L.size()) throw Error(GPF_EINVAL, "term index out of range");
+        const TermDesc& t = L[idx];
+        out->kind = t.kind;
+        out->coef_re = t.coef.real();
+        out->coef_im = t.coef.imag();
+        out->brick = t.brick;
+        out->has_lap = t.has_lap ? 1 : 0;
+        out->lap = t.lap;
+        out->user = t.user;
+    });
+}
+
+void gpf_options_default(gpf_options* o) {
+    if (!o) return;
+    o->precision = GPF_FP64;
+    o->stepper = GPF_EULER;
+    o->filter_order = 0;
+    o->fused = 1;
+    o->device = 0;
+}
+
+int gpf_solver_create(gpf_model* m, const int* dims, int ndim, double dt, const gpf_options* o, gpf_solver** out) {
+    return guarded([&] {
+        if (!m || !dims || !out) throw Error(GPF_EINVAL, "bad argument");
+        if (!m->initialised) throw Error(GPF_ESTATE, "model is not initialised (call gpf_model_init)");
+        gpf_options opt;
+        gpf_options_default(&opt);
+        if (o) opt = *o;
+        if (opt.stepper != GPF_EULER && opt.stepper != GPF_RK4) throw Error(GPF_EINVAL, "Unknown stepper scheme");
+        set_device(opt.device);
+        auto s = std::make_unique<gpf_solver>();
+        s->stepper = opt.stepper;
+        s->dt = dt;
+        s->device = opt.device;
+        if (opt.precision == GPF_FP32) s->eng.reset(new Engine<float>(m->m, dims, ndim, dt, opt));
+        else if (opt.precision == GPF_FP64) s->eng.reset(new Engine<double>(m->m, dims, ndim, dt, opt));
+        else throw Error(GPF_EINVAL, "unknown precision");
+        *out = s.release();
+    });
+}
+
+void gpf_solver_destroy(gpf_solver* s) {
+    if (!s) return;
+    try {
+        set_device(s->device);
+    } catch (...) {
+    }
+    delete s;
+}
+
+int gpf_solver_propagate(gpf_solver* s, int nsteps) {
+    return guarded([&] {
+        if (!s || nsteps < 0) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        for (int i = 0; i < nsteps; ++i) {
+            s->eng->step();
+            if (s->stepper == GPF_EULER) s->current_step++;  // Euler.Step increments (euler.go:46)
+            s->eng->on_step_finished();
+        }
+    });
+}
+
+int gpf_solver_stepper_propagate(gpf_solver* s, int nsteps) {
+    return guarded([&] {
+        if (!s || nsteps < 0) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        for (int i = 0; i < nsteps; ++i) {
+            s->eng->step();
+            s->current_step++;  // Euler.Step / RK4.Propagate increment
+        }
+    });
+}
+
+int gpf_solver_get_time(gpf_solver* s, double* t) {
+    return guarded([&] {
+        if (!s || !t) throw Error(GPF_EINVAL, "bad argument");
+        *t = (double)s->current_step * s->dt;
+    });
+}
+
+int gpf_solver_get_field(gpf_solver* s, int idx, double* out, long n) {
+    return guarded([&] {
+        if (!s || !out) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        if (n != s->eng->size()) throw Error(GPF_EINVAL, "buffer length does not match the grid");
+        s->eng->get_field(idx, out);
+    });
+}
+
+int gpf_solver_set_field(gpf_solver* s, int idx, const double* in, long n) {
+    return guarded([&] {
+        if (!s || !in) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        if (n != s->eng->size()) throw Error(GPF_EINVAL, "buffer length does not match the grid");
+        s->eng->set_field(idx, in);
+    });
+}
+
+int gpf_solver_energy(gpf_solver* s, int idx, int npoly, const double* poly, double kappa, double* energy,
+                      double* mass) {
+    return guarded([&] {
+        if (!s || !energy || !mass || (npoly && !poly)) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        s->eng->energy(idx, npoly, poly, kappa, energy, mass);
+    });
+}
+
+int gpf_solver_monitor_energy(gpf_solver* s, int npoly, const double* poly, double kappa) {
+    return guarded([&] {
+        if (!s || (npoly && !poly)) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        s->eng->monitor(npoly, poly, kappa);
+    });
+}
+
+int gpf_solver_energy_series(gpf_solver* s, double* energy, double* mass, long cap, long* n) {
+    return guarded([&] {
+        if (!s || !n) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        *n = s->eng->series(energy, mass, cap);
+    });
+}
+
+int gpf_solver_synchronize(gpf_solver* s) {
+    return guarded([&] {
+        if (!s) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        HIPCHK(hipStreamSynchronize(s->eng->stream()));
+    });
+}
+
+int gpf_solver_is_fused(const gpf_solver* s) { return (s && s->eng && s->eng->fused()) ? 1 : 0; }
+
+int gpf_solver_enable_timing(gpf_solver* s, int on) {
+    return guarded([&] {
+        if (!s) throw Error(GPF_EINVAL, "bad argument");
+        s->eng->enable_timing(on != 0);
+    });
+}
+
+int gpf_solver_pass_times(gpf_solver* s, int maxpass, double* ms, long* launches, char* names, int namelen) {
+    int np = 0;
+    int rc = guarded([&] {
+        if (!s) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        np = s->eng->pass_times(maxpass, ms, launches, names, namelen);
+    });
+    return rc ? -rc : np;
+}
+
+void* gpf_solver_stream(gpf_solver* s) { return (s && s->eng) ? (void*)s->eng->stream() : nullptr; }
+
+int gpf_fft_create(const int* dims, int ndim, int precision, int device, gpf_fft** out) {
+    return guarded([&] {
+        if (!dims || !out) throw Error(GPF_EINVAL, "bad argument");
+        set_device(device);
+        auto f = std::make_unique<gpf_fft>();
+        f->device = device;
+        if (precision == GPF_FP32) f->impl.reset(new FullFFT<float>(dims, ndim));
+        else f->impl.reset(new FullFFT<double>(dims, ndim));
+        *out = f.release();
+    });
+}
+
+void gpf_fft_destroy(gpf_fft* f) { delete f; }
+
+int gpf_fft_forward(gpf_fft* f, double* data, long n) {
+    return guarded([&] {
+        if (!f || !data) throw Error(GPF_EINVAL, "bad argument");
+        (void)n;
+        set_device(f->device);
+        f->impl->exec(data, -1);
+    });
+}
+
+int gpf_fft_inverse(gpf_fft* f, double* data, long n) {
+    return guarded([&] {
+        if (!f || !data) throw Error(GPF_EINVAL, "bad argument");
+        (void)n;
+        set_device(f->device);
+        f->impl->exec(data, +1);
+    });
+}
+
+int gpf_fft_freq(const int* dims, int ndim, long i, double* out) {
+    return guarded([&] {
+        if (!dims || !out || ndim < 2 || ndim > 3) throw Error(GPF_EINVAL, "bad argument");
+        const long c = i % dims[1];
+        const long r = (i / dims[1]) % dims[0];
+        out[1] = (double)c / (double)dims[1];
+        out[0] = (double)r / (double)dims[0];
+        if (ndim > 2) out[2] = (double)(i / ((long)dims[0] * dims[1])) / (double)dims[2];
+        for (int a = 0; a < ndim; ++a)
+            if (out[a] > 0.5) out[a] -= 1.0;
+    });
+}
+
+long gpf_fft_conjugate_node(const int* dims, int ndim, long i) {
+    if (!dims || ndim < 2 || ndim > 3) return -1;
+    const long c = i % dims[1];
+    const long r = (i / dims[1]) % dims[0];
+    const long cc = (dims[1] - c) % dims[1];
+    const long cr = (dims[0] - r) % dims[0];
+    long cd = 0;
+    if (ndim == 3) cd = (dims[2] - i / ((long)dims[0] * dims[1])) % dims[2];
+    return cd * dims[0] * dims[1] + cr * dims[1] + cc;
+}
+
+int gpf_vandeven_table(int order, double* table) {
+    return guarded([&] {
+        if (!table || order < 1) throw Error(GPF_EINVAL, "bad argument");
+        vandeven_table(order, table);
+    });
+}
+
+double gpf_vandeven_eval(const double* data, double x) {
+    const double N = 999.0;
+    int idx = (int)(x * N);
+    if (idx >= 999) return data[999];
+    const double dx = 1.0 / N;
+    const double dy = data[idx + 1] - data[idx];
+    const double x0 = (double)idx * dx;
+    return data[idx] + (x - x0) * dy / dx;
+}
+
+}  // extern "C"

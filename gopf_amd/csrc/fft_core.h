@@ -1,0 +1,277 @@
+// fft_core.h -- device-side building blocks of the spectral step (gfx950).
+//
+// Complex lines are transformed in LDS with a self-sorting (Stockham) mixed-
+// radix schedule.  Each thread of a line owns E elements per stage; a stage of
+// radix R runs E/R radix-R butterflies per thread entirely in registers, and
+// stages exchange data through LDS.  Transform convention (FFTW's, which the
+// reference calls through pfutil/fftWrap.go:19-20):
+//   forward (DIR = -1): X[k] = sum_n x[n] exp(-2 pi i n k / N)
+//   inverse (DIR = +1): x[n] = sum_k X[k] exp(+2 pi i n k / N)   (unnormalised)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gpf {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cpx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> mk(T a, T b) { cpx<T> r; r.x = a; r.y = b; return r; }
+template <typename T>
+__device__ __forceinline__ cpx<T> add(cpx<T> a, cpx<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
+template <typename T>
+__device__ __forceinline__ cpx<T> sub(cpx<T> a, cpx<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
+template <typename T>
+__device__ __forceinline__ cpx<T> mul(cpx<T> a, cpx<T> b) {
+    return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> scale(cpx<T> a, T s) { return mk<T>(a.x * s, a.y * s); }
+template <typename T>
+__device__ __forceinline__ cpx<T> conjg(cpx<T> a) { return mk<T>(a.x, -a.y); }
+// multiply by (DIR * i): DIR=-1 -> -i, DIR=+1 -> +i
+template <int DIR, typename T>
+__device__ __forceinline__ cpx<T> mul_i(cpx<T> a) {
+    return (DIR < 0) ? mk<T>(a.y, -a.x) : mk<T>(-a.y, a.x);
+}
+
+// ---------------------------------------------------------------------------
+// compile-time radix schedules
+// ---------------------------------------------------------------------------
+template <int N>
+struct Sched;  // NR stages, radices R[], E elements per thread
+#define GPF_SCHED(n, e, ...)                                \
+    template <>                                             \
+    struct Sched<n> {                                       \
+        static constexpr int E = e;                         \
+        static constexpr int R[] = {__VA_ARGS__};           \
+        static constexpr int NR = sizeof(R) / sizeof(int);  \
+        static constexpr int T = n / e;                     \
+    };
+GPF_SCHED(2, 2, 2)
+GPF_SCHED(4, 4, 4)
+GPF_SCHED(8, 8, 8)
+GPF_SCHED(16, 16, 16)
+GPF_SCHED(32, 8, 8, 4)
+GPF_SCHED(64, 8, 8, 8)
+GPF_SCHED(128, 16, 8, 16)
+GPF_SCHED(256, 16, 16, 16)
+GPF_SCHED(512, 8, 8, 8, 8)
+GPF_SCHED(1024, 16, 16, 16, 4)
+GPF_SCHED(12, 12, 4, 3)
+GPF_SCHED(24, 24, 8, 3)
+GPF_SCHED(48, 12, 4, 4, 3)
+GPF_SCHED(96, 24, 8, 4, 3)
+GPF_SCHED(192, 24, 8, 8, 3)
+GPF_SCHED(384, 24, 8, 8, 2, 3)
+GPF_SCHED(768, 48, 16, 16, 3)
+#undef GPF_SCHED
+
+// span before stage S
+template <int N, int S>
+struct Span {
+    static constexpr int v = Span<N, S - 1>::v * Sched<N>::R[S - 1];
+};
+template <int N>
+struct Span<N, 0> {
+    static constexpr int v = 1;
+};
+
+// ---------------------------------------------------------------------------
+// small DFTs in registers
+// ---------------------------------------------------------------------------
+template <int DIR, typename T>
+__device__ __forceinline__ void dft2(cpx<T>& a0, cpx<T>& a1) {
+    cpx<T> t = sub(a0, a1);
+    a0 = add(a0, a1);
+    a1 = t;
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft4(cpx<T>& a0, cpx<T>& a1, cpx<T>& a2, cpx<T>& a3) {
+    cpx<T> t0 = add(a0, a2), t1 = sub(a0, a2);
+    cpx<T> t2 = add(a1, a3), t3 = mul_i<DIR>(sub(a1, a3));
+    a0 = add(t0, t2);
+    a2 = sub(t0, t2);
+    a1 = add(t1, t3);
+    a3 = sub(t1, t3);
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft3(cpx<T>* a) {
+    const T c = T(-0.5);
+    const T s = T(DIR) * T(0.86602540378443864676372317075294);  // DIR * sqrt(3)/2
+    cpx<T> sm = add(a[1], a[2]), df = sub(a[1], a[2]);
+    cpx<T> b = mk<T>(a[0].x + c * sm.x, a[0].y + c * sm.y);
+    // i * s * df
+    cpx<T> r = mk<T>(-s * df.y, s * df.x);
+    a[0] = add(a[0], sm);
+    a[1] = add(b, r);
+    a[2] = sub(b, r);
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft8(cpx<T>* a) {
+    const T h = T(0.70710678118654752440084436210485);
+    cpx<T> e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+    cpx<T> o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+    dft4<DIR>(e0, e1, e2, e3);
+    dft4<DIR>(o0, o1, o2, o3);
+    // twiddles W8^k, W8 = exp(DIR * 2 pi i / 8)
+    cpx<T> w1o = mk<T>(h * (o1.x - T(DIR) * o1.y), h * (o1.y + T(DIR) * o1.x));
+    cpx<T> w2o = mul_i<DIR>(o2);
+    cpx<T> w3o = mk<T>(h * (-o3.x - T(DIR) * o3.y), h * (-o3.y + T(DIR) * o3.x));
+    a[0] = add(e0, o0);
+    a[4] = sub(e0, o0);
+    a[1] = add(e1, w1o);
+    a[5] = sub(e1, w1o);
+    a[2] = add(e2, w2o);
+    a[6] = sub(e2, w2o);
+    a[3] = add(e3, w3o);
+    a[7] = sub(e3, w3o);
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft16(cpx<T>* a) {
+    // 4 x 4 split: m = m2 + 4 m1, k = k1 + 4 k2
+    cpx<T> b[16];
+#pragma unroll
+    for (int m2 = 0; m2 < 4; ++m2) {
+        cpx<T> x0 = a[m2], x1 = a[m2 + 4], x2 = a[m2 + 8], x3 = a[m2 + 12];
+        dft4<DIR>(x0, x1, x2, x3);
+        b[m2 * 4 + 0] = x0;
+        b[m2 * 4 + 1] = x1;
+        b[m2 * 4 + 2] = x2;
+        b[m2 * 4 + 3] = x3;
+    }
+    // twiddle W16^(m2 k1)
+    const T c1 = T(0.92387953251128675612818318939679);
+    const T s1 = T(0.38268343236508977172845998403040);
+    const T h = T(0.70710678118654752440084436210485);
+    const T sg = T(DIR);
+    // W16^e for e in 1..9 with W16 = exp(DIR 2 pi i / 16)
+    const cpx<T> w1 = mk<T>(c1, sg * s1), w2 = mk<T>(h, sg * h), w3 = mk<T>(s1, sg * c1);
+    const cpx<T> w6 = mk<T>(-h, sg * h), w9 = mk<T>(-c1, -sg * s1);
+    b[1 * 4 + 1] = mul(b[1 * 4 + 1], w1);
+    b[1 * 4 + 2] = mul(b[1 * 4 + 2], w2);
+    b[1 * 4 + 3] = mul(b[1 * 4 + 3], w3);
+    b[2 * 4 + 1] = mul(b[2 * 4 + 1], w2);
+    b[2 * 4 + 2] = mul_i<DIR>(b[2 * 4 + 2]);
+    b[2 * 4 + 3] = mul(b[2 * 4 + 3], w6);
+    b[3 * 4 + 1] = mul(b[3 * 4 + 1], w3);
+    b[3 * 4 + 2] = mul(b[3 * 4 + 2], w6);
+    b[3 * 4 + 3] = mul(b[3 * 4 + 3], w9);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        cpx<T> x0 = b[0 * 4 + k1], x1 = b[1 * 4 + k1], x2 = b[2 * 4 + k1], x3 = b[3 * 4 + k1];
+        dft4<DIR>(x0, x1, x2, x3);
+        a[k1] = x0;
+        a[k1 + 4] = x1;
+        a[k1 + 8] = x2;
+        a[k1 + 12] = x3;
+    }
+}
+
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void dftR(cpx<T>* a) {
+    if constexpr (R == 2) {
+        dft2<DIR>(a[0], a[1]);
+    } else if constexpr (R == 3) {
+        dft3<DIR>(a);
+    } else if constexpr (R == 4) {
+        dft4<DIR>(a[0], a[1], a[2], a[3]);
+    } else if constexpr (R == 8) {
+        dft8<DIR>(a);
+    } else if constexpr (R == 16) {
+        dft16<DIR>(a);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// one Stockham stage on a line held in LDS
+//   line element p lives at buf[p * es]; thread t in [0, T) of the line.
+//   tw: table of exp(-2 pi i q / N), q in [0, N)
+// ---------------------------------------------------------------------------
+template <typename T, int N, int S, int DIR>
+__device__ __forceinline__ void stage(cpx<T>* buf, int es, int t, const cpx<T>* __restrict__ tw) {
+    constexpr int R = Sched<N>::R[S];
+    constexpr int E = Sched<N>::E;
+    constexpr int TT = Sched<N>::T;
+    constexpr int NB = E / R;       // butterflies per thread
+    constexpr int NR = N / R;       // input stride
+    constexpr int NS = Span<N, S>::v;
+    constexpr int TWSTEP = N / (NS * R);
+    cpx<T> v[E];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = t + b * TT;
+#pragma unroll
+        for (int m = 0; m < R; ++m) v[b * R + m] = buf[(j + m * NR) * es];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = t + b * TT;
+        const int k = (NS == 1) ? 0 : (j % NS);
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int m = 1; m < R; ++m) {
+                cpx<T> w = tw[(k * m * TWSTEP) % N];
+                if (DIR > 0) w.y = -w.y;
+                v[b * R + m] = mul(v[b * R + m], w);
+            }
+        }
+        dftR<R, DIR>(&v[b * R]);
+        const int base = (NS == 1) ? j * R : ((j / NS) * NS * R + k);
+#pragma unroll
+        for (int m = 0; m < R; ++m) buf[(base + m * NS) * es] = v[b * R + m];
+    }
+    __syncthreads();
+}
+
+template <typename T, int N, int S, int DIR>
+__device__ __forceinline__ void stages_from(cpx<T>* buf, int es, int t, const cpx<T>* __restrict__ tw) {
+    if constexpr (S < Sched<N>::NR) {
+        stage<T, N, S, DIR>(buf, es, t, tw);
+        stages_from<T, N, S + 1, DIR>(buf, es, t, tw);
+    }
+}
+
+// Full in-place transform of a line in LDS.  All threads of the block must
+// call it (it synchronises the block); threads with t >= T idle through.
+template <typename T, int N, int DIR>
+__device__ __forceinline__ void fft_line(cpx<T>* buf, int es, int t, bool active,
+                                         const cpx<T>* __restrict__ tw) {
+    // stage() synchronises internally; inactive threads must still reach the barriers
+    if (active) {
+        stages_from<T, N, 0, DIR>(buf, es, t, tw);
+    } else {
+        for (int s = 0; s < 2 * Sched<N>::NR; ++s) __syncthreads();
+    }
+}
+
+// Brute-force DFT of a line of runtime length n (n threads per line, t = output index).
+template <typename T, int DIR>
+__device__ __forceinline__ void dft_line_rt(cpx<T>* buf, int es, int n, int t, bool active,
+                                            const cpx<T>* __restrict__ tw) {
+    cpx<T> acc = mk<T>(T(0), T(0));
+    if (active) {
+        int q = 0;
+        for (int m = 0; m < n; ++m) {
+            cpx<T> w = tw[q];
+            if (DIR > 0) w.y = -w.y;
+            acc = add(acc, mul(buf[m * es], w));
+            q += t;
+            if (q >= n) q -= n;
+        }
+    }
+    __syncthreads();
+    if (active) buf[t * es] = acc;
+    __syncthreads();
+}
+
+}  // namespace gpf

@@ -1,0 +1,417 @@
+// pointwise.hip -- real-space and k-space pointwise kernels of the general stepper.
+//
+//   k_derived     SyncDerivedFields / DerivedFieldCalcFromDesc (model.go:237-241,
+//                 util.go:38-65) and RegisterFunction polynomials (model.go:400-412)
+//   k_kupdate     GetRHS / GetDenum sums of ConcreteTerm closures (rhsBuilder.go:125-190,
+//                 model.go:283-314) fused with the Euler / RK4 stage algebra
+//                 (euler.go:32-38, rk4.go:77-127) and ApplyModalFilter (util.go:125-132)
+//   k_kgrad       i 2 pi f_d u(k) with f_d = +0.5 zeroed (squareGradientTerm.go:44-50)
+//   k_kelastic    scalar Khachaturyan kernel K(k): force (effectiveForce.go:26-35), 3x3
+//                 displacement solve (linearElasticity.go:33-63), strains
+//                 (linearElasticity.go:67-83) and the sigma*:eps sum (homoLinElast.go:72-89)
+//                 collapsed by linearity into one multiplier of FFT(H)
+#include "fft_core.h"
+#include "pointwise.h"
+
+namespace gpf {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+__device__ __forceinline__ void kcoords(const KGrid& g, long idx, long nmain, int& kx, int& ky, int& kz) {
+    const int nxh = g.nx / 2;
+    if (idx < nmain) {
+        kx = (int)(idx % nxh);
+        const long r = idx / nxh;
+        ky = (int)(r % g.ny);
+        kz = (int)(r / g.ny);
+    } else {
+        const long j = idx - nmain;
+        kx = nxh;
+        ky = (int)(j % g.ny);
+        kz = (int)(j / g.ny);
+    }
+}
+
+__device__ __forceinline__ double wrapf(int k, int n) {
+    double f = (double)k / (double)n;
+    return (f > 0.5) ? f - 1.0 : f;
+}
+
+// reference Freq components (fftWrap.go:57-74): f[0] row (y), f[1] col (x), f[2] depth (z)
+__device__ __forceinline__ void ref_freq(const KGrid& g, int kx, int ky, int kz, double f[3]) {
+    f[0] = wrapf(ky, g.ny);
+    f[1] = wrapf(kx, g.nx);
+    f[2] = (g.ndim == 3) ? wrapf(kz, g.nz) : 0.0;
+}
+
+__device__ __forceinline__ double ipow(double x, int p) {
+    double v = 1.0;
+    for (int q = 0; q < p; ++q) v *= x;
+    return v;
+}
+
+__device__ __forceinline__ double vandeven(const double* __restrict__ tab, double x) {
+    const double N = 999.0;
+    int idx = (int)(x * N);
+    if (idx >= 999) return tab[999];
+    double dx = 1.0 / N;
+    double dy = tab[idx + 1] - tab[idx];
+    double x0 = (double)idx * dx;
+    return tab[idx] + (x - x0) * dy / dx;
+}
+
+template <typename T>
+__device__ __forceinline__ T powT(T x, double p) {
+    const double r = rint(p);
+    if (r == p && fabs(p) <= 64.0) {
+        int ip = (int)fabs(p);
+        T v = T(1);
+        T b = x;
+        while (ip) {
+            if (ip & 1) v *= b;
+            b *= b;
+            ip >>= 1;
+        }
+        return (p < 0) ? T(1) / v : v;
+    }
+    return (T)pow((double)x, p);
+}
+
+template <typename T>
+__global__ void k_derived(DerivedArgs<T> a) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.n; i += (long)gridDim.x * blockDim.x) {
+        T x[kMaxFields];
+        for (int f = 0; f < a.nf; ++f) x[f] = a.f[f] ? a.f[f][i] : T(0);
+        T acc = T(0);
+        for (int t = 0; t < a.nterm; ++t) {
+            T v = (T)a.coef[t];
+            for (int f = 0; f < a.nf; ++f)
+                if (a.pw[t][f] != 0.0) v *= powT(x[f], a.pw[t][f]);
+            acc += v;
+        }
+        a.out[i] = acc;
+    }
+}
+
+template <typename T>
+__global__ void k_kupdate(KUpdateArgs<T> a) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
+        int kx, ky, kz;
+        kcoords(a.g, i, a.nmain, kx, ky, kz);
+        double f[3];
+        ref_freq(a.g, kx, ky, kz, f);
+        const double f2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+        const double L = -(4.0 * kPi * kPi) * f2;
+        double rx = 0.0, ry = 0.0;
+        for (int t = 0; t < a.nterm; ++t) {
+            const double m = a.coef[t] * ipow(L, a.pw[t]);
+            if (a.brick[t]) {
+                const cpx<T> b = a.brick[t][i];
+                rx += m * (double)b.x;
+                ry += m * (double)b.y;
+            } else {
+                rx += m;
+            }
+        }
+        double D = 0.0;
+        for (int t = 0; t < a.denom.n; ++t) D += a.denom.coef[t] * ipow(L, a.denom.pow[t]);
+        double s = 1.0 / (1.0 - a.gamma * D);
+        if (a.filter) s *= vandeven(a.filter, sqrt(f2) * 2.0 / kPi);
+        double cx = 0.0, cy = 0.0;
+        if (a.cur) {
+            const cpx<T> c = a.cur[i];
+            cx = (double)c.x;
+            cy = (double)c.y;
+        }
+        a.dst[i] = mk<T>((T)(s * (a.alpha * cx + a.beta * rx)), (T)(s * (a.alpha * cy + a.beta * ry)));
+    }
+}
+
+template <typename T>
+__global__ void k_kgrad(KGradArgs<T> a, int ndim) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
+        int kx, ky, kz;
+        kcoords(a.g, i, a.nmain, kx, ky, kz);
+        double f[3];
+        ref_freq(a.g, kx, ky, kz, f);
+        const cpx<T> u = a.src[i];
+        for (int d = 0; d < ndim; ++d) {
+            double fd = f[d];
+            if (fabs(fd - 0.5) < 1e-10) fd = 0.0;
+            const double w = 2.0 * kPi * fd;
+            // (0 + i w) * u
+            a.out[d][i] = mk<T>((T)(-w * (double)u.y), (T)(w * (double)u.x));
+        }
+    }
+}
+
+// 3x3 LU with partial pivoting; returns false if singular
+__device__ bool solve3(double A[3][3], double b[3], double x[3]) {
+    int piv[3] = {0, 1, 2};
+    for (int k = 0; k < 3; ++k) {
+        int p = k;
+        double best = fabs(A[k][k]);
+        for (int i = k + 1; i < 3; ++i)
+            if (fabs(A[i][k]) > best) { best = fabs(A[i][k]); p = i; }
+        if (best == 0.0) return false;
+        if (p != k) {
+            for (int j = 0; j < 3; ++j) { double t = A[k][j]; A[k][j] = A[p][j]; A[p][j] = t; }
+            int t = piv[k]; piv[k] = piv[p]; piv[p] = t;
+        }
+        for (int i = k + 1; i < 3; ++i) {
+            A[i][k] /= A[k][k];
+            for (int j = k + 1; j < 3; ++j) A[i][j] -= A[i][k] * A[k][j];
+        }
+    }
+    double y[3];
+    for (int i = 0; i < 3; ++i) {
+        double s = b[piv[i]];
+        for (int j = 0; j < i; ++j) s -= A[i][j] * y[j];
+        y[i] = s;
+    }
+    for (int i = 2; i >= 0; --i) {
+        double s = y[i];
+        for (int j = i + 1; j < 3; ++j) s -= A[i][j] * x[j];
+        x[i] = s / A[i][i];
+    }
+    return true;
+}
+
+__device__ double elastic_kernel(const ElasticParams& ep, const double f[3]) {
+    if (fabs(f[0]) < 1e-10 && fabs(f[1]) < 1e-10 && fabs(f[2]) < 1e-10) return 0.0;
+    double G[3][3];
+    const double tp2 = 4.0 * kPi * kPi;
+    for (int m = 0; m < 3; ++m)
+        for (int n = 0; n < 3; ++n) {
+            double e = 0.0;
+            for (int j = 0; j < 3; ++j)
+                for (int l = 0; l < 3; ++l) e += ep.C[m * 27 + j * 9 + n * 3 + l] * f[j] * f[l];
+            G[m][n] = e * tp2;
+        }
+    // force direction b_c = sum_j sigma*_cj f_j for c < Dim (u = -i 2 pi G^-1 b H)
+    double b[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < ep.dim; ++c)
+        for (int j = 0; j < 3; ++j) b[c] += ep.sig[c * 3 + j] * f[j];
+    double v[3];
+    if (!solve3(G, b, v)) return 0.0;
+    double fm[3];
+    for (int c = 0; c < 3; ++c) fm[c] = (fabs(fabs(f[c]) - 0.5) < 1e-10) ? 0.0 : f[c];
+    // eps_ab = 2 pi^2 (f'_b v_a + f'_a v_b) H ; K = sum_{a<=b<Dim} fac sigma*_ab eps_ab / H
+    double K = 0.0;
+    for (int a = 0; a < ep.dim; ++a)
+        for (int bb = a; bb < ep.dim; ++bb) {
+            const double fac = (a != bb) ? 2.0 : 1.0;
+            K += fac * ep.sig[a * 3 + bb] * 2.0 * kPi * kPi * (fm[bb] * v[a] + fm[a] * v[bb]);
+        }
+    return K;
+}
+
+template <typename T>
+__global__ void k_kelastic(KElasticArgs<T> a) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
+        int kx, ky, kz;
+        kcoords(a.g, i, a.nmain, kx, ky, kz);
+        double f[3];
+        ref_freq(a.g, kx, ky, kz, f);
+        const double K = elastic_kernel(a.ep, f);
+        const cpx<T> s = a.src[i];
+        a.dst[i] = mk<T>((T)(K * (double)s.x), (T)(K * (double)s.y));
+    }
+}
+
+template <typename T>
+__global__ void k_sqsum(const T* w0, const T* w1, const T* w2, int nw, T factor, T* out, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        T acc = w0[i] * w0[i];
+        if (nw > 1) acc += w1[i] * w1[i];
+        if (nw > 2) acc += w2[i] * w2[i];
+        out[i] = factor * acc;
+    }
+}
+
+template <typename T>
+__global__ void k_indicator(const T* phi, T* out, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const T x = phi[i];
+        out[i] = T(3) * x * x - T(2) * x * x * x;
+    }
+}
+
+template <typename T>
+__global__ void k_elastic_real(const T* phi, const T* S, double two_estar, T* out, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const T x = phi[i];
+        const T H = T(3) * x * x - T(2) * x * x * x;
+        const T dH = T(6) * x - T(6) * x * x;
+        out[i] = dH * (S[i] - (T)two_estar * H);
+    }
+}
+
+template <typename T>
+__global__ void k_copy(const T* src, T* dst, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__device__ double block_sum(double v, double* red) {
+    const int tid = threadIdx.x;
+    red[tid] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+struct KSum {
+    double s = 0.0, c = 0.0;
+    __device__ void add(double v) {
+        double t = s + v;
+        c += (fabs(s) >= fabs(v)) ? ((s - t) + v) : ((v - t) + s);
+        s = t;
+    }
+    __device__ double get() const { return s + c; }
+};
+
+constexpr int kRedThreads = 256;
+
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) k_real_sums(const T* x, long n, int npoly, const double* poly,
+                                                            double* pf, double* px) {
+    __shared__ double red[kRedThreads];
+    KSum sf, sx;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double c = (double)x[i];
+        double f = 0.0;
+        for (int q = npoly - 1; q >= 0; --q) f = f * c + poly[q];
+        sf.add(f);
+        sx.add(c);
+    }
+    const double a = block_sum(sf.get(), red);
+    const double b = block_sum(sx.get(), red);
+    if (threadIdx.x == 0) {
+        pf[blockIdx.x] = a;
+        px[blockIdx.x] = b;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) k_kgrad_energy(const cpx<T>* C, KGrid g, long nmain, long nk,
+                                                               double* partial) {
+    __shared__ double red[kRedThreads];
+    KSum s;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nk; i += (long)gridDim.x * blockDim.x) {
+        int kx, ky, kz;
+        kcoords(g, i, nmain, kx, ky, kz);
+        double f[3];
+        ref_freq(g, kx, ky, kz, f);
+        const double f2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+        const double w = (kx == 0 || 2 * kx == g.nx) ? 1.0 : 2.0;
+        const cpx<T> c = C[i];
+        s.add(w * 4.0 * kPi * kPi * f2 * ((double)c.x * (double)c.x + (double)c.y * (double)c.y));
+    }
+    const double a = block_sum(s.get(), red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = a;
+}
+
+__global__ void __launch_bounds__(kRedThreads) k_reduce2(const double* a, long na, double ca, const double* b,
+                                                          long nb, double cb, double* out) {
+    __shared__ double red[kRedThreads];
+    KSum sa, sb;
+    for (long i = threadIdx.x; i < na; i += blockDim.x) sa.add(a[i]);
+    for (long i = threadIdx.x; i < nb; i += blockDim.x) sb.add(b[i]);
+    const double x = block_sum(sa.get(), red);
+    const double y = block_sum(sb.get(), red);
+    if (threadIdx.x == 0) out[0] = ca * x + cb * y;
+}
+
+inline unsigned grid_for(long n, int thr = 256) {
+    long g = (n + thr - 1) / thr;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_derived(const DerivedArgs<T>& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_derived<T>, dim3(grid_for(a.n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_kupdate(const KUpdateArgs<T>& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_kupdate<T>, dim3(grid_for(a.nk)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_kgrad(const KGradArgs<T>& a, int ndim, hipStream_t s) {
+    hipLaunchKernelGGL(k_kgrad<T>, dim3(grid_for(a.nk)), dim3(256), 0, s, a, ndim);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_kelastic(const KElasticArgs<T>& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_kelastic<T>, dim3(grid_for(a.nk)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_sqsum(const T* const* w, int nw, T factor, T* out, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_sqsum<T>, dim3(grid_for(n)), dim3(256), 0, s, w[0], nw > 1 ? w[1] : w[0],
+                       nw > 2 ? w[2] : w[0], nw, factor, out, n);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_indicator(const T* phi, T* out, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_indicator<T>, dim3(grid_for(n)), dim3(256), 0, s, phi, out, n);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_elastic_real(const T* phi, const T* S, double two_estar, T* out, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_elastic_real<T>, dim3(grid_for(n)), dim3(256), 0, s, phi, S, two_estar, out, n);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_copy(const T* src, T* dst, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy<T>, dim3(grid_for(n)), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_real_sums(const T* x, long n, int npoly, const double* poly, double* pf, double* px, long nblocks,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_real_sums<T>, dim3((unsigned)nblocks), dim3(kRedThreads), 0, s, x, n, npoly, poly, pf, px);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_kgrad_energy(const cpx<T>* C, KGrid g, long nmain, long nk, double* partial, long nblocks,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_kgrad_energy<T>, dim3((unsigned)nblocks), dim3(kRedThreads), 0, s, C, g, nmain, nk, partial);
+    return hipGetLastError();
+}
+hipError_t launch_reduce2(const double* a, long na, double ca, const double* b, long nb, double cb, double* out,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kRedThreads), 0, s, a, na, ca, b, nb, cb, out);
+    return hipGetLastError();
+}
+
+#define GPF_INST(T)                                                                                            \
+    template hipError_t launch_derived<T>(const DerivedArgs<T>&, hipStream_t);                                \
+    template hipError_t launch_kupdate<T>(const KUpdateArgs<T>&, hipStream_t);                                \
+    template hipError_t launch_kgrad<T>(const KGradArgs<T>&, int, hipStream_t);                               \
+    template hipError_t launch_kelastic<T>(const KElasticArgs<T>&, hipStream_t);                              \
+    template hipError_t launch_sqsum<T>(const T* const*, int, T, T*, long, hipStream_t);                      \
+    template hipError_t launch_indicator<T>(const T*, T*, long, hipStream_t);                                 \
+    template hipError_t launch_elastic_real<T>(const T*, const T*, double, T*, long, hipStream_t);            \
+    template hipError_t launch_copy<T>(const T*, T*, long, hipStream_t);                                      \
+    template hipError_t launch_real_sums<T>(const T*, long, int, const double*, double*, double*, long,       \
+                                            hipStream_t);                                                     \
+    template hipError_t launch_kgrad_energy<T>(const cpx<T>*, KGrid, long, long, double*, long, hipStream_t);
+GPF_INST(float)
+GPF_INST(double)
+#undef GPF_INST
+
+}  // namespace gpf

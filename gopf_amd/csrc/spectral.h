@@ -1,0 +1,142 @@
+// spectral.h -- host-visible launch interface of the HIP spectral kernels.
+//
+// Data layout in HBM (all arrays device-resident across steps):
+//   real-space field      T[nz][ny][nx]                 (nx fastest; nz = 1 in 2D)
+//   half spectrum along x split in two arrays:
+//     main                cpx<T>[nz][ny][nx/2]           kx = 0 .. nx/2-1
+//     nyq                 cpx<T>[nz][ny]                 kx = nx/2 (the x-Nyquist plane)
+//   full complex array    cpx<T>[nz][ny][nx]             (FourierTransform drop-in)
+// The x index of the layout is the reference's "col", y is "row" and z is
+// "depth" (pfutil/indexPositionConversion.go:9-11, FFTW plan n = [D0, D1, D2]).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace gpf {
+
+template <typename T>
+struct cpx;
+
+constexpr int kMaxArr = 4;
+constexpr int kMaxPoly = 9;     // polynomial degree < kMaxPoly
+constexpr int kMaxKTerms = 8;   // k-space multiplier terms
+
+// line grid of one pencil segment
+struct Seg {
+    int na = 0, nb = 0;         // lines: a in [0, na), b in [0, nb); tile = TW consecutive b
+    long sa = 0, sb = 0, es = 0;
+    int ksrc[3] = {3, 3, 3};    // source of (kx, ky, kz): 0 = a, 1 = b, 2 = p, 3 = fixed
+    int kfix[3] = {0, 0, 0};
+};
+
+// k-space multiplier sum  M(k) = sum_t coef_t * L(k)^pow_t,  L = -(2 pi |f|)^2
+struct KPoly {
+    int n = 0;
+    double coef[kMaxKTerms];
+    int pow[kMaxKTerms];
+};
+
+// geometry of the transform grid
+struct KGrid {
+    int nx = 1, ny = 1, nz = 1;
+    int ndim = 2;
+};
+
+template <typename T>
+struct PencilArgs {
+    const cpx<T>* tw = nullptr;          // twiddle table for the pencil length
+    cpx<T>* arr[kMaxArr][2] = {};        // [array][segment] output (and input when src is null)
+    const cpx<T>* src[kMaxArr][2] = {};  // optional separate input (out-of-place pass)
+    int narr = 0;
+    Seg seg[2];
+    int nseg = 1;
+    int tiles0 = 0;                      // tiles of segment 0 (segment 1 follows)
+    int TW = 16;                         // lines per block (power of two)
+    int n = 0;                           // pencil length (runtime sizes)
+};
+
+// fused "forward -> k-space update -> inverse" pencil pass of the single-field fast path
+template <typename T>
+struct PencilUpdateArgs {
+    PencilArgs<T> p;                     // arr[0] = field spectrum (in/out), arr[1] = explicit source
+    KGrid g;
+    double dt = 0.0;
+    double scale = 1.0;                  // 1 / N_total folded into the update
+    KPoly denom;                         // implicit part D(k)
+    KPoly lin;                           // explicit terms linear in the field E(k)
+    int src_lap = 0;                     // LAP power of the explicit source
+    int has_src = 0;
+    const double* filter = nullptr;      // Vandeven table (1000 entries) or null
+    double* mass = nullptr;              // FP64 k = 0 mode (in/out) or null
+    double* egrad_partial = nullptr;     // per-block FP64 sum of w |k|^2 |C|^2 or null
+};
+
+// x-row R2C of pairs of real arrays into main/nyq half spectra
+template <typename T>
+struct RowR2CArgs {
+    const cpx<T>* tw = nullptr;
+    const T* in[kMaxArr] = {};           // real rows (may be null -> zero)
+    cpx<T>* main[kMaxArr] = {};
+    cpx<T>* nyq[kMaxArr] = {};
+    int narr = 0;                        // arrays processed in pairs
+    long rows = 0;
+    int nx = 0;
+    int TW = 16;
+    T in_scale = T(1);
+};
+
+// x-row C2R of half spectra into real rows (scaled)
+template <typename T>
+struct RowC2RArgs {
+    const cpx<T>* tw = nullptr;
+    const cpx<T>* main[kMaxArr] = {};
+    const cpx<T>* nyq[kMaxArr] = {};
+    T* out[kMaxArr] = {};
+    int narr = 0;
+    long rows = 0;
+    int nx = 0;
+    int TW = 16;
+    T scale = T(1);
+};
+
+// fused x-row pass of the single-field fast path:
+//   S (x half spectrum) --C2R--> c --poly--> g --R2C--> (C, G) half spectra
+template <typename T>
+struct RowFusedArgs {
+    const cpx<T>* tw = nullptr;
+    const cpx<T>* s_main = nullptr;
+    const cpx<T>* s_nyq = nullptr;
+    cpx<T>* c_main = nullptr;
+    cpx<T>* c_nyq = nullptr;
+    cpx<T>* g_main = nullptr;
+    cpx<T>* g_nyq = nullptr;
+    T* c_out = nullptr;                  // optional real-space copy of c
+    long rows = 0;
+    int nx = 0;
+    int TW = 16;
+    int npoly = 0;                       // g(c) = sum_q gpoly[q] c^q
+    double gpoly[kMaxPoly];
+    int nbulk = 0;                       // optional energy density f(c) = sum_q bulk[q] c^q
+    double bulk[kMaxPoly];
+    double* bulk_partial = nullptr;      // per-block FP64 sum of f(c)
+    double* mass_partial = nullptr;      // per-block FP64 sum of c
+};
+
+// launchers (implemented in spectral.hip); n is the line length.
+template <typename T> hipError_t launch_pencils(const PencilArgs<T>& a, int n, int dir, hipStream_t s);
+template <typename T> hipError_t launch_pencil_update(const PencilUpdateArgs<T>& a, int n, hipStream_t s);
+template <typename T> hipError_t launch_rows_r2c(const RowR2CArgs<T>& a, hipStream_t s);
+template <typename T> hipError_t launch_rows_c2r(const RowC2RArgs<T>& a, hipStream_t s);
+template <typename T> hipError_t launch_rows_fused(const RowFusedArgs<T>& a, hipStream_t s);
+
+bool fft_size_supported(int n);      // sizes with a device transform
+bool fft_size_fast(int n);           // sizes with a compiled Stockham schedule
+int fft_threads_per_line(int n);     // threads per line of the transform
+
+// blocks a launch uses (for sizing per-block partial-sum buffers)
+long rows_blocks(long rows, int TW);
+long pencil_blocks(const Seg* seg, int nseg, int TW, int* tiles0);
+
+}  // namespace gpf

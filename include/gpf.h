@@ -1,0 +1,168 @@
+/*
+ * gpf.h -- C ABI of the MI355X spectral-step library (libgopf_amd.so).
+ *
+ * The entry points are what a cgo binding of gopf's spectral time step
+ * needs (see INTEGRATION.md for the Go side).  Each group replaces one
+ * reference interface, cited as path:line in the gopf repository:
+ *
+ *   gpf_model_*   pf.Model (pf/model.go:116-260): fields, scalars, equation
+ *                 strings, registered terms; gpf_model_init is Model.Init and
+ *                 parses the equations exactly as pf/rhsBuilder.go:26-242.
+ *   gpf_solver_*  pf.Solver + pf.TimeStepper (pf/solver.go:15-120,
+ *                 pf/euler.go:16-59, pf/rk4.go:29-135): fields live in HBM,
+ *                 steps run on one MI355X.
+ *   gpf_fft_*     pf.FourierTransform implemented by pfutil.FFTWWrapper
+ *                 (pf/solver.go:22-26, pfutil/fftWrap.go:16-95).
+ *   gpf_vandeven_* pf.Vandeven (pf/vandeven.go:13-40).
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a non-zero GPF_E* code on
+ *     failure; gpf_last_error() returns the message of the last failure on
+ *     the calling thread.  (The reference panics; the Go wrapper turns a
+ *     non-zero code into a panic with that message.)
+ *   - Complex host arrays are interleaved complex128 (double re, double im),
+ *     the memory layout of Go's []complex128.  Host buffers are only
+ *     borrowed for the duration of a call.
+ *   - Grids follow FFTW's row-major layout, plan n = [D0, D1(, D2)], last
+ *     index fastest (pfutil/fftWrap.go:19).
+ *   - Handles are not thread-safe; one handle owns one HIP stream.
+ */
+#ifndef GPF_H
+#define GPF_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPF_OK 0
+#define GPF_EINVAL 1     /* invalid argument / unsupported configuration */
+#define GPF_EPARSE 2     /* equation parsing failed (the reference panics) */
+#define GPF_EDEVICE 3    /* HIP runtime error / no device */
+#define GPF_ESTATE 4     /* call out of order (e.g. solver before Model.Init) */
+
+#define GPF_FP64 0       /* plain FP64: complex128 transforms, FP64 pointwise work */
+#define GPF_FP32 1       /* FP32 bulk; FP64 k = 0 mode, denominators and reductions */
+
+#define GPF_EULER 0      /* pf.Euler */
+#define GPF_RK4 1        /* pf.RK4 */
+
+typedef struct gpf_model gpf_model;
+typedef struct gpf_solver gpf_solver;
+typedef struct gpf_fft gpf_fft;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* gpf_last_error(void);
+
+/* ---------------- pf.Model ---------------- */
+
+/* pf.NewModel (pf/model.go:130-138) */
+int gpf_model_create(gpf_model** out);
+void gpf_model_destroy(gpf_model* m);
+/* pf.NewField + Model.AddField (pf/model.go:44-57, 141-144); data may be NULL (zeros) */
+int gpf_model_add_field(gpf_model* m, const char* name, const double* data, long n);
+/* pf.NewScalar + Model.AddScalar (pf/model.go:92-98, 147-149) */
+int gpf_model_add_scalar(gpf_model* m, const char* name, double re, double im);
+/* Model.AddEquation (pf/model.go:157-162): strips spaces, creates derived fields */
+int gpf_model_add_equation(gpf_model* m, const char* eq);
+/* pf.NewSquareGradient + Model.RegisterExplicitTerm (pf/squareGradientTerm.go:24-37,
+ * pf/model.go:362-364); implicit != 0 registers it with RegisterImplicitTerm */
+int gpf_model_register_square_gradient(gpf_model* m, const char* term, const char* field, double factor,
+                                       int implicit);
+/* pf.NewHomogeneousModolus + RegisterExplicitTerm (pf/homoLinElast.go:137-150):
+ * C is C_ijkl (81 doubles, index i*27+j*9+k*3+l), misfit is 3x3 row-major */
+int gpf_model_register_homogeneous_modulus(gpf_model* m, const char* term, const char* field,
+                                           const double* C, const double* misfit);
+/* Model.RegisterFunction (pf/model.go:400-412) for a polynomial of the fields' real parts:
+ * value = sum_t coefs[t] * prod_f re(field_f)^powers[t * nfields + f]; the device cannot run
+ * Go closures, so the function is given as data. */
+int gpf_model_register_polynomial(gpf_model* m, const char* name, int nterms, const double* coefs,
+                                  const double* powers, int nfields);
+/* Model.Init (pf/model.go:244-260): Build every equation (pf/rhsBuilder.go:26-54) */
+int gpf_model_init(gpf_model* m);
+
+/* introspection of the parsed model (Model.AllFieldNames, Model.RHS) */
+typedef struct {
+    int kind;          /* 0 ConcreteTerm, 1 SquaredGradient, 2 HomogeneousModulusLinElast */
+    double coef_re;    /* concrete: sign * prod Pow(scalar, p); user terms: +-1 */
+    double coef_im;
+    int brick;         /* brick index (fields, then derived fields) or -1 */
+    int has_lap;       /* the term string contains LAP */
+    int lap;           /* LaplacianN power (concrete) / summed LAP prefixes (user terms) */
+    int user;          /* index among the registered terms of its kind, or -1 */
+} gpf_term;
+
+int gpf_model_num_bricks(const gpf_model* m, int* nfields, int* nderived);
+int gpf_model_brick_name(const gpf_model* m, int idx, char* buf, int buflen);
+int gpf_model_num_terms(const gpf_model* m, int eq, int* nterms, int* ndenum);
+int gpf_model_get_term(const gpf_model* m, int eq, int denum, int idx, gpf_term* out);
+
+/* ---------------- pf.Solver / pf.TimeStepper ---------------- */
+
+typedef struct {
+    int precision;     /* GPF_FP64 or GPF_FP32 */
+    int stepper;       /* GPF_EULER or GPF_RK4 (Solver.SetStepper, pf/solver.go:88-103) */
+    int filter_order;  /* > 0: Stepper.SetFilter(NewVandeven(order)); 0: no filter */
+    int fused;         /* 1: allow the fused single-field path; 0: general path only */
+    int device;        /* HIP device ordinal */
+} gpf_options;
+
+void gpf_options_default(gpf_options* o);
+
+/* pf.NewSolver (pf/solver.go:40-63) with the stepper chosen by o; the model must
+ * have been initialised.  Uploads the fields; they stay resident in HBM. */
+int gpf_solver_create(gpf_model* m, const int* dims, int ndim, double dt, const gpf_options* o,
+                      gpf_solver** out);
+void gpf_solver_destroy(gpf_solver* s);
+/* Solver.Propagate (pf/solver.go:70-85): nsteps x (Stepper.Step + OnStepFinished) */
+int gpf_solver_propagate(gpf_solver* s, int nsteps);
+/* Euler.Propagate / RK4.Propagate (pf/euler.go:55-59, pf/rk4.go:130-135) */
+int gpf_solver_stepper_propagate(gpf_solver* s, int nsteps);
+/* TimeStepper.GetTime (pf/euler.go:50-52, pf/rk4.go:139-141) */
+int gpf_solver_get_time(gpf_solver* s, double* t);
+/* copy field idx (Model.Fields order) to / from host complex128 (imag set to 0 / ignored) */
+int gpf_solver_get_field(gpf_solver* s, int idx, double* out, long n);
+int gpf_solver_set_field(gpf_solver* s, int idx, const double* in, long n);
+/* free energy F = sum_x sum_q poly[q] c^q + kappa/2 sum_x |grad c|^2 and mass sum_x c of field
+ * idx, in FP64 (compensated).  No reference counterpart: a monitor (cf. pf/monitor.go). */
+int gpf_solver_energy(gpf_solver* s, int idx, int npoly, const double* poly, double kappa,
+                      double* energy, double* mass);
+/* record the energy of field 0 before every step into a device series (fused path) */
+int gpf_solver_monitor_energy(gpf_solver* s, int npoly, const double* poly, double kappa);
+int gpf_solver_energy_series(gpf_solver* s, double* energy, double* mass, long cap, long* n);
+int gpf_solver_synchronize(gpf_solver* s);
+/* 1 if the fused single-field path runs, 0 for the general path */
+int gpf_solver_is_fused(const gpf_solver* s);
+/* per-pass device timing: enable, then read the summed milliseconds and launch counts
+ * of each pass recorded with HIP events on the solver's stream */
+int gpf_solver_enable_timing(gpf_solver* s, int on);
+int gpf_solver_pass_times(gpf_solver* s, int maxpass, double* ms, long* launches, char* names, int namelen);
+/* the HIP stream the solver launches on (hipStream_t) */
+void* gpf_solver_stream(gpf_solver* s);
+
+/* ---------------- pf.FourierTransform (pfutil.FFTWWrapper) ---------------- */
+
+/* pfutil.NewFFTW (pfutil/fftWrap.go:16-23) */
+int gpf_fft_create(const int* dims, int ndim, int precision, int device, gpf_fft** out);
+void gpf_fft_destroy(gpf_fft* f);
+/* FFTWWrapper.FFT / IFFT (pfutil/fftWrap.go:26-39): in place on host complex128, unnormalised */
+int gpf_fft_forward(gpf_fft* f, double* data, long n);
+int gpf_fft_inverse(gpf_fft* f, double* data, long n);
+/* FFTWWrapper.Freq (pfutil/fftWrap.go:57-74): out[ndim] */
+int gpf_fft_freq(const int* dims, int ndim, long i, double* out);
+/* FFTWWrapper.ConjugateNode (pfutil/fftWrap.go:78-95) */
+long gpf_fft_conjugate_node(const int* dims, int ndim, long i);
+
+/* ---------------- pf.Vandeven ---------------- */
+
+/* NewVandeven (pf/vandeven.go:13-27): 1000-entry table */
+int gpf_vandeven_table(int order, double* table);
+/* Vandeven.Eval (pf/vandeven.go:30-40) */
+double gpf_vandeven_eval(const double* table, double x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPF_H */
