@@ -172,8 +172,8 @@ struct Geometry {
 
 static int choose_tw(int n, int narr, size_t elem, int min_tw) {
     const int T = fft_threads_per_line(n);
-    int tw = 256 / T;
-    if (tw < 1) tw = 1;
+    int tw = 1;
+    while (tw * 2 * T <= 256) tw *= 2;  // lines per block: a power of two
     if (tw > 64) tw = 64;
     while (tw < min_tw && tw * 2 * T <= 1024) tw *= 2;
     while (tw * T > 1024 && tw > 1) tw /= 2;
@@ -373,7 +373,9 @@ private:
     std::vector<Buf<T>> rs_;
     // fused-path storage
     Buf<cpx<T>> X_, Y_, G_;
-    Buf<double> mass_;
+    Buf<double> mean_;          // FP32 fused path: FP64 mean of the field (the arrays hold c - mean)
+    bool split_mean_ = false;
+    double host_mean();
     // monitors
     bool mon_ = false;
     std::vector<double> mon_poly_;
@@ -417,7 +419,8 @@ Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gp
         X_.alloc(g_.nk);
         Y_.alloc(g_.nk);
         G_.alloc(g_.nk);
-        mass_.alloc(1);
+        mean_.alloc(1);
+        split_mean_ = (sizeof(T) == 4);
         pass_names_ = {"rows_fused", "pencils_fwd", "pencil_update", "pencils_inv"};
     } else {
         // compile every equation for the general path
@@ -662,11 +665,12 @@ void Engine<T>::set_field(int idx, const double* in) {
     double s = 0.0, c = 0.0;
     for (long j = 0; j < g_.n; ++j) {
         const double v = in[2 * j];
-        h[j] = (T)v;
         const double t = s + v;
         c += (std::fabs(s) >= std::fabs(v)) ? ((s - t) + v) : ((v - t) + s);
         s = t;
     }
+    const double mean = (fp_.ok && split_mean_) ? (s + c) / (double)g_.n : 0.0;
+    for (long j = 0; j < g_.n; ++j) h[j] = (T)(in[2 * j] - mean);
     if (fp_.ok) {
         T* tmp = rscratch(0);
         HIPCHK(hipMemcpyAsync(tmp, h.data(), g_.n * sizeof(T), hipMemcpyHostToDevice, st_));
@@ -681,8 +685,7 @@ void Engine<T>::set_field(int idx, const double* in) {
         a.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
         a.in_scale = T(1) / (T)g_.nx;
         HIPCHK(launch_rows_r2c<T>(a, st_));
-        const double mass = s + c;
-        HIPCHK(hipMemcpyAsync(mass_.get(), &mass, sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPCHK(hipMemcpyAsync(mean_.get(), &mean, sizeof(double), hipMemcpyHostToDevice, st_));
         HIPCHK(hipStreamSynchronize(st_));
     } else {
         HIPCHK(hipMemcpyAsync(F_[idx].get(), h.data(), g_.n * sizeof(T), hipMemcpyHostToDevice, st_));
@@ -713,10 +716,20 @@ void Engine<T>::get_field(int idx, double* out) {
         HIPCHK(hipMemcpyAsync(h.data(), F_[idx].get(), g_.n * sizeof(T), hipMemcpyDeviceToHost, st_));
     }
     HIPCHK(hipStreamSynchronize(st_));
+    const double mean = host_mean();
     for (long j = 0; j < g_.n; ++j) {
-        out[2 * j] = (double)h[j];
+        out[2 * j] = (double)h[j] + mean;
         out[2 * j + 1] = 0.0;
     }
+}
+
+template <typename T>
+double Engine<T>::host_mean() {
+    if (!(fp_.ok && split_mean_)) return 0.0;
+    double mu = 0.0;
+    HIPCHK(hipMemcpyAsync(&mu, mean_.get(), sizeof(double), hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+    return mu;
 }
 
 template <typename T>
@@ -746,7 +759,7 @@ void Engine<T>::energy(int idx, int npoly, const double* poly, double kappa, dou
     std::vector<double> hp(kMaxPoly, 0.0);
     for (int q = 0; q < npoly; ++q) hp[q] = poly[q];
     HIPCHK(hipMemcpyAsync(dp.get(), hp.data(), kMaxPoly * sizeof(double), hipMemcpyHostToDevice, st_));
-    HIPCHK(launch_real_sums<T>(c, g_.n, npoly, dp.get(), pf.get(), px.get(), nb, st_));
+    HIPCHK(launch_real_sums<T>(c, g_.n, host_mean(), npoly, dp.get(), pf.get(), px.get(), nb, st_));
     cpx<T>* kc = kscratch(kMaxArr);
     r2c({c}, {kc}, T(1));
     HIPCHK(launch_kgrad_energy<T>(kc, g_.kg, g_.nmain, g_.nk, pg.get(), nb, st_));
@@ -802,6 +815,7 @@ void Engine<T>::fused_step() {
     ra.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
     ra.npoly = (int)fp_.gpoly.size();
     for (int q = 0; q < ra.npoly; ++q) ra.gpoly[q] = fp_.gpoly[q];
+    ra.mean = split_mean_ ? mean_.get() : nullptr;
     const long nblk_a = rows_blocks(g_.rows, ra.TW);
     PencilUpdateArgs<T> u;
     const int axis = (g_.ndim == 3) ? 2 : 1;
@@ -824,7 +838,7 @@ void Engine<T>::fused_step() {
     u.src_lap = fp_.src_lap;
     u.has_src = fp_.has_src ? 1 : 0;
     u.filter = has_filter_ ? filter_.get() : nullptr;
-    u.mass = (sizeof(T) == 4) ? mass_.get() : nullptr;
+    u.mean = split_mean_ ? mean_.get() : nullptr;
     int tiles0 = 0;
     const long nblk_c = pencil_blocks(u.p.seg, u.p.nseg, u.p.TW, &tiles0);
     if (mon_) {

@@ -77,7 +77,8 @@ template <typename T> hipError_t launch_indicator(const T* phi, T* out, long n, 
 template <typename T> hipError_t launch_elastic_real(const T* phi, const T* S, double two_estar, T* out, long n, hipStream_t s);
 template <typename T> hipError_t launch_copy(const T* src, T* dst, long n, hipStream_t s);
 // per-block FP64 sums of f(x) = sum_q poly[q] x^q and of x over a real array
-template <typename T> hipError_t launch_real_sums(const T* x, long n, int npoly, const double* poly,
+// (x is shifted by the FP64 offset before both sums)
+template <typename T> hipError_t launch_real_sums(const T* x, long n, double offset, int npoly, const double* poly,
                                                   double* partial_f, double* partial_x, long nblocks, hipStream_t s);
 // per-block FP64 sums of w |k|^2 |C|^2 over a half spectrum
 template <typename T> hipError_t launch_kgrad_energy(const cpx<T>* C, KGrid g, long nmain, long nk,

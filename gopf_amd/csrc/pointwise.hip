@@ -281,12 +281,12 @@ struct KSum {
 constexpr int kRedThreads = 256;
 
 template <typename T>
-__global__ void __launch_bounds__(kRedThreads) k_real_sums(const T* x, long n, int npoly, const double* poly,
-                                                            double* pf, double* px) {
+__global__ void __launch_bounds__(kRedThreads) k_real_sums(const T* x, long n, double offset, int npoly,
+                                                            const double* poly, double* pf, double* px) {
     __shared__ double red[kRedThreads];
     KSum sf, sx;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double c = (double)x[i];
+        const double c = (double)x[i] + offset;
         double f = 0.0;
         for (int q = npoly - 1; q >= 0; --q) f = f * c + poly[q];
         sf.add(f);
@@ -381,9 +381,10 @@ hipError_t launch_copy(const T* src, T* dst, long n, hipStream_t s) {
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_real_sums(const T* x, long n, int npoly, const double* poly, double* pf, double* px, long nblocks,
-                            hipStream_t s) {
-    hipLaunchKernelGGL(k_real_sums<T>, dim3((unsigned)nblocks), dim3(kRedThreads), 0, s, x, n, npoly, poly, pf, px);
+hipError_t launch_real_sums(const T* x, long n, double offset, int npoly, const double* poly, double* pf, double* px,
+                            long nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_real_sums<T>, dim3((unsigned)nblocks), dim3(kRedThreads), 0, s, x, n, offset, npoly, poly,
+                       pf, px);
     return hipGetLastError();
 }
 template <typename T>
@@ -407,8 +408,8 @@ hipError_t launch_reduce2(const double* a, long na, double ca, const double* b, 
     template hipError_t launch_indicator<T>(const T*, T*, long, hipStream_t);                                 \
     template hipError_t launch_elastic_real<T>(const T*, const T*, double, T*, long, hipStream_t);            \
     template hipError_t launch_copy<T>(const T*, T*, long, hipStream_t);                                      \
-    template hipError_t launch_real_sums<T>(const T*, long, int, const double*, double*, double*, long,       \
-                                            hipStream_t);                                                     \
+    template hipError_t launch_real_sums<T>(const T*, long, double, int, const double*, double*, double*,     \
+                                            long, hipStream_t);                                               \
     template hipError_t launch_kgrad_energy<T>(const cpx<T>*, KGrid, long, long, double*, long, hipStream_t);
 GPF_INST(float)
 GPF_INST(double)

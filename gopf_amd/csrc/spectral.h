@@ -69,7 +69,8 @@ struct PencilUpdateArgs {
     int src_lap = 0;                     // LAP power of the explicit source
     int has_src = 0;
     const double* filter = nullptr;      // Vandeven table (1000 entries) or null
-    double* mass = nullptr;              // FP64 k = 0 mode (in/out) or null
+    double* mean = nullptr;              // FP64 k = 0 mode: the field mean (in/out), or null when the
+                                         // k = 0 coefficient is carried by the arrays themselves
     double* egrad_partial = nullptr;     // per-block FP64 sum of w |k|^2 |C|^2 or null
 };
 
@@ -113,6 +114,7 @@ struct RowFusedArgs {
     cpx<T>* g_main = nullptr;
     cpx<T>* g_nyq = nullptr;
     T* c_out = nullptr;                  // optional real-space copy of c
+    const double* mean = nullptr;        // FP64 mean held outside the arrays (state = c - mean), or null
     long rows = 0;
     int nx = 0;
     int TW = 16;

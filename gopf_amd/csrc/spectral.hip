@@ -226,12 +226,12 @@ __global__ void k_pencil_update(PencilUpdateArgs<T> u) {
             eg += w * (-L) * ((double)c.x * (double)c.x + (double)c.y * (double)c.y);
         }
         cpx<T> r;
-        if (u.mass && kc[0] == 0 && kc[1] == 0 && kc[2] == 0) {
-            // k = 0: the conserved mode advances in FP64 (mass is sum_x c)
-            const double m0 = *u.mass;
-            const double mnew = m1 * m0 + m2 * (double)gs.x;
-            *u.mass = mnew;
-            r = mk<T>((T)(mnew * u.scale), T(0));
+        if (u.mean && kc[0] == 0 && kc[1] == 0 && kc[2] == 0) {
+            // k = 0 advances in FP64: the arrays hold c - mean, so their k = 0 coefficient stays 0
+            // and mean_new = m1(0) mean + m2(0) G(0) / N
+            const double mu = *u.mean;
+            *u.mean = m1 * mu + m2 * (double)gs.x * u.scale;
+            r = mk<T>(T(0), T(0));
         } else {
             const T s1 = (T)(m1 * u.scale), s2 = (T)(m2 * u.scale);
             r = mk<T>(s1 * c.x + s2 * gs.x, s1 * c.y + s2 * gs.y);
@@ -371,16 +371,19 @@ __global__ void k_rows_fused(RowFusedArgs<T> a) {
     line_fft<T, N, +1>(L + l, TWP, n, t, a.tw);
     // real space: c, g(c), optional output and FP64 partial sums
     KSum sb, sm;
+    const double mu = a.mean ? *a.mean : 0.0;
+    const T muT = (T)mu;
     for (int idx = threadIdx.x; idx < TW * n; idx += blockDim.x) {
         const int ll = idx / n, p = idx - (idx / n) * n;
-        const T c = L[p * TWP + ll].x;
+        const T dc = L[p * TWP + ll].x;   // c - mean (mean = 0 when the arrays carry k = 0)
+        const T c = dc + muT;
         T gv = T(0);
         for (int q = a.npoly - 1; q >= 0; --q) gv = gv * c + (T)a.gpoly[q];
-        L[p * TWP + ll] = mk<T>(c, gv);
+        L[p * TWP + ll] = mk<T>(dc, gv);
         if (ll < nvalid) {
             if (a.c_out) a.c_out[(row0 + ll) * n + p] = c;
             if (a.bulk_partial) {
-                const double cd = (double)c;
+                const double cd = (double)dc + mu;
                 double f = 0.0;
                 for (int q = a.nbulk - 1; q >= 0; --q) f = f * cd + a.bulk[q];
                 sb.add(f);
