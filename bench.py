@@ -48,7 +48,7 @@ def build_solver(G, W, n, dims, c0, precision, dt):
         m.AddScalar(k, v)
     m.AddEquation(W.CH_EQUATION)
     m.Init()
-    s = G.Solver(m, dims, dt, precision=precision)
+    s = G.Solver(m, dims, dt, precision=precision, autotune=True)
     if not s.fused:
         raise RuntimeError("the Cahn-Hilliard model did not compile to the fused path")
     return s
@@ -108,7 +108,7 @@ def main() -> int:
     c0 = W.ch_initial(1, n)
     print(f"# building FP32-bulk solver {dims}", file=sys.stderr, flush=True)
     s32 = build_solver(G, W, n, dims, c0, "fp32", args.dt)
-    s32.StepperPropagate(args.warmup)
+    s32.StepperPropagate(args.warmup)   # the first step runs the lines-per-block autotuner
     el32 = time_steps(s32, args.steps)
     # per-pass device times over a second timed region
     tsteps = min(args.steps, 200)
@@ -148,7 +148,7 @@ def main() -> int:
     step_bytes = sum(pass_bytes(k, N, N, N, 4) for k in passes)
     roof = {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": algo,
-            "avg_launch_ms": ms / cnt,
+            "avg_launch_ms": ms / cnt, "lines_per_block": s32.Tuning(),
             "passes_ms_per_launch": {k: v[0] / v[1] for k, v in passes.items()},
             "step_effective_GBs": step_bytes * args.steps / el32 / 1e9,
             "step_frac": step_bytes * args.steps / el32 / 1e9 / HBM_PEAK_GBS}

@@ -38,7 +38,7 @@ class GpfError(RuntimeError):
 
 class _Options(C.Structure):
     _fields_ = [("precision", C.c_int), ("stepper", C.c_int), ("filter_order", C.c_int),
-                ("fused", C.c_int), ("device", C.c_int)]
+                ("fused", C.c_int), ("device", C.c_int), ("autotune", C.c_int)]
 
 
 class _Term(C.Structure):
@@ -97,6 +97,7 @@ def _declare(L) -> None:
         "gpf_solver_enable_timing": (C.c_int, [_vp, C.c_int]),
         "gpf_solver_pass_times": (C.c_int, [_vp, C.c_int, _dp, _lp, C.c_char_p, C.c_int]),
         "gpf_solver_stream": (_vp, [_vp]),
+        "gpf_solver_tuning": (C.c_int, [_vp, _ip, C.c_int]),
         "gpf_fft_create": (C.c_int, [_ip, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
         "gpf_fft_destroy": (None, [_vp]),
         "gpf_fft_forward": (C.c_int, [_vp, _dp, C.c_long]),
@@ -297,7 +298,8 @@ class Solver:
     """pf.Solver with a device TimeStepper (solver.go:29-134)."""
 
     def __init__(self, model: Model, domainSize: Sequence[int], dt: float, stepper: str = "euler",
-                 precision: str = "fp64", filter_order: int = 0, fused: bool = True, device: int = 0):
+                 precision: str = "fp64", filter_order: int = 0, fused: bool = True, device: int = 0,
+                 autotune: bool = False):
         o = _Options()
         lib().gpf_options_default(C.byref(o))
         o.precision = GPF_FP32 if precision in ("fp32", "float32") else GPF_FP64
@@ -307,6 +309,7 @@ class Solver:
         o.filter_order = int(filter_order)
         o.fused = 1 if fused else 0
         o.device = int(device)
+        o.autotune = 1 if autotune else 0
         h = _vp()
         self.dims = [int(d) for d in domainSize]
         self.n = int(np.prod(self.dims))
@@ -393,6 +396,11 @@ class Solver:
             nm = names.raw[i * 64:(i + 1) * 64].split(b"\0", 1)[0].decode()
             out[nm] = (float(ms[i]), int(cnt[i]))
         return out
+
+    def Tuning(self) -> Dict[str, int]:
+        v = (C.c_int * 4)()
+        lib().gpf_solver_tuning(self._h, v, 4)
+        return {"rows_fused": v[0], "pencils_fwd": v[1], "pencil_update": v[2], "pencils_inv": v[3]}
 
     def Stream(self) -> int:
         return int(lib().gpf_solver_stream(self._h) or 0)

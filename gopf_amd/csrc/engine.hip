@@ -170,14 +170,20 @@ struct Geometry {
     }
 };
 
+static bool tw_fits(int n, int tw, size_t elem) {
+    const int T = fft_threads_per_line(n);
+    return tw * T <= kMaxBlockThreads && (size_t)(n + n / 16 + 1) * (tw + 1) * elem <= 150 * 1024;
+}
+
 static int choose_tw(int n, int narr, size_t elem, int min_tw) {
     const int T = fft_threads_per_line(n);
     int tw = 1;
     while (tw * 2 * T <= 256) tw *= 2;  // lines per block: a power of two
     if (tw > 64) tw = 64;
-    while (tw < min_tw && tw * 2 * T <= 1024) tw *= 2;
-    while (tw * T > 1024 && tw > 1) tw /= 2;
-    while (tw > 1 && (size_t)narr * n * (tw + 1) * elem > 80 * 1024) tw /= 2;
+    while (tw < min_tw && tw * 2 * T <= 256) tw *= 2;
+    while (tw * T > kMaxBlockThreads && tw > 1) tw /= 2;
+    (void)narr;
+    while (tw > 1 && (size_t)(n + n / 16 + 1) * (tw + 1) * elem > 80 * 1024) tw /= 2;
     return tw;
 }
 
@@ -308,6 +314,7 @@ public:
     virtual int pass_times(int maxpass, double* ms, long* launches, char* names, int namelen) = 0;
     virtual hipStream_t stream() const = 0;
     virtual long size() const = 0;
+    virtual int tuning(int* tws, int maxn) = 0;
 };
 
 template <typename T>
@@ -328,6 +335,7 @@ public:
     int pass_times(int maxpass, double* ms, long* launches, char* names, int namelen) override;
     hipStream_t stream() const override { return st_; }
     long size() const override { return g_.n; }
+    int tuning(int* tws, int maxn) override;
 
 private:
     // transforms
@@ -342,6 +350,12 @@ private:
     void tend(int pass);
     // fused path
     void fused_step();
+    RowFusedArgs<T> rows_args(const cpx<T>* X, cpx<T>* Y, cpx<T>* Gb, double* mean, int tw);
+    PencilUpdateArgs<T> update_args(cpx<T>* Y, cpx<T>* Gb, double* mean, int tw);
+    void pencils_tw(const std::vector<cpx<T>*>& dst, int axis, int dir, int tw);
+    void autotune();
+    bool autotune_ = false, tuned_ = false;
+    int tw_rows_ = 0, tw_pf_ = 0, tw_pu_ = 0, tw_pi_ = 0;
     // general path
     void general_step_euler();
     void general_step_rk4();
@@ -393,7 +407,7 @@ private:
 
 template <typename T>
 Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gpf_options& o)
-    : dt_(dt), stepper_(o.stepper), mcopy_(m) {
+    : dt_(dt), stepper_(o.stepper), mcopy_(m), autotune_(o.autotune != 0) {
     g_.init(dims, ndim);
     HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
     nf_ = (int)m.field_names.size();
@@ -801,35 +815,39 @@ long Engine<T>::series(double* e, double* m, long cap) {
 // fused single-field step
 // ---------------------------------------------------------------------------
 template <typename T>
-void Engine<T>::fused_step() {
+RowFusedArgs<T> Engine<T>::rows_args(const cpx<T>* X, cpx<T>* Y, cpx<T>* Gb, double* mean, int tw) {
     RowFusedArgs<T> ra;
-    ra.tw = tw(g_.nx);
-    ra.s_main = X_.get();
-    ra.s_nyq = X_.get() + g_.nmain;
-    ra.c_main = Y_.get();
-    ra.c_nyq = Y_.get() + g_.nmain;
-    ra.g_main = G_.get();
-    ra.g_nyq = G_.get() + g_.nmain;
+    ra.tw = this->tw(g_.nx);
+    ra.s_main = X;
+    ra.s_nyq = X + g_.nmain;
+    ra.c_main = Y;
+    ra.c_nyq = Y + g_.nmain;
+    ra.g_main = Gb;
+    ra.g_nyq = Gb + g_.nmain;
     ra.rows = g_.rows;
     ra.nx = g_.nx;
-    ra.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
+    ra.TW = tw > 0 ? tw : choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
     ra.npoly = (int)fp_.gpoly.size();
     for (int q = 0; q < ra.npoly; ++q) ra.gpoly[q] = fp_.gpoly[q];
-    ra.mean = split_mean_ ? mean_.get() : nullptr;
-    const long nblk_a = rows_blocks(g_.rows, ra.TW);
+    ra.mean = split_mean_ ? mean : nullptr;
+    return ra;
+}
+
+template <typename T>
+PencilUpdateArgs<T> Engine<T>::update_args(cpx<T>* Y, cpx<T>* Gb, double* mean, int tw) {
     PencilUpdateArgs<T> u;
     const int axis = (g_.ndim == 3) ? 2 : 1;
     const int nu = (axis == 2) ? g_.nz : g_.ny;
-    u.p.tw = tw(nu);
+    u.p.tw = this->tw(nu);
     u.p.narr = fp_.has_src ? 2 : 1;
-    u.p.arr[0][0] = Y_.get();
-    u.p.arr[0][1] = Y_.get() + g_.nmain;
-    u.p.arr[1][0] = G_.get();
-    u.p.arr[1][1] = G_.get() + g_.nmain;
+    u.p.arr[0][0] = Y;
+    u.p.arr[0][1] = Y + g_.nmain;
+    u.p.arr[1][0] = Gb;
+    u.p.arr[1][1] = Gb + g_.nmain;
     u.p.seg[0] = (axis == 2) ? seg_z_main(g_) : seg_y_main(g_);
     u.p.seg[1] = (axis == 2) ? seg_z_nyq(g_) : seg_y_nyq(g_);
     u.p.nseg = 2;
-    u.p.TW = choose_tw(nu, u.p.narr, sizeof(cpx<T>), 8);
+    u.p.TW = tw > 0 ? tw : choose_tw(nu, u.p.narr, sizeof(cpx<T>), 8);
     u.g = g_.kg;
     u.dt = dt_;
     u.scale = 1.0 / (double)g_.n;
@@ -838,7 +856,92 @@ void Engine<T>::fused_step() {
     u.src_lap = fp_.src_lap;
     u.has_src = fp_.has_src ? 1 : 0;
     u.filter = has_filter_ ? filter_.get() : nullptr;
-    u.mean = split_mean_ ? mean_.get() : nullptr;
+    u.mean = split_mean_ ? mean : nullptr;
+    return u;
+}
+
+template <typename T>
+void Engine<T>::pencils_tw(const std::vector<cpx<T>*>& dst, int axis, int dir, int tw) {
+    const int n = (axis == 1) ? g_.ny : g_.nz;
+    PencilArgs<T> a;
+    a.tw = this->tw(n);
+    a.narr = (int)dst.size();
+    a.seg[0] = (axis == 1) ? seg_y_main(g_) : seg_z_main(g_);
+    a.seg[1] = (axis == 1) ? seg_y_nyq(g_) : seg_z_nyq(g_);
+    a.nseg = 2;
+    a.TW = tw > 0 ? tw : choose_tw(n, a.narr, sizeof(cpx<T>), 8);
+    for (int q = 0; q < a.narr; ++q) {
+        a.arr[q][0] = dst[q];
+        a.arr[q][1] = dst[q] + g_.nmain;
+    }
+    HIPCHK(launch_pencils<T>(a, n, dir, st_));
+}
+
+// Lines-per-block autotuner of the fused step: every pass is timed with HIP
+// events for each admissible tile width on scratch copies of the state.
+template <typename T>
+void Engine<T>::autotune() {
+    if (!fp_.ok || tuned_) return;
+    Buf<cpx<T>> X(g_.nk), Y(g_.nk), Gb(g_.nk);
+    Buf<double> mu(1);
+    HIPCHK(hipMemcpyAsync(X.get(), X_.get(), g_.nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+    HIPCHK(hipMemcpyAsync(mu.get(), mean_.get(), sizeof(double), hipMemcpyDeviceToDevice, st_));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const int nu = (g_.ndim == 3) ? g_.nz : g_.ny;
+    auto time_it = [&](auto&& launch) {
+        launch();
+        launch();
+        HIPCHK(hipEventRecord(e0, st_));
+        const int reps = 5;
+        for (int r = 0; r < reps; ++r) launch();
+        HIPCHK(hipEventRecord(e1, st_));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        return ms / reps;
+    };
+    const int cand[] = {4, 8, 16, 32, 64};
+    auto best_of = [&](int n, auto&& launch_with) {
+        int best = 0;
+        float bt = 1e30f;
+        for (int tw : cand) {
+            if (!tw_fits(n, tw, sizeof(cpx<T>))) continue;
+            const float t = time_it([&] { launch_with(tw); });
+            if (t < bt) { bt = t; best = tw; }
+        }
+        return best;
+    };
+    tw_rows_ = best_of(g_.nx, [&](int tw) { HIPCHK(launch_rows_fused<T>(rows_args(X.get(), Y.get(), Gb.get(), mu.get(), tw), st_)); });
+    if (g_.ndim == 3) {
+        tw_pf_ = best_of(g_.ny, [&](int tw) {
+            if (fp_.has_src) pencils_tw({Y.get(), Gb.get()}, 1, -1, tw);
+            else pencils_tw({Y.get()}, 1, -1, tw);
+        });
+        tw_pi_ = best_of(g_.ny, [&](int tw) { pencils_tw({Y.get()}, 1, +1, tw); });
+    }
+    tw_pu_ = best_of(nu, [&](int tw) { HIPCHK(launch_pencil_update<T>(update_args(Y.get(), Gb.get(), mu.get(), tw), nu, st_)); });
+    HIPCHK(hipEventDestroy(e0));
+    HIPCHK(hipEventDestroy(e1));
+    HIPCHK(hipStreamSynchronize(st_));
+    tuned_ = true;
+}
+
+template <typename T>
+int Engine<T>::tuning(int* tws, int maxn) {
+    const int v[4] = {tw_rows_, tw_pf_, tw_pu_, tw_pi_};
+    for (int i = 0; i < 4 && i < maxn; ++i) tws[i] = v[i];
+    return 4;
+}
+
+template <typename T>
+void Engine<T>::fused_step() {
+    if (autotune_ && !tuned_) autotune();
+    RowFusedArgs<T> ra = rows_args(X_.get(), Y_.get(), G_.get(), mean_.get(), tw_rows_);
+    const long nblk_a = rows_blocks(g_.rows, ra.TW);
+    const int nu = (g_.ndim == 3) ? g_.nz : g_.ny;
+    PencilUpdateArgs<T> u = update_args(Y_.get(), G_.get(), mean_.get(), tw_pu_);
     int tiles0 = 0;
     const long nblk_c = pencil_blocks(u.p.seg, u.p.nseg, u.p.TW, &tiles0);
     if (mon_) {
@@ -864,8 +967,8 @@ void Engine<T>::fused_step() {
     tend(0);
     if (g_.ndim == 3) {
         tbegin(1);
-        if (fp_.has_src) pencils({Y_.get(), G_.get()}, {}, 1, -1);
-        else pencils({Y_.get()}, {}, 1, -1);
+        if (fp_.has_src) pencils_tw({Y_.get(), G_.get()}, 1, -1, tw_pf_);
+        else pencils_tw({Y_.get()}, 1, -1, tw_pf_);
         tend(1);
     }
     tbegin(2);
@@ -873,7 +976,7 @@ void Engine<T>::fused_step() {
     tend(2);
     if (g_.ndim == 3) {
         tbegin(3);
-        pencils({Y_.get()}, {}, 1, +1);
+        pencils_tw({Y_.get()}, 1, +1, tw_pi_);
         tend(3);
     }
     if (mon_) {
@@ -1399,6 +1502,7 @@ void gpf_options_default(gpf_options* o) {
     o->filter_order = 0;
     o->fused = 1;
     o->device = 0;
+    o->autotune = 0;
 }
 
 int gpf_solver_create(gpf_model* m, const int* dims, int ndim, double dt, const gpf_options* o, gpf_solver** out) {
@@ -1528,6 +1632,11 @@ int gpf_solver_pass_times(gpf_solver* s, int maxpass, double* ms, long* launches
         np = s->eng->pass_times(maxpass, ms, launches, names, namelen);
     });
     return rc ? -rc : np;
+}
+
+int gpf_solver_tuning(gpf_solver* s, int* tws, int maxn) {
+    if (!s || !s->eng || !tws) return -GPF_EINVAL;
+    return s->eng->tuning(tws, maxn);
 }
 
 void* gpf_solver_stream(gpf_solver* s) { return (s && s->eng) ? (void*)s->eng->stream() : nullptr; }

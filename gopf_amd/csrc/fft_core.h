@@ -192,86 +192,115 @@ __device__ __forceinline__ void dftR(cpx<T>* a) {
 }
 
 // ---------------------------------------------------------------------------
-// one Stockham stage on a line held in LDS
-//   line element p lives at buf[p * es]; thread t in [0, T) of the line.
-//   tw: table of exp(-2 pi i q / N), q in [0, N)
+// register-resident transform of one line
+//
+// "Canonical" register order: thread t (of T per line) holds v[q] = x[t + T q],
+// q in [0, E).  Stage s of radix R runs E/R butterflies per thread; butterfly
+// b has index j = t + b T, reads x[j + m N/R] and writes
+// y[(j / NS) NS R + (j % NS) + m NS] (Stockham, NS = span before the stage).
+// The first stage reads canonical registers directly and the last stage
+// produces canonical registers, so loads and stores of whole lines need no
+// LDS staging; stages exchange through LDS (padded one slot per 16).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ int lds_pad(int p) { return p + (p >> 4); }
+constexpr int lds_line(int n) { return n + n / 16 + 1; }
+
 template <typename T, int N, int S, int DIR>
-__device__ __forceinline__ void stage(cpx<T>* buf, int es, int t, const cpx<T>* __restrict__ tw) {
+__device__ __forceinline__ void stage_compute(cpx<T>* a, int t, const cpx<T>* __restrict__ tw) {
     constexpr int R = Sched<N>::R[S];
     constexpr int E = Sched<N>::E;
     constexpr int TT = Sched<N>::T;
-    constexpr int NB = E / R;       // butterflies per thread
-    constexpr int NR = N / R;       // input stride
+    constexpr int NB = E / R;
     constexpr int NS = Span<N, S>::v;
     constexpr int TWSTEP = N / (NS * R);
-    cpx<T> v[E];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        const int j = t + b * TT;
-#pragma unroll
-        for (int m = 0; m < R; ++m) v[b * R + m] = buf[(j + m * NR) * es];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        const int j = t + b * TT;
-        const int k = (NS == 1) ? 0 : (j % NS);
         if constexpr (NS > 1) {
+            const int j = t + b * TT;
+            const int k = j % NS;
 #pragma unroll
             for (int m = 1; m < R; ++m) {
                 cpx<T> w = tw[(k * m * TWSTEP) % N];
                 if (DIR > 0) w.y = -w.y;
-                v[b * R + m] = mul(v[b * R + m], w);
+                a[b * R + m] = mul(a[b * R + m], w);
             }
         }
-        dftR<R, DIR>(&v[b * R]);
-        const int base = (NS == 1) ? j * R : ((j / NS) * NS * R + k);
-#pragma unroll
-        for (int m = 0; m < R; ++m) buf[(base + m * NS) * es] = v[b * R + m];
+        dftR<R, DIR>(&a[b * R]);
     }
-    __syncthreads();
 }
 
 template <typename T, int N, int S, int DIR>
-__device__ __forceinline__ void stages_from(cpx<T>* buf, int es, int t, const cpx<T>* __restrict__ tw) {
+__device__ __forceinline__ void exchange_and_compute(cpx<T>* a, cpx<T>* lds, int es, int t,
+                                                     const cpx<T>* __restrict__ tw) {
     if constexpr (S < Sched<N>::NR) {
-        stage<T, N, S, DIR>(buf, es, t, tw);
-        stages_from<T, N, S + 1, DIR>(buf, es, t, tw);
-    }
-}
-
-// Full in-place transform of a line in LDS.  All threads of the block must
-// call it (it synchronises the block); threads with t >= T idle through.
-template <typename T, int N, int DIR>
-__device__ __forceinline__ void fft_line(cpx<T>* buf, int es, int t, bool active,
-                                         const cpx<T>* __restrict__ tw) {
-    // stage() synchronises internally; inactive threads must still reach the barriers
-    if (active) {
-        stages_from<T, N, 0, DIR>(buf, es, t, tw);
-    } else {
-        for (int s = 0; s < 2 * Sched<N>::NR; ++s) __syncthreads();
-    }
-}
-
-// Brute-force DFT of a line of runtime length n (n threads per line, t = output index).
-template <typename T, int DIR>
-__device__ __forceinline__ void dft_line_rt(cpx<T>* buf, int es, int n, int t, bool active,
-                                            const cpx<T>* __restrict__ tw) {
-    cpx<T> acc = mk<T>(T(0), T(0));
-    if (active) {
-        int q = 0;
-        for (int m = 0; m < n; ++m) {
-            cpx<T> w = tw[q];
-            if (DIR > 0) w.y = -w.y;
-            acc = add(acc, mul(buf[m * es], w));
-            q += t;
-            if (q >= n) q -= n;
+        constexpr int TT = Sched<N>::T;
+        constexpr int E = Sched<N>::E;
+        constexpr int RP = Sched<N>::R[S - 1];
+        constexpr int NBP = E / RP;
+        constexpr int NSP = Span<N, S - 1>::v;
+        constexpr int R = Sched<N>::R[S];
+        constexpr int NB = E / R;
+#pragma unroll
+        for (int b = 0; b < NBP; ++b) {
+            const int j = t + b * TT;
+            const int base = (NSP == 1) ? j * RP : ((j / NSP) * NSP * RP + (j % NSP));
+#pragma unroll
+            for (int m = 0; m < RP; ++m) lds[lds_pad(base + m * NSP) * es] = a[b * RP + m];
         }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int j = t + b * TT;
+#pragma unroll
+            for (int m = 0; m < R; ++m) a[b * R + m] = lds[lds_pad(j + m * (N / R)) * es];
+        }
+        __syncthreads();
+        stage_compute<T, N, S, DIR>(a, t, tw);
+        exchange_and_compute<T, N, S + 1, DIR>(a, lds, es, t, tw);
+    }
+}
+
+// In-place transform of a line held in canonical registers v[E].  lds is this
+// line's exchange buffer (element p at lds[lds_pad(p) * es]).  Every thread of
+// the block must call it: it synchronises the block between stages.
+template <typename T, int N, int DIR>
+__device__ __forceinline__ void fft_canon(cpx<T>* v, cpx<T>* lds, int es, int t, const cpx<T>* __restrict__ tw) {
+    constexpr int E = Sched<N>::E;
+    constexpr int NR = Sched<N>::NR;
+    constexpr int R0 = Sched<N>::R[0];
+    constexpr int NB0 = E / R0;
+    constexpr int RL = Sched<N>::R[NR - 1];
+    constexpr int NBL = E / RL;
+    cpx<T> a[E];
+#pragma unroll
+    for (int b = 0; b < NB0; ++b)
+#pragma unroll
+        for (int m = 0; m < R0; ++m) a[b * R0 + m] = v[b + m * NB0];
+    stage_compute<T, N, 0, DIR>(a, t, tw);
+    exchange_and_compute<T, N, 1, DIR>(a, lds, es, t, tw);
+#pragma unroll
+    for (int b = 0; b < NBL; ++b)
+#pragma unroll
+        for (int m = 0; m < RL; ++m) v[b + m * NBL] = a[b * RL + m];
+}
+
+// Brute-force DFT for runtime lengths n <= 64: one thread per element (T = n, E = 1).
+template <typename T, int DIR>
+__device__ __forceinline__ void dft_canon_rt(cpx<T>* v, cpx<T>* lds, int es, int n, int t,
+                                             const cpx<T>* __restrict__ tw) {
+    lds[lds_pad(t) * es] = v[0];
+    __syncthreads();
+    cpx<T> acc = mk<T>(T(0), T(0));
+    int q = 0;
+    for (int m = 0; m < n; ++m) {
+        cpx<T> w = tw[q];
+        if (DIR > 0) w.y = -w.y;
+        acc = add(acc, mul(lds[lds_pad(m) * es], w));
+        q += t;
+        if (q >= n) q -= n;
     }
     __syncthreads();
-    if (active) buf[t * es] = acc;
-    __syncthreads();
+    v[0] = acc;
 }
 
 }  // namespace gpf

@@ -20,6 +20,7 @@ template <typename T>
 struct cpx;
 
 constexpr int kMaxArr = 4;
+constexpr int kMaxBlockThreads = 512;  // threads per block of every transform kernel
 constexpr int kMaxPoly = 9;     // polynomial degree < kMaxPoly
 constexpr int kMaxKTerms = 8;   // k-space multiplier terms
 

@@ -26,13 +26,21 @@ namespace gpf {
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
+constexpr int kMaxBlock = kMaxBlockThreads;
+
+// elements per thread and threads per line of the transform of length n
+template <int N>
+struct LineShape {
+    static constexpr int E = (N > 0) ? Sched<(N > 0 ? N : 2)>::E : 1;
+    static constexpr int T = (N > 0) ? Sched<(N > 0 ? N : 2)>::T : 0;  // 0: runtime (= n)
+};
 
 template <typename T, int N, int DIR>
-__device__ __forceinline__ void line_fft(cpx<T>* buf, int es, int n, int t, const cpx<T>* __restrict__ tw) {
+__device__ __forceinline__ void line_fft(cpx<T>* v, cpx<T>* lds, int es, int n, int t, const cpx<T>* __restrict__ tw) {
     if constexpr (N > 0) {
-        fft_line<T, N, DIR>(buf, es, t, true, tw);
+        fft_canon<T, N, DIR>(v, lds, es, t, tw);
     } else {
-        dft_line_rt<T, DIR>(buf, es, n, t, true, tw);
+        dft_canon_rt<T, DIR>(v, lds, es, n, t, tw);
     }
 }
 
@@ -97,291 +105,332 @@ struct TileInfo {
     int ia, b0;   // a index and first b of the tile
     long base;    // element offset of line (ia, b0)
     int nvalid;   // valid lines in the tile
+    long sb, es;  // line and element strides of the segment
+    int ksrc0, ksrc1, ksrc2, kfix0, kfix1, kfix2;
 };
 
+// The segment is chosen with selects (no dynamic indexing of the argument
+// struct, which would copy it to scratch memory).
 template <typename T>
 __device__ __forceinline__ TileInfo decode_tile(const PencilArgs<T>& a) {
     TileInfo ti;
     int tile = blockIdx.x;
-    ti.s = 0;
-    if (tile >= a.tiles0) {
-        tile -= a.tiles0;
-        ti.s = 1;
-    }
-    const Seg& g = a.seg[ti.s];
-    const int tpa = (g.nb + a.TW - 1) / a.TW;
+    const bool s1 = tile >= a.tiles0;
+    ti.s = s1 ? 1 : 0;
+    if (s1) tile -= a.tiles0;
+    const int nb = s1 ? a.seg[1].nb : a.seg[0].nb;
+    const long sa = s1 ? a.seg[1].sa : a.seg[0].sa;
+    ti.sb = s1 ? a.seg[1].sb : a.seg[0].sb;
+    ti.es = s1 ? a.seg[1].es : a.seg[0].es;
+    ti.ksrc0 = s1 ? a.seg[1].ksrc[0] : a.seg[0].ksrc[0];
+    ti.ksrc1 = s1 ? a.seg[1].ksrc[1] : a.seg[0].ksrc[1];
+    ti.ksrc2 = s1 ? a.seg[1].ksrc[2] : a.seg[0].ksrc[2];
+    ti.kfix0 = s1 ? a.seg[1].kfix[0] : a.seg[0].kfix[0];
+    ti.kfix1 = s1 ? a.seg[1].kfix[1] : a.seg[0].kfix[1];
+    ti.kfix2 = s1 ? a.seg[1].kfix[2] : a.seg[0].kfix[2];
+    const int tpa = (nb + a.TW - 1) / a.TW;
     ti.ia = tile / tpa;
     ti.b0 = (tile - ti.ia * tpa) * a.TW;
-    ti.base = (long)ti.ia * g.sa + (long)ti.b0 * g.sb;
-    ti.nvalid = min(a.TW, g.nb - ti.b0);
+    ti.base = (long)ti.ia * sa + (long)ti.b0 * ti.sb;
+    ti.nvalid = min(a.TW, nb - ti.b0);
     return ti;
 }
 
-// global tile <-> LDS (layout lds[p * TWP + l])
-template <typename T>
-__device__ __forceinline__ void tile_load(const cpx<T>* __restrict__ src, const Seg& g, const TileInfo& ti,
-                                          cpx<T>* L, int n, int TW, int TWP) {
-    const int nthr = blockDim.x;
-    const int total = TW * n;
-    if (g.es == 1) {
-        for (int idx = threadIdx.x; idx < total; idx += nthr) {
-            const int l = idx / n, p = idx - (idx / n) * n;
-            cpx<T> v = mk<T>(T(0), T(0));
-            if (l < ti.nvalid) v = src[ti.base + (long)l * g.sb + p];
-            L[p * TWP + l] = v;
-        }
-    } else {
-        for (int idx = threadIdx.x; idx < total; idx += nthr) {
-            const int l = idx & (TW - 1), p = idx / TW;
-            cpx<T> v = mk<T>(T(0), T(0));
-            if (l < ti.nvalid) v = src[ti.base + (long)l * g.sb + (long)p * g.es];
-            L[p * TWP + l] = v;
-        }
-    }
+template <typename P>
+__device__ __forceinline__ P pick(P a0, P a1, int s) { return s ? a1 : a0; }
+
+__device__ __forceinline__ int ksel(int src, int fix, int a, int b, int p) {
+    return (src == 0) ? a : (src == 1) ? b : (src == 2) ? p : fix;
 }
 
-template <typename T>
-__device__ __forceinline__ void tile_store(cpx<T>* __restrict__ dst, const Seg& g, const TileInfo& ti,
-                                           const cpx<T>* L, int n, int TW, int TWP) {
-    const int nthr = blockDim.x;
-    const int total = TW * n;
-    if (g.es == 1) {
-        for (int idx = threadIdx.x; idx < total; idx += nthr) {
-            const int l = idx / n, p = idx - (idx / n) * n;
-            if (l < ti.nvalid) dst[ti.base + (long)l * g.sb + p] = L[p * TWP + l];
-        }
+// Thread -> (line l, thread-in-line t).  Contiguous lines (es == 1) put t
+// fastest so a wave reads consecutive elements of a line; strided lines put
+// l fastest so a wave reads adjacent lines (consecutive addresses).
+struct LineMap {
+    int l, t;
+    int es;          // LDS element stride
+    int off;         // LDS offset of this thread's line
+};
+
+__device__ __forceinline__ LineMap line_map(bool contiguous, int TW, int TT, int n) {
+    LineMap m;
+    if (contiguous) {
+        m.l = threadIdx.x / TT;
+        m.t = threadIdx.x - m.l * TT;
+        m.es = 1;
+        m.off = m.l * lds_line(n);
     } else {
-        for (int idx = threadIdx.x; idx < total; idx += nthr) {
-            const int l = idx & (TW - 1), p = idx / TW;
-            if (l < ti.nvalid) dst[ti.base + (long)l * g.sb + (long)p * g.es] = L[p * TWP + l];
-        }
+        m.l = threadIdx.x & (TW - 1);
+        m.t = threadIdx.x / TW;
+        m.es = TW + 1;
+        m.off = m.l;
     }
+    return m;
+}
+
+__host__ __device__ constexpr size_t lds_elems(int n, int TW) {
+    // both layouts fit in lds_line(n) * (TW + 1) elements
+    return (size_t)lds_line(n) * (size_t)(TW + 1);
+}
+
+template <typename T, int E>
+__device__ __forceinline__ void load_line(cpx<T>* v, const cpx<T>* __restrict__ src, long base, long es, int t, int TT,
+                                          bool valid) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = valid ? src[base + (long)(t + TT * e) * es] : mk<T>(T(0), T(0));
+}
+
+template <typename T, int E>
+__device__ __forceinline__ void store_line(const cpx<T>* v, cpx<T>* __restrict__ dst, long base, long es, int t, int TT,
+                                           bool valid) {
+    if (!valid) return;
+#pragma unroll
+    for (int e = 0; e < E; ++e) dst[base + (long)(t + TT * e) * es] = v[e];
 }
 
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
 template <typename T, int N, int DIR>
-__global__ void k_pencils(PencilArgs<T> a) {
+__global__ void __launch_bounds__(kMaxBlock) k_pencils(PencilArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
+    constexpr int E = LineShape<N>::E;
     const int n = (N > 0) ? N : a.n;
-    const int TW = a.TW, TWP = TW + 1;
+    const int TT = (N > 0) ? LineShape<N>::T : n;
     const TileInfo ti = decode_tile(a);
-    const Seg& g = a.seg[ti.s];
-    for (int q = 0; q < a.narr; ++q) {
-        const cpx<T>* in = a.src[q][ti.s] ? a.src[q][ti.s] : a.arr[q][ti.s];
-        tile_load(in, g, ti, lds + q * n * TWP, n, TW, TWP);
+    const LineMap lm = line_map(ti.es == 1, a.TW, TT, n);
+    const bool valid = lm.l < ti.nvalid;
+    const long base = ti.base + (long)lm.l * ti.sb;
+#pragma unroll
+    for (int q = 0; q < kMaxArr; ++q) {
+        if (q < a.narr) {
+            cpx<T>* out = pick(a.arr[q][0], a.arr[q][1], ti.s);
+            const cpx<T>* s = pick(a.src[q][0], a.src[q][1], ti.s);
+            const cpx<T>* in = s ? s : out;
+            cpx<T> v[E];
+            load_line<T, E>(v, in, base, ti.es, lm.t, TT, valid);
+            line_fft<T, N, DIR>(v, lds + lm.off, lm.es, n, lm.t, a.tw);
+            store_line<T, E>(v, out, base, ti.es, lm.t, TT, valid);
+        }
     }
-    __syncthreads();
-    const int l = threadIdx.x & (TW - 1), t = threadIdx.x / TW;
-    for (int q = 0; q < a.narr; ++q) line_fft<T, N, DIR>(lds + q * n * TWP + l, TWP, n, t, a.tw);
-    for (int q = 0; q < a.narr; ++q) tile_store(a.arr[q][ti.s], g, ti, lds + q * n * TWP, n, TW, TWP);
 }
 
 template <typename T, int N>
-__global__ void k_pencil_update(PencilUpdateArgs<T> u) {
+__global__ void __launch_bounds__(kMaxBlock) k_pencil_update(PencilUpdateArgs<T> u) {
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
+    constexpr int E = LineShape<N>::E;
     const PencilArgs<T>& a = u.p;
     const int n = (N > 0) ? N : a.n;
-    const int TW = a.TW, TWP = TW + 1;
+    const int TT = (N > 0) ? LineShape<N>::T : n;
     const TileInfo ti = decode_tile(a);
-    const Seg& g = a.seg[ti.s];
-    cpx<T>* Lc = lds;
-    cpx<T>* Lg = lds + n * TWP;
-    tile_load(a.arr[0][ti.s], g, ti, Lc, n, TW, TWP);
-    if (u.has_src) tile_load(a.arr[1][ti.s], g, ti, Lg, n, TW, TWP);
-    __syncthreads();
-    const int l = threadIdx.x & (TW - 1), t = threadIdx.x / TW;
-    line_fft<T, N, -1>(Lc + l, TWP, n, t, a.tw);
-    if (u.has_src) line_fft<T, N, -1>(Lg + l, TWP, n, t, a.tw);
-    // k-space update of this tile
-    double eg = 0.0;
-    const int total = TW * n;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int ll = idx & (TW - 1), p = idx / TW;
-        if (ll >= ti.nvalid) continue;
-        const int src[3] = {ti.ia, ti.b0 + ll, p};
-        int kc[3];
+    const LineMap lm = line_map(ti.es == 1, a.TW, TT, n);
+    const bool valid = lm.l < ti.nvalid;
+    const long base = ti.base + (long)lm.l * ti.sb;
+    cpx<T>* carr = pick(a.arr[0][0], a.arr[0][1], ti.s);
+    const cpx<T>* garr = pick(a.arr[1][0], a.arr[1][1], ti.s);
+    const double inx = 1.0 / (double)u.g.nx, iny = 1.0 / (double)u.g.ny, inz = 1.0 / (double)u.g.nz;
+    cpx<T> c[E], gs[E];
+    load_line<T, E>(c, carr, base, ti.es, lm.t, TT, valid);
+    line_fft<T, N, -1>(c, lds + lm.off, lm.es, n, lm.t, a.tw);
+    if (u.has_src) {
+        load_line<T, E>(gs, garr, base, ti.es, lm.t, TT, valid);
+        line_fft<T, N, -1>(gs, lds + lm.off, lm.es, n, lm.t, a.tw);
+    } else {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) kc[d] = (g.ksrc[d] == 3) ? g.kfix[d] : src[g.ksrc[d]];
-        const double fx = wrapf(kc[0], u.g.nx);
-        const double fy = wrapf(kc[1], u.g.ny);
-        const double fz = (u.g.ndim == 3) ? wrapf(kc[2], u.g.nz) : 0.0;
+        for (int e = 0; e < E; ++e) gs[e] = mk<T>(T(0), T(0));
+    }
+    // k-space update of this thread's elements
+    double eg = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int p = lm.t + TT * e;
+        const int bl = ti.b0 + lm.l;
+        int kc[3];
+        kc[0] = ksel(ti.ksrc0, ti.kfix0, ti.ia, bl, p);
+        kc[1] = ksel(ti.ksrc1, ti.kfix1, ti.ia, bl, p);
+        kc[2] = ksel(ti.ksrc2, ti.kfix2, ti.ia, bl, p);
+        const double fx = (double)(2 * kc[0] > u.g.nx ? kc[0] - u.g.nx : kc[0]) * inx;
+        const double fy = (double)(2 * kc[1] > u.g.ny ? kc[1] - u.g.ny : kc[1]) * iny;
+        const double fz = (u.g.ndim == 3) ? (double)(2 * kc[2] > u.g.nz ? kc[2] - u.g.nz : kc[2]) * inz : 0.0;
         const double f2 = fx * fx + fy * fy + fz * fz;
         const double L = -(4.0 * kPi * kPi) * f2;
         const double D = kpoly_eval(u.denom, L);
-        const double E = kpoly_eval(u.lin, L);
+        const double Ek = kpoly_eval(u.lin, L);
         double A = 1.0 / (1.0 - u.dt * D);
         if (u.filter) A *= vandeven(u.filter, sqrt(f2) * 2.0 / kPi);
         double Lp = 1.0;
         for (int q = 0; q < u.src_lap; ++q) Lp *= L;
-        const double m1 = A * (1.0 + u.dt * E);
+        const double m1 = A * (1.0 + u.dt * Ek);
         const double m2 = A * u.dt * Lp;
-        cpx<T> c = Lc[p * TWP + ll];
-        cpx<T> gs = u.has_src ? Lg[p * TWP + ll] : mk<T>(T(0), T(0));
-        if (u.egrad_partial) {
+        const cpx<T> cv = c[e];
+        const cpx<T> gv = gs[e];
+        if (valid && u.egrad_partial) {
             const double w = (kc[0] == 0 || 2 * kc[0] == u.g.nx) ? 1.0 : 2.0;
-            eg += w * (-L) * ((double)c.x * (double)c.x + (double)c.y * (double)c.y);
+            eg += w * (-L) * ((double)cv.x * (double)cv.x + (double)cv.y * (double)cv.y);
         }
-        cpx<T> r;
         if (u.mean && kc[0] == 0 && kc[1] == 0 && kc[2] == 0) {
             // k = 0 advances in FP64: the arrays hold c - mean, so their k = 0 coefficient stays 0
             // and mean_new = m1(0) mean + m2(0) G(0) / N
-            const double mu = *u.mean;
-            *u.mean = m1 * mu + m2 * (double)gs.x * u.scale;
-            r = mk<T>(T(0), T(0));
+            if (valid) {
+                const double mu = *u.mean;
+                *u.mean = m1 * mu + m2 * (double)gv.x * u.scale;
+            }
+            c[e] = mk<T>(T(0), T(0));
         } else {
             const T s1 = (T)(m1 * u.scale), s2 = (T)(m2 * u.scale);
-            r = mk<T>(s1 * c.x + s2 * gs.x, s1 * c.y + s2 * gs.y);
+            c[e] = mk<T>(s1 * cv.x + s2 * gv.x, s1 * cv.y + s2 * gv.y);
         }
-        Lc[p * TWP + ll] = r;
     }
     if (u.egrad_partial) {
-        double* red = reinterpret_cast<double*>(lds + (u.has_src ? 2 : 1) * n * TWP);
-        __syncthreads();
+        double* red = reinterpret_cast<double*>(lds + lds_elems(n, a.TW));
         const double tot = block_sum(eg, red);
         if (threadIdx.x == 0) u.egrad_partial[blockIdx.x] = tot;
     }
-    __syncthreads();
-    line_fft<T, N, +1>(Lc + l, TWP, n, t, a.tw);
-    tile_store(a.arr[0][ti.s], g, ti, Lc, n, TW, TWP);
+    line_fft<T, N, +1>(c, lds + lm.off, lm.es, n, lm.t, a.tw);
+    store_line<T, E>(c, carr, base, ti.es, lm.t, TT, valid);
 }
 
-template <typename T, int N>
-__global__ void k_rows_r2c(RowR2CArgs<T> a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
-    const int n = (N > 0) ? N : a.nx;
+// x-rows: t fastest; the half spectrum split needs Z(k) and Z(n-k) from other threads
+template <typename T, int E>
+__device__ __forceinline__ void split_store(const cpx<T>* v, cpx<T>* L, int n, int t, int TT, long row, bool valid,
+                                            cpx<T>* amain, cpx<T>* anyq, cpx<T>* bmain, cpx<T>* bnyq) {
     const int nh = n / 2;
-    const int TW = a.TW, TWP = TW + 1;
-    const long row0 = (long)blockIdx.x * TW;
-    const int nvalid = (int)min((long)TW, a.rows - row0);
-    const int npair = (a.narr + 1) / 2;
-    for (int q2 = 0; q2 < npair; ++q2) {
-        const T* ia = a.in[2 * q2];
-        const T* ib = (2 * q2 + 1 < a.narr) ? a.in[2 * q2 + 1] : nullptr;
-        cpx<T>* L = lds + q2 * n * TWP;
-        for (int idx = threadIdx.x; idx < TW * n; idx += blockDim.x) {
-            const int l = idx / n, p = idx - (idx / n) * n;
-            T x = T(0), y = T(0);
-            if (l < nvalid) {
-                const long off = (row0 + l) * n + p;
-                if (ia) x = ia[off] * a.in_scale;
-                if (ib) y = ib[off] * a.in_scale;
-            }
-            L[p * TWP + l] = mk<T>(x, y);
-        }
-    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) L[lds_pad(t + TT * e)] = v[e];
     __syncthreads();
-    const int l = threadIdx.x & (TW - 1), t = threadIdx.x / TW;
-    for (int q2 = 0; q2 < npair; ++q2) line_fft<T, N, -1>(lds + q2 * n * TWP + l, TWP, n, t, a.tw);
-    for (int q2 = 0; q2 < npair; ++q2) {
-        const cpx<T>* L = lds + q2 * n * TWP;
-        const int qa = 2 * q2, qb = 2 * q2 + 1;
-        const int m = nh + 1;
-        for (int idx = threadIdx.x; idx < TW * m; idx += blockDim.x) {
-            const int ll = idx / m, k = idx - (idx / m) * m;
-            if (ll >= nvalid) continue;
-            const long row = row0 + ll;
-            const cpx<T> z = L[k * TWP + ll];
-            const cpx<T> zm = L[((n - k) % n) * TWP + ll];
+    if (valid) {
+        for (int k = t; k <= nh; k += TT) {
+            const cpx<T> z = L[lds_pad(k)];
+            const cpx<T> zm = L[lds_pad((n - k) % n)];
             // A = (z + conj zm) / 2,  B = (z - conj zm) / (2 i)
             const cpx<T> A = mk<T>(T(0.5) * (z.x + zm.x), T(0.5) * (z.y - zm.y));
             const cpx<T> B = mk<T>(T(0.5) * (z.y + zm.y), T(-0.5) * (z.x - zm.x));
             if (k < nh) {
-                a.main[qa][row * nh + k] = A;
-                if (qb < a.narr) a.main[qb][row * nh + k] = B;
+                amain[row * nh + k] = A;
+                if (bmain) bmain[row * nh + k] = B;
             } else {
-                a.nyq[qa][row] = A;
-                if (qb < a.narr) a.nyq[qb][row] = B;
+                anyq[row] = A;
+                if (bnyq) bnyq[row] = B;
             }
         }
+    }
+    __syncthreads();
+}
+
+// Hermitian extension of a half-spectrum row at canonical positions
+template <typename T, int E>
+__device__ __forceinline__ void load_herm(cpx<T>* v, const cpx<T>* __restrict__ m, const cpx<T>* __restrict__ q,
+                                          long row, int n, int t, int TT, bool valid) {
+    const int nh = n / 2;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int p = t + TT * e;
+        cpx<T> x = mk<T>(T(0), T(0));
+        if (valid) {
+            if (p < nh) x = m[row * nh + p];
+            else if (p == nh) x = q[row];
+            else x = conjg(m[row * nh + (n - p)]);
+        }
+        v[e] = x;
     }
 }
 
 template <typename T, int N>
-__global__ void k_rows_c2r(RowC2RArgs<T> a) {
+__global__ void __launch_bounds__(kMaxBlock) k_rows_r2c(RowR2CArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
+    constexpr int E = LineShape<N>::E;
     const int n = (N > 0) ? N : a.nx;
-    const int nh = n / 2;
-    const int TW = a.TW, TWP = TW + 1;
-    const long row0 = (long)blockIdx.x * TW;
-    const int nvalid = (int)min((long)TW, a.rows - row0);
-    const int npair = (a.narr + 1) / 2;
-    const int m = nh + 1;
-    for (int q2 = 0; q2 < npair; ++q2) {
-        const int qa = 2 * q2, qb = 2 * q2 + 1;
-        cpx<T>* L = lds + q2 * n * TWP;
-        for (int idx = threadIdx.x; idx < TW * m; idx += blockDim.x) {
-            const int ll = idx / m, k = idx - (idx / m) * m;
-            cpx<T> A = mk<T>(T(0), T(0)), B = mk<T>(T(0), T(0));
-            if (ll < nvalid) {
-                const long row = row0 + ll;
-                A = (k < nh) ? a.main[qa][row * nh + k] : a.nyq[qa][row];
-                if (qb < a.narr) B = (k < nh) ? a.main[qb][row * nh + k] : a.nyq[qb][row];
+    const int TT = (N > 0) ? LineShape<N>::T : n;
+    const LineMap lm = line_map(true, a.TW, TT, n);
+    const long row = (long)blockIdx.x * a.TW + lm.l;
+    const bool valid = row < a.rows;
+#pragma unroll
+    for (int q = 0; q < kMaxArr; q += 2) {
+        if (q >= a.narr) break;
+        const T* ia = a.in[q];
+        const T* ib = (q + 1 < a.narr) ? a.in[q + 1] : nullptr;
+        cpx<T> v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const long off = row * n + lm.t + TT * e;
+            T x = T(0), y = T(0);
+            if (valid) {
+                if (ia) x = ia[off] * a.in_scale;
+                if (ib) y = ib[off] * a.in_scale;
             }
-            // Z(k) = A(k) + i B(k); Z(n-k) = conj(A(k)) + i conj(B(k))
-            L[k * TWP + ll] = mk<T>(A.x - B.y, A.y + B.x);
-            if (k > 0 && k < nh) L[(n - k) * TWP + ll] = mk<T>(A.x + B.y, -A.y + B.x);
+            v[e] = mk<T>(x, y);
         }
+        line_fft<T, N, -1>(v, lds + lm.off, 1, n, lm.t, a.tw);
+        split_store<T, E>(v, lds + lm.off, n, lm.t, TT, row, valid, a.main[q], a.nyq[q],
+                          ib ? a.main[q + 1] : nullptr, ib ? a.nyq[q + 1] : nullptr);
     }
-    __syncthreads();
-    const int l = threadIdx.x & (TW - 1), t = threadIdx.x / TW;
-    for (int q2 = 0; q2 < npair; ++q2) line_fft<T, N, +1>(lds + q2 * n * TWP + l, TWP, n, t, a.tw);
-    for (int q2 = 0; q2 < npair; ++q2) {
-        const int qa = 2 * q2, qb = 2 * q2 + 1;
-        const cpx<T>* L = lds + q2 * n * TWP;
-        for (int idx = threadIdx.x; idx < TW * n; idx += blockDim.x) {
-            const int ll = idx / n, p = idx - (idx / n) * n;
-            if (ll >= nvalid) continue;
-            const long off = (row0 + ll) * n + p;
-            const cpx<T> z = L[p * TWP + ll];
-            a.out[qa][off] = z.x * a.scale;
-            if (qb < a.narr) a.out[qb][off] = z.y * a.scale;
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(kMaxBlock) k_rows_c2r(RowC2RArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
+    constexpr int E = LineShape<N>::E;
+    const int n = (N > 0) ? N : a.nx;
+    const int TT = (N > 0) ? LineShape<N>::T : n;
+    const LineMap lm = line_map(true, a.TW, TT, n);
+    const long row = (long)blockIdx.x * a.TW + lm.l;
+    const bool valid = row < a.rows;
+#pragma unroll
+    for (int q = 0; q < kMaxArr; q += 2) {
+        if (q >= a.narr) break;
+        const bool two = q + 1 < a.narr;
+        cpx<T> A[E], v[E];
+        load_herm<T, E>(A, a.main[q], a.nyq[q], row, n, lm.t, TT, valid);
+        if (two) {
+            load_herm<T, E>(v, a.main[q + 1], a.nyq[q + 1], row, n, lm.t, TT, valid);
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = mk<T>(A[e].x - v[e].y, A[e].y + v[e].x);  // A + i B
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = A[e];
+        }
+        line_fft<T, N, +1>(v, lds + lm.off, 1, n, lm.t, a.tw);
+        if (valid) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const long off = row * n + lm.t + TT * e;
+                a.out[q][off] = v[e].x * a.scale;
+                if (two) a.out[q + 1][off] = v[e].y * a.scale;
+            }
         }
     }
 }
 
 template <typename T, int N>
-__global__ void k_rows_fused(RowFusedArgs<T> a) {
+__global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cpx<T>* L = reinterpret_cast<cpx<T>*>(smem);
+    cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
+    constexpr int E = LineShape<N>::E;
     const int n = (N > 0) ? N : a.nx;
-    const int nh = n / 2;
-    const int TW = a.TW, TWP = TW + 1;
-    const long row0 = (long)blockIdx.x * TW;
-    const int nvalid = (int)min((long)TW, a.rows - row0);
-    const int m = nh + 1;
-    // Hermitian extension of the state row
-    for (int idx = threadIdx.x; idx < TW * m; idx += blockDim.x) {
-        const int ll = idx / m, k = idx - (idx / m) * m;
-        cpx<T> A = mk<T>(T(0), T(0));
-        if (ll < nvalid) {
-            const long row = row0 + ll;
-            A = (k < nh) ? a.s_main[row * nh + k] : a.s_nyq[row];
-        }
-        L[k * TWP + ll] = A;
-        if (k > 0 && k < nh) L[(n - k) * TWP + ll] = conjg(A);
-    }
-    __syncthreads();
-    const int l = threadIdx.x & (TW - 1), t = threadIdx.x / TW;
-    line_fft<T, N, +1>(L + l, TWP, n, t, a.tw);
-    // real space: c, g(c), optional output and FP64 partial sums
+    const int TT = (N > 0) ? LineShape<N>::T : n;
+    const LineMap lm = line_map(true, a.TW, TT, n);
+    const long row = (long)blockIdx.x * a.TW + lm.l;
+    const bool valid = row < a.rows;
+    cpx<T> v[E];
+    load_herm<T, E>(v, a.s_main, a.s_nyq, row, n, lm.t, TT, valid);
+    line_fft<T, N, +1>(v, lds + lm.off, 1, n, lm.t, a.tw);
+    // real space: c = (c - mean) + mean, g(c), optional output and FP64 partial sums
     KSum sb, sm;
     const double mu = a.mean ? *a.mean : 0.0;
     const T muT = (T)mu;
-    for (int idx = threadIdx.x; idx < TW * n; idx += blockDim.x) {
-        const int ll = idx / n, p = idx - (idx / n) * n;
-        const T dc = L[p * TWP + ll].x;   // c - mean (mean = 0 when the arrays carry k = 0)
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const T dc = v[e].x;
         const T c = dc + muT;
         T gv = T(0);
         for (int q = a.npoly - 1; q >= 0; --q) gv = gv * c + (T)a.gpoly[q];
-        L[p * TWP + ll] = mk<T>(dc, gv);
-        if (ll < nvalid) {
-            if (a.c_out) a.c_out[(row0 + ll) * n + p] = c;
+        v[e] = mk<T>(dc, gv);
+        if (valid) {
+            if (a.c_out) a.c_out[row * n + lm.t + TT * e] = c;
             if (a.bulk_partial) {
                 const double cd = (double)dc + mu;
                 double f = 0.0;
@@ -392,8 +441,7 @@ __global__ void k_rows_fused(RowFusedArgs<T> a) {
         }
     }
     if (a.bulk_partial) {
-        double* red = reinterpret_cast<double*>(L + n * TWP);
-        __syncthreads();
+        double* red = reinterpret_cast<double*>(lds + lds_elems(n, a.TW));
         const double b = block_sum(sb.get(), red);
         const double ms = block_sum(sm.get(), red);
         if (threadIdx.x == 0) {
@@ -401,24 +449,8 @@ __global__ void k_rows_fused(RowFusedArgs<T> a) {
             a.mass_partial[blockIdx.x] = ms;
         }
     }
-    __syncthreads();
-    line_fft<T, N, -1>(L + l, TWP, n, t, a.tw);
-    for (int idx = threadIdx.x; idx < TW * m; idx += blockDim.x) {
-        const int ll = idx / m, k = idx - (idx / m) * m;
-        if (ll >= nvalid) continue;
-        const long row = row0 + ll;
-        const cpx<T> z = L[k * TWP + ll];
-        const cpx<T> zm = L[((n - k) % n) * TWP + ll];
-        const cpx<T> C = mk<T>(T(0.5) * (z.x + zm.x), T(0.5) * (z.y - zm.y));
-        const cpx<T> G = mk<T>(T(0.5) * (z.y + zm.y), T(-0.5) * (z.x - zm.x));
-        if (k < nh) {
-            a.c_main[row * nh + k] = C;
-            a.g_main[row * nh + k] = G;
-        } else {
-            a.c_nyq[row] = C;
-            a.g_nyq[row] = G;
-        }
-    }
+    line_fft<T, N, -1>(v, lds + lm.off, 1, n, lm.t, a.tw);
+    split_store<T, E>(v, lds + lm.off, n, lm.t, TT, row, valid, a.c_main, a.c_nyq, a.g_main, a.g_nyq);
 }
 
 int threads_per_line(int n) {
@@ -490,7 +522,7 @@ hipError_t launch_pencils(const PencilArgs<T>& a0, int n, int dir, hipStream_t s
     a.n = n;
     const long blocks = pencil_blocks(a.seg, a.nseg, a.TW, &a.tiles0);
     const int thr = a.TW * threads_per_line(n);
-    const size_t smem = (size_t)a.narr * n * (a.TW + 1) * sizeof(cpx<T>);
+    const size_t smem = lds_elems(n, a.TW) * sizeof(cpx<T>);
     return dispatch_size(n, [&](auto NC) -> hipError_t {
         constexpr int NN = decltype(NC)::value;
         if (dir < 0) return launch_k(k_pencils<T, NN, -1>, dim3(blocks), dim3(thr), smem, s, a);
@@ -504,7 +536,7 @@ hipError_t launch_pencil_update(const PencilUpdateArgs<T>& u0, int n, hipStream_
     u.p.n = n;
     const long blocks = pencil_blocks(u.p.seg, u.p.nseg, u.p.TW, &u.p.tiles0);
     const int thr = u.p.TW * threads_per_line(n);
-    size_t smem = (size_t)(u.has_src ? 2 : 1) * n * (u.p.TW + 1) * sizeof(cpx<T>);
+    size_t smem = lds_elems(n, u.p.TW) * sizeof(cpx<T>);
     if (u.egrad_partial) smem += (size_t)thr * sizeof(double);
     return dispatch_size(n, [&](auto NC) -> hipError_t {
         constexpr int NN = decltype(NC)::value;
@@ -516,7 +548,7 @@ template <typename T>
 hipError_t launch_rows_r2c(const RowR2CArgs<T>& a, hipStream_t s) {
     const long blocks = rows_blocks(a.rows, a.TW);
     const int thr = a.TW * threads_per_line(a.nx);
-    const size_t smem = (size_t)((a.narr + 1) / 2) * a.nx * (a.TW + 1) * sizeof(cpx<T>);
+    const size_t smem = lds_elems(a.nx, a.TW) * sizeof(cpx<T>);
     return dispatch_size(a.nx, [&](auto NC) -> hipError_t {
         constexpr int NN = decltype(NC)::value;
         return launch_k(k_rows_r2c<T, NN>, dim3(blocks), dim3(thr), smem, s, a);
@@ -527,7 +559,7 @@ template <typename T>
 hipError_t launch_rows_c2r(const RowC2RArgs<T>& a, hipStream_t s) {
     const long blocks = rows_blocks(a.rows, a.TW);
     const int thr = a.TW * threads_per_line(a.nx);
-    const size_t smem = (size_t)((a.narr + 1) / 2) * a.nx * (a.TW + 1) * sizeof(cpx<T>);
+    const size_t smem = lds_elems(a.nx, a.TW) * sizeof(cpx<T>);
     return dispatch_size(a.nx, [&](auto NC) -> hipError_t {
         constexpr int NN = decltype(NC)::value;
         return launch_k(k_rows_c2r<T, NN>, dim3(blocks), dim3(thr), smem, s, a);
@@ -538,7 +570,7 @@ template <typename T>
 hipError_t launch_rows_fused(const RowFusedArgs<T>& a, hipStream_t s) {
     const long blocks = rows_blocks(a.rows, a.TW);
     const int thr = a.TW * threads_per_line(a.nx);
-    size_t smem = (size_t)a.nx * (a.TW + 1) * sizeof(cpx<T>);
+    size_t smem = lds_elems(a.nx, a.TW) * sizeof(cpx<T>);
     if (a.bulk_partial) smem += (size_t)thr * sizeof(double);
     return dispatch_size(a.nx, [&](auto NC) -> hipError_t {
         constexpr int NN = decltype(NC)::value;

@@ -106,6 +106,8 @@ typedef struct {
     int filter_order;  /* > 0: Stepper.SetFilter(NewVandeven(order)); 0: no filter */
     int fused;         /* 1: allow the fused single-field path; 0: general path only */
     int device;        /* HIP device ordinal */
+    int autotune;      /* 1: time every pass of the fused step for each admissible tile width
+                          (lines per block) on the first step and keep the fastest */
 } gpf_options;
 
 void gpf_options_default(gpf_options* o);
@@ -138,6 +140,9 @@ int gpf_solver_is_fused(const gpf_solver* s);
  * of each pass recorded with HIP events on the solver's stream */
 int gpf_solver_enable_timing(gpf_solver* s, int on);
 int gpf_solver_pass_times(gpf_solver* s, int maxpass, double* ms, long* launches, char* names, int namelen);
+/* lines per block chosen for the fused passes (rows, y forward, update, y inverse; 0 = default
+ * heuristic); returns the number of values written */
+int gpf_solver_tuning(gpf_solver* s, int* tws, int maxn);
 /* the HIP stream the solver launches on (hipStream_t) */
 void* gpf_solver_stream(gpf_solver* s);
 
