@@ -106,6 +106,9 @@ def main() -> int:
     dims = [N, N, N]
     n = N ** 3
     c0 = W.ch_initial(1, n)
+    copy_bytes = 1 << 30
+    copy_gbs = G.CopyBandwidth(copy_bytes, reps=10)
+    print(f"# device copy of {copy_bytes} B: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
     print(f"# building FP32-bulk solver {dims}", file=sys.stderr, flush=True)
     s32 = build_solver(G, W, n, dims, c0, "fp32", args.dt)
     s32.StepperPropagate(args.warmup)   # the first step runs the lines-per-block autotuner
@@ -151,7 +154,8 @@ def main() -> int:
             "avg_launch_ms": ms / cnt, "lines_per_block": s32.Tuning(),
             "passes_ms_per_launch": {k: v[0] / v[1] for k, v in passes.items()},
             "step_effective_GBs": step_bytes * args.steps / el32 / 1e9,
-            "step_frac": step_bytes * args.steps / el32 / 1e9 / HBM_PEAK_GBS}
+            "step_frac": step_bytes * args.steps / el32 / 1e9 / HBM_PEAK_GBS,
+            "measured_copy_GBs": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs}
     cpu = None
     if not args.no_cpu:
         print("# timing the CPU oracle (one step)", file=sys.stderr, flush=True)

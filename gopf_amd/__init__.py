@@ -98,6 +98,7 @@ def _declare(L) -> None:
         "gpf_solver_pass_times": (C.c_int, [_vp, C.c_int, _dp, _lp, C.c_char_p, C.c_int]),
         "gpf_solver_stream": (_vp, [_vp]),
         "gpf_solver_tuning": (C.c_int, [_vp, _ip, C.c_int]),
+        "gpf_device_copy_bandwidth": (C.c_int, [C.c_long, C.c_int, C.c_int, _dp]),
         "gpf_fft_create": (C.c_int, [_ip, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
         "gpf_fft_destroy": (None, [_vp]),
         "gpf_fft_forward": (C.c_int, [_vp, _dp, C.c_long]),
@@ -398,9 +399,10 @@ class Solver:
         return out
 
     def Tuning(self) -> Dict[str, int]:
-        v = (C.c_int * 4)()
-        lib().gpf_solver_tuning(self._h, v, 4)
-        return {"rows_fused": v[0], "pencils_fwd": v[1], "pencil_update": v[2], "pencils_inv": v[3]}
+        v = (C.c_int * 5)()
+        lib().gpf_solver_tuning(self._h, v, 5)
+        return {"rows_fused": v[0], "pencils_fwd": v[1], "pencil_update": v[2], "pencils_inv": v[3],
+                "update_axis": "zy"[2 - v[4]] if v[4] in (1, 2) else "y"}
 
     def Stream(self) -> int:
         return int(lib().gpf_solver_stream(self._h) or 0)
@@ -447,6 +449,13 @@ class FourierTransform:
 
     def ConjugateNode(self, i: int) -> int:
         return int(lib().gpf_fft_conjugate_node(_ints(self.Dimensions), len(self.Dimensions), int(i)))
+
+
+def CopyBandwidth(nbytes: int, reps: int = 20, device: int = 0) -> float:
+    """Measured device copy bandwidth in GB/s (read + write bytes per second)."""
+    ms = C.c_double()
+    _check(lib().gpf_device_copy_bandwidth(int(nbytes), int(reps), int(device), C.byref(ms)))
+    return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
 
 def Freq(dims: Sequence[int], i: int) -> List[float]:

@@ -170,13 +170,13 @@ struct Geometry {
     }
 };
 
-static bool tw_fits(int n, int tw, size_t elem) {
-    const int T = fft_threads_per_line(n);
+static bool tw_fits(int n, int tw, size_t elem, int variant = 0) {
+    const int T = fft_threads_per_line(n, variant);
     return tw * T <= kMaxBlockThreads && (size_t)(n + n / 16 + 1) * (tw + 1) * elem <= 150 * 1024;
 }
 
-static int choose_tw(int n, int narr, size_t elem, int min_tw) {
-    const int T = fft_threads_per_line(n);
+static int choose_tw(int n, int narr, size_t elem, int min_tw, int variant = 0) {
+    const int T = fft_threads_per_line(n, variant);
     int tw = 1;
     while (tw * 2 * T <= 256) tw *= 2;  // lines per block: a power of two
     if (tw > 64) tw = 64;
@@ -356,6 +356,12 @@ private:
     void autotune();
     bool autotune_ = false, tuned_ = false;
     int tw_rows_ = 0, tw_pf_ = 0, tw_pu_ = 0, tw_pi_ = 0;
+    // 3D: axis of the fused forward-update-inverse pass (2 = z, 1 = y); the other
+    // of y / z is transformed forward before and inverse after it
+    int upd_axis_ = 2;
+    int upd_axis() const { return (g_.ndim == 3) ? upd_axis_ : 1; }
+    int other_axis() const { return upd_axis() == 2 ? 1 : 2; }
+    int axis_len(int axis) const { return axis == 2 ? g_.nz : g_.ny; }
     // general path
     void general_step_euler();
     void general_step_rk4();
@@ -836,7 +842,7 @@ RowFusedArgs<T> Engine<T>::rows_args(const cpx<T>* X, cpx<T>* Y, cpx<T>* Gb, dou
 template <typename T>
 PencilUpdateArgs<T> Engine<T>::update_args(cpx<T>* Y, cpx<T>* Gb, double* mean, int tw) {
     PencilUpdateArgs<T> u;
-    const int axis = (g_.ndim == 3) ? 2 : 1;
+    const int axis = upd_axis();
     const int nu = (axis == 2) ? g_.nz : g_.ny;
     u.p.tw = this->tw(nu);
     u.p.narr = fp_.has_src ? 2 : 1;
@@ -847,7 +853,7 @@ PencilUpdateArgs<T> Engine<T>::update_args(cpx<T>* Y, cpx<T>* Gb, double* mean, 
     u.p.seg[0] = (axis == 2) ? seg_z_main(g_) : seg_y_main(g_);
     u.p.seg[1] = (axis == 2) ? seg_z_nyq(g_) : seg_y_nyq(g_);
     u.p.nseg = 2;
-    u.p.TW = tw > 0 ? tw : choose_tw(nu, u.p.narr, sizeof(cpx<T>), 8);
+    u.p.TW = tw > 0 ? tw : choose_tw(nu, u.p.narr, sizeof(cpx<T>), 8, 1);
     u.g = g_.kg;
     u.dt = dt_;
     u.scale = 1.0 / (double)g_.n;
@@ -889,7 +895,6 @@ void Engine<T>::autotune() {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    const int nu = (g_.ndim == 3) ? g_.nz : g_.ny;
     auto time_it = [&](auto&& launch) {
         launch();
         launch();
@@ -903,25 +908,49 @@ void Engine<T>::autotune() {
         return ms / reps;
     };
     const int cand[] = {4, 8, 16, 32, 64};
-    auto best_of = [&](int n, auto&& launch_with) {
+    auto best_of = [&](int n, auto&& launch_with, int variant = 0) {
         int best = 0;
         float bt = 1e30f;
         for (int tw : cand) {
-            if (!tw_fits(n, tw, sizeof(cpx<T>))) continue;
+            if (!tw_fits(n, tw, sizeof(cpx<T>), variant)) continue;
             const float t = time_it([&] { launch_with(tw); });
             if (t < bt) { bt = t; best = tw; }
         }
         return best;
     };
     tw_rows_ = best_of(g_.nx, [&](int tw) { HIPCHK(launch_rows_fused<T>(rows_args(X.get(), Y.get(), Gb.get(), mu.get(), tw), st_)); });
-    if (g_.ndim == 3) {
-        tw_pf_ = best_of(g_.ny, [&](int tw) {
-            if (fp_.has_src) pencils_tw({Y.get(), Gb.get()}, 1, -1, tw);
-            else pencils_tw({Y.get()}, 1, -1, tw);
-        });
-        tw_pi_ = best_of(g_.ny, [&](int tw) { pencils_tw({Y.get()}, 1, +1, tw); });
+    // for each order of the y / z passes, tune the tile widths and keep the faster order
+    const int orders = (g_.ndim == 3) ? 2 : 1;
+    float best_total = 1e30f;
+    int best[4] = {0, 0, 0, 2};
+    for (int o = 0; o < orders; ++o) {
+        upd_axis_ = (o == 0) ? 2 : 1;
+        int pf = 0, pi = 0, pu = 0;
+        float total = 0.f;
+        const int no = axis_len(other_axis()), nu = axis_len(upd_axis());
+        if (g_.ndim == 3) {
+            pf = best_of(no, [&](int tw) {
+                if (fp_.has_src) pencils_tw({Y.get(), Gb.get()}, other_axis(), -1, tw);
+                else pencils_tw({Y.get()}, other_axis(), -1, tw);
+            });
+            pi = best_of(no, [&](int tw) { pencils_tw({Y.get()}, other_axis(), +1, tw); });
+            total += time_it([&] {
+                if (fp_.has_src) pencils_tw({Y.get(), Gb.get()}, other_axis(), -1, pf);
+                else pencils_tw({Y.get()}, other_axis(), -1, pf);
+            });
+            total += time_it([&] { pencils_tw({Y.get()}, other_axis(), +1, pi); });
+        }
+        pu = best_of(nu, [&](int tw) { HIPCHK(launch_pencil_update<T>(update_args(Y.get(), Gb.get(), mu.get(), tw), nu, st_)); }, 1);
+        total += time_it([&] { HIPCHK(launch_pencil_update<T>(update_args(Y.get(), Gb.get(), mu.get(), pu), nu, st_)); });
+        if (total < best_total) {
+            best_total = total;
+            best[0] = pf; best[1] = pu; best[2] = pi; best[3] = upd_axis_;
+        }
     }
-    tw_pu_ = best_of(nu, [&](int tw) { HIPCHK(launch_pencil_update<T>(update_args(Y.get(), Gb.get(), mu.get(), tw), nu, st_)); });
+    tw_pf_ = best[0];
+    tw_pu_ = best[1];
+    tw_pi_ = best[2];
+    upd_axis_ = best[3];
     HIPCHK(hipEventDestroy(e0));
     HIPCHK(hipEventDestroy(e1));
     HIPCHK(hipStreamSynchronize(st_));
@@ -930,9 +959,9 @@ void Engine<T>::autotune() {
 
 template <typename T>
 int Engine<T>::tuning(int* tws, int maxn) {
-    const int v[4] = {tw_rows_, tw_pf_, tw_pu_, tw_pi_};
-    for (int i = 0; i < 4 && i < maxn; ++i) tws[i] = v[i];
-    return 4;
+    const int v[5] = {tw_rows_, tw_pf_, tw_pu_, tw_pi_, upd_axis()};
+    for (int i = 0; i < 5 && i < maxn; ++i) tws[i] = v[i];
+    return 5;
 }
 
 template <typename T>
@@ -940,7 +969,7 @@ void Engine<T>::fused_step() {
     if (autotune_ && !tuned_) autotune();
     RowFusedArgs<T> ra = rows_args(X_.get(), Y_.get(), G_.get(), mean_.get(), tw_rows_);
     const long nblk_a = rows_blocks(g_.rows, ra.TW);
-    const int nu = (g_.ndim == 3) ? g_.nz : g_.ny;
+    const int nu = axis_len(upd_axis());
     PencilUpdateArgs<T> u = update_args(Y_.get(), G_.get(), mean_.get(), tw_pu_);
     int tiles0 = 0;
     const long nblk_c = pencil_blocks(u.p.seg, u.p.nseg, u.p.TW, &tiles0);
@@ -967,8 +996,8 @@ void Engine<T>::fused_step() {
     tend(0);
     if (g_.ndim == 3) {
         tbegin(1);
-        if (fp_.has_src) pencils_tw({Y_.get(), G_.get()}, 1, -1, tw_pf_);
-        else pencils_tw({Y_.get()}, 1, -1, tw_pf_);
+        if (fp_.has_src) pencils_tw({Y_.get(), G_.get()}, other_axis(), -1, tw_pf_);
+        else pencils_tw({Y_.get()}, other_axis(), -1, tw_pf_);
         tend(1);
     }
     tbegin(2);
@@ -976,7 +1005,7 @@ void Engine<T>::fused_step() {
     tend(2);
     if (g_.ndim == 3) {
         tbegin(3);
-        pencils_tw({Y_.get()}, 1, +1, tw_pi_);
+        pencils_tw({Y_.get()}, other_axis(), +1, tw_pi_);
         tend(3);
     }
     if (mon_) {
@@ -1632,6 +1661,32 @@ int gpf_solver_pass_times(gpf_solver* s, int maxpass, double* ms, long* launches
         np = s->eng->pass_times(maxpass, ms, launches, names, namelen);
     });
     return rc ? -rc : np;
+}
+
+int gpf_device_copy_bandwidth(long bytes, int reps, int device, double* ms_per_copy) {
+    return guarded([&] {
+        if (bytes < 4 || reps < 1 || !ms_per_copy) throw Error(GPF_EINVAL, "bad argument");
+        set_device(device);
+        const long n = (bytes / (long)sizeof(float)) & ~3L;
+        Buf<float> a(n), b(n);
+        hipStream_t st;
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipMemsetAsync(a.get(), 0, n * sizeof(float), st));
+        HIPCHK(launch_copy_bw(a.get(), b.get(), n, st));
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r) HIPCHK(launch_copy_bw(a.get(), b.get(), n, st));
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        *ms_per_copy = ms / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(st);
+    });
 }
 
 int gpf_solver_tuning(gpf_solver* s, int* tws, int maxn) {

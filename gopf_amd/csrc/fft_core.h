@@ -42,16 +42,19 @@ __device__ __forceinline__ cpx<T> mul_i(cpx<T> a) {
 // ---------------------------------------------------------------------------
 // compile-time radix schedules
 // ---------------------------------------------------------------------------
-template <int N>
-struct Sched;  // NR stages, radices R[], E elements per thread
-#define GPF_SCHED(n, e, ...)                                \
+// V = 0: default schedule; V = 1: low-register schedule (E <= 12) for kernels
+// that keep several transformed lines in registers at once
+template <int N, int V = 0>
+struct Sched : Sched<N, 0> {};
+#define GPF_SCHED_V(n, v, e, ...)                           \
     template <>                                             \
-    struct Sched<n> {                                       \
+    struct Sched<n, v> {                                    \
         static constexpr int E = e;                         \
         static constexpr int R[] = {__VA_ARGS__};           \
         static constexpr int NR = sizeof(R) / sizeof(int);  \
         static constexpr int T = n / e;                     \
     };
+#define GPF_SCHED(n, e, ...) GPF_SCHED_V(n, 0, e, __VA_ARGS__)
 GPF_SCHED(2, 2, 2)
 GPF_SCHED(4, 4, 4)
 GPF_SCHED(8, 8, 8)
@@ -69,15 +72,24 @@ GPF_SCHED(96, 24, 8, 4, 3)
 GPF_SCHED(192, 24, 8, 8, 3)
 GPF_SCHED(384, 24, 8, 8, 2, 3)
 GPF_SCHED(768, 48, 16, 16, 3)
+GPF_SCHED_V(128, 1, 8, 8, 4, 4)
+GPF_SCHED_V(256, 1, 8, 8, 8, 4)
+GPF_SCHED_V(1024, 1, 8, 8, 8, 4, 4)
+GPF_SCHED_V(24, 1, 12, 4, 2, 3)
+GPF_SCHED_V(96, 1, 12, 4, 4, 2, 3)
+GPF_SCHED_V(192, 1, 12, 4, 4, 4, 3)
+GPF_SCHED_V(384, 1, 12, 4, 4, 4, 2, 3)
+GPF_SCHED_V(768, 1, 12, 4, 4, 4, 4, 3)
 #undef GPF_SCHED
+#undef GPF_SCHED_V
 
 // span before stage S
-template <int N, int S>
+template <int N, int V, int S>
 struct Span {
-    static constexpr int v = Span<N, S - 1>::v * Sched<N>::R[S - 1];
+    static constexpr int v = Span<N, V, S - 1>::v * Sched<N, V>::R[S - 1];
 };
-template <int N>
-struct Span<N, 0> {
+template <int N, int V>
+struct Span<N, V, 0> {
     static constexpr int v = 1;
 };
 
@@ -205,13 +217,13 @@ __device__ __forceinline__ void dftR(cpx<T>* a) {
 __device__ __forceinline__ int lds_pad(int p) { return p + (p >> 4); }
 constexpr int lds_line(int n) { return n + n / 16 + 1; }
 
-template <typename T, int N, int S, int DIR>
+template <typename T, int N, int V, int S, int DIR>
 __device__ __forceinline__ void stage_compute(cpx<T>* a, int t, const cpx<T>* __restrict__ tw) {
-    constexpr int R = Sched<N>::R[S];
-    constexpr int E = Sched<N>::E;
-    constexpr int TT = Sched<N>::T;
+    constexpr int R = Sched<N, V>::R[S];
+    constexpr int E = Sched<N, V>::E;
+    constexpr int TT = Sched<N, V>::T;
     constexpr int NB = E / R;
-    constexpr int NS = Span<N, S>::v;
+    constexpr int NS = Span<N, V, S>::v;
     constexpr int TWSTEP = N / (NS * R);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -229,16 +241,16 @@ __device__ __forceinline__ void stage_compute(cpx<T>* a, int t, const cpx<T>* __
     }
 }
 
-template <typename T, int N, int S, int DIR>
+template <typename T, int N, int V, int S, int DIR>
 __device__ __forceinline__ void exchange_and_compute(cpx<T>* a, cpx<T>* lds, int es, int t,
                                                      const cpx<T>* __restrict__ tw) {
-    if constexpr (S < Sched<N>::NR) {
-        constexpr int TT = Sched<N>::T;
-        constexpr int E = Sched<N>::E;
-        constexpr int RP = Sched<N>::R[S - 1];
+    if constexpr (S < Sched<N, V>::NR) {
+        constexpr int TT = Sched<N, V>::T;
+        constexpr int E = Sched<N, V>::E;
+        constexpr int RP = Sched<N, V>::R[S - 1];
         constexpr int NBP = E / RP;
-        constexpr int NSP = Span<N, S - 1>::v;
-        constexpr int R = Sched<N>::R[S];
+        constexpr int NSP = Span<N, V, S - 1>::v;
+        constexpr int R = Sched<N, V>::R[S];
         constexpr int NB = E / R;
 #pragma unroll
         for (int b = 0; b < NBP; ++b) {
@@ -255,29 +267,29 @@ __device__ __forceinline__ void exchange_and_compute(cpx<T>* a, cpx<T>* lds, int
             for (int m = 0; m < R; ++m) a[b * R + m] = lds[lds_pad(j + m * (N / R)) * es];
         }
         __syncthreads();
-        stage_compute<T, N, S, DIR>(a, t, tw);
-        exchange_and_compute<T, N, S + 1, DIR>(a, lds, es, t, tw);
+        stage_compute<T, N, V, S, DIR>(a, t, tw);
+        exchange_and_compute<T, N, V, S + 1, DIR>(a, lds, es, t, tw);
     }
 }
 
 // In-place transform of a line held in canonical registers v[E].  lds is this
 // line's exchange buffer (element p at lds[lds_pad(p) * es]).  Every thread of
 // the block must call it: it synchronises the block between stages.
-template <typename T, int N, int DIR>
+template <typename T, int N, int DIR, int V = 0>
 __device__ __forceinline__ void fft_canon(cpx<T>* v, cpx<T>* lds, int es, int t, const cpx<T>* __restrict__ tw) {
-    constexpr int E = Sched<N>::E;
-    constexpr int NR = Sched<N>::NR;
-    constexpr int R0 = Sched<N>::R[0];
+    constexpr int E = Sched<N, V>::E;
+    constexpr int NR = Sched<N, V>::NR;
+    constexpr int R0 = Sched<N, V>::R[0];
     constexpr int NB0 = E / R0;
-    constexpr int RL = Sched<N>::R[NR - 1];
+    constexpr int RL = Sched<N, V>::R[NR - 1];
     constexpr int NBL = E / RL;
     cpx<T> a[E];
 #pragma unroll
     for (int b = 0; b < NB0; ++b)
 #pragma unroll
         for (int m = 0; m < R0; ++m) a[b * R0 + m] = v[b + m * NB0];
-    stage_compute<T, N, 0, DIR>(a, t, tw);
-    exchange_and_compute<T, N, 1, DIR>(a, lds, es, t, tw);
+    stage_compute<T, N, V, 0, DIR>(a, t, tw);
+    exchange_and_compute<T, N, V, 1, DIR>(a, lds, es, t, tw);
 #pragma unroll
     for (int b = 0; b < NBL; ++b)
 #pragma unroll

@@ -255,6 +255,20 @@ __global__ void k_copy(const T* src, T* dst, long n) {
         dst[i] = src[i];
 }
 
+// each thread moves 4 float4 per iteration with all loads issued before the stores
+__global__ void __launch_bounds__(256) k_copy4(const float4* __restrict__ src, float4* __restrict__ dst, long n4) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        const float4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n4; i += stride) dst[i] = src[i];
+}
+
 __device__ double block_sum(double v, double* red) {
     const int tid = threadIdx.x;
     red[tid] = v;
@@ -393,6 +407,15 @@ hipError_t launch_kgrad_energy(const cpx<T>* C, KGrid g, long nmain, long nk, do
     hipLaunchKernelGGL(k_kgrad_energy<T>, dim3((unsigned)nblocks), dim3(kRedThreads), 0, s, C, g, nmain, nk, partial);
     return hipGetLastError();
 }
+hipError_t launch_copy_bw(const float* src, float* dst, long n, hipStream_t s) {
+    const long n4 = n / 4;
+    long g = (n4 + 1023) / 1024;
+    if (g > 16384) g = 16384;
+    hipLaunchKernelGGL(k_copy4, dim3((unsigned)g), dim3(256), 0, s, reinterpret_cast<const float4*>(src),
+                       reinterpret_cast<float4*>(dst), n4);
+    return hipGetLastError();
+}
+
 hipError_t launch_reduce2(const double* a, long na, double ca, const double* b, long nb, double cb, double* out,
                           hipStream_t s) {
     hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kRedThreads), 0, s, a, na, ca, b, nb, cb, out);

@@ -136,7 +136,7 @@ template <typename T> hipError_t launch_rows_fused(const RowFusedArgs<T>& a, hip
 
 bool fft_size_supported(int n);      // sizes with a device transform
 bool fft_size_fast(int n);           // sizes with a compiled Stockham schedule
-int fft_threads_per_line(int n);     // threads per line of the transform
+int fft_threads_per_line(int n, int variant = 0);  // threads per line (variant 1: the update pass)
 
 // blocks a launch uses (for sizing per-block partial-sum buffers)
 long rows_blocks(long rows, int TW);

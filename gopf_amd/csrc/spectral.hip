@@ -29,16 +29,22 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr int kMaxBlock = kMaxBlockThreads;
 
 // elements per thread and threads per line of the transform of length n
-template <int N>
+template <int N, int V = 0>
 struct LineShape {
-    static constexpr int E = (N > 0) ? Sched<(N > 0 ? N : 2)>::E : 1;
-    static constexpr int T = (N > 0) ? Sched<(N > 0 ? N : 2)>::T : 0;  // 0: runtime (= n)
+    static constexpr int E = (N > 0) ? Sched<(N > 0 ? N : 2), V>::E : 1;
+    static constexpr int T = (N > 0) ? Sched<(N > 0 ? N : 2), V>::T : 0;  // 0: runtime (= n)
 };
 
-template <typename T, int N, int DIR>
+// schedule of the plain passes: the low-register variant once a thread would hold 24+ elements
+template <int N>
+constexpr int plain_variant() {
+    return (N > 0 && LineShape<N, 0>::E >= 24) ? 1 : 0;
+}
+
+template <typename T, int N, int DIR, int V = 0>
 __device__ __forceinline__ void line_fft(cpx<T>* v, cpx<T>* lds, int es, int n, int t, const cpx<T>* __restrict__ tw) {
     if constexpr (N > 0) {
-        fft_canon<T, N, DIR>(v, lds, es, t, tw);
+        fft_canon<T, N, DIR, V>(v, lds, es, t, tw);
     } else {
         dft_canon_rt<T, DIR>(v, lds, es, n, t, tw);
     }
@@ -48,6 +54,22 @@ __device__ __forceinline__ double wrapf(int k, int n) {
     double f = (double)k / (double)n;
     return (f > 0.5) ? f - 1.0 : f;
 }
+
+// 1/x in FP64: correctly rounded division for the plain FP64 path; hardware
+// reciprocal refined by two Newton steps (relative error ~1e-16) for FP32-bulk
+template <typename T>
+__device__ __forceinline__ double recip(double x) {
+    if constexpr (sizeof(T) == 8) {
+        return 1.0 / x;
+    } else {
+        double r = __builtin_amdgcn_rcp(x);
+        r = r * (2.0 - x * r);
+        r = r * (2.0 - x * r);
+        return r;
+    }
+}
+
+__device__ __forceinline__ int wrapk(int k, int n) { return (2 * k > n) ? k - n : k; }
 
 __device__ __forceinline__ double kpoly_eval(const KPoly& p, double L) {
     double acc = 0.0;
@@ -195,9 +217,10 @@ template <typename T, int N, int DIR>
 __global__ void __launch_bounds__(kMaxBlock) k_pencils(PencilArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
-    constexpr int E = LineShape<N>::E;
+    constexpr int V = plain_variant<N>();
+    constexpr int E = LineShape<N, V>::E;
     const int n = (N > 0) ? N : a.n;
-    const int TT = (N > 0) ? LineShape<N>::T : n;
+    const int TT = (N > 0) ? LineShape<N, V>::T : n;
     const TileInfo ti = decode_tile(a);
     const LineMap lm = line_map(ti.es == 1, a.TW, TT, n);
     const bool valid = lm.l < ti.nvalid;
@@ -210,7 +233,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencils(PencilArgs<T> a) {
             const cpx<T>* in = s ? s : out;
             cpx<T> v[E];
             load_line<T, E>(v, in, base, ti.es, lm.t, TT, valid);
-            line_fft<T, N, DIR>(v, lds + lm.off, lm.es, n, lm.t, a.tw);
+            line_fft<T, N, DIR, V>(v, lds + lm.off, lm.es, n, lm.t, a.tw);
             store_line<T, E>(v, out, base, ti.es, lm.t, TT, valid);
         }
     }
@@ -220,10 +243,11 @@ template <typename T, int N>
 __global__ void __launch_bounds__(kMaxBlock) k_pencil_update(PencilUpdateArgs<T> u) {
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
-    constexpr int E = LineShape<N>::E;
+    constexpr int V = 1;  // low-register schedule: three lines stay in registers
+    constexpr int E = LineShape<N, V>::E;
     const PencilArgs<T>& a = u.p;
     const int n = (N > 0) ? N : a.n;
-    const int TT = (N > 0) ? LineShape<N>::T : n;
+    const int TT = (N > 0) ? LineShape<N, V>::T : n;
     const TileInfo ti = decode_tile(a);
     const LineMap lm = line_map(ti.es == 1, a.TW, TT, n);
     const bool valid = lm.l < ti.nvalid;
@@ -231,64 +255,84 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencil_update(PencilUpdateArgs<T>
     cpx<T>* carr = pick(a.arr[0][0], a.arr[0][1], ti.s);
     const cpx<T>* garr = pick(a.arr[1][0], a.arr[1][1], ti.s);
     const double inx = 1.0 / (double)u.g.nx, iny = 1.0 / (double)u.g.ny, inz = 1.0 / (double)u.g.nz;
-    cpx<T> c[E], gs[E];
-    load_line<T, E>(c, carr, base, ti.es, lm.t, TT, valid);
-    line_fft<T, N, -1>(c, lds + lm.off, lm.es, n, lm.t, a.tw);
+    // 1) k-space multipliers of this thread's elements, computed before any data is
+    //    live to keep register pressure low.  The pencil coordinate p is one of
+    //    (kx, ky, kz); the other two are fixed for the line.
+    const int bl = ti.b0 + lm.l;
+    const int k0 = ksel(ti.ksrc0, ti.kfix0, ti.ia, bl, 0);
+    const int k1 = ksel(ti.ksrc1, ti.kfix1, ti.ia, bl, 0);
+    const int k2 = ksel(ti.ksrc2, ti.kfix2, ti.ia, bl, 0);
+    const int vary = (ti.ksrc0 == 2) ? 0 : (ti.ksrc1 == 2) ? 1 : 2;
+    const double f0 = wrapk(k0, u.g.nx) * inx, f1 = wrapk(k1, u.g.ny) * iny;
+    const double f2c = (u.g.ndim == 3) ? wrapk(k2, u.g.nz) * inz : 0.0;
+    const double fixed2 = (vary == 0 ? 0.0 : f0 * f0) + (vary == 1 ? 0.0 : f1 * f1) + (vary == 2 ? 0.0 : f2c * f2c);
+    const int nv = (vary == 0) ? u.g.nx : (vary == 1) ? u.g.ny : u.g.nz;
+    const double invv = (vary == 0) ? inx : (vary == 1) ? iny : inz;
+    const double wk = (k0 == 0 || 2 * k0 == u.g.nx) ? 1.0 : 2.0;  // Hermitian weight of the kx plane
+    // thread holding k = 0 (p = 0 is element 0 of thread 0 of the line)
+    const bool has_k0 = valid && u.mean && lm.t == 0 && (vary == 0 || k0 == 0) && (vary == 1 || k1 == 0) &&
+                        (vary == 2 || k2 == 0);
+    T s1a[E], s2a[E];
+    double m1_0 = 1.0, m2_0 = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int p = lm.t + TT * e;
+        const double fv = wrapk(p, nv) * invv;
+        const double f2 = fixed2 + fv * fv;
+        const double L = -(4.0 * kPi * kPi) * f2;
+        double A = recip<T>(1.0 - u.dt * kpoly_eval(u.denom, L));
+        if (u.filter) A *= vandeven(u.filter, sqrt(f2) * 2.0 / kPi);
+        double Lp = 1.0;
+        for (int q = 0; q < u.src_lap; ++q) Lp *= L;
+        const double m1 = A * (1.0 + u.dt * kpoly_eval(u.lin, L));
+        const double m2 = A * u.dt * Lp;
+        if (e == 0) {
+            m1_0 = m1;
+            m2_0 = m2;
+        }
+        s1a[e] = (T)(m1 * u.scale);
+        s2a[e] = (T)(m2 * u.scale);
+    }
+    // 2) explicit source: forward transform, scale by its multiplier
+    cpx<T> gs[E];
+    double g0 = 0.0;
     if (u.has_src) {
         load_line<T, E>(gs, garr, base, ti.es, lm.t, TT, valid);
-        line_fft<T, N, -1>(gs, lds + lm.off, lm.es, n, lm.t, a.tw);
+        line_fft<T, N, -1, V>(gs, lds + lm.off, lm.es, n, lm.t, a.tw);
+        g0 = (double)gs[0].x;
+#pragma unroll
+        for (int e = 0; e < E; ++e) gs[e] = mk<T>(s2a[e] * gs[e].x, s2a[e] * gs[e].y);
     } else {
 #pragma unroll
         for (int e = 0; e < E; ++e) gs[e] = mk<T>(T(0), T(0));
     }
-    // k-space update of this thread's elements
+    // 3) field: forward transform, semi-implicit update
+    cpx<T> c[E];
+    load_line<T, E>(c, carr, base, ti.es, lm.t, TT, valid);
+    line_fft<T, N, -1, V>(c, lds + lm.off, lm.es, n, lm.t, a.tw);
     double eg = 0.0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const int p = lm.t + TT * e;
-        const int bl = ti.b0 + lm.l;
-        int kc[3];
-        kc[0] = ksel(ti.ksrc0, ti.kfix0, ti.ia, bl, p);
-        kc[1] = ksel(ti.ksrc1, ti.kfix1, ti.ia, bl, p);
-        kc[2] = ksel(ti.ksrc2, ti.kfix2, ti.ia, bl, p);
-        const double fx = (double)(2 * kc[0] > u.g.nx ? kc[0] - u.g.nx : kc[0]) * inx;
-        const double fy = (double)(2 * kc[1] > u.g.ny ? kc[1] - u.g.ny : kc[1]) * iny;
-        const double fz = (u.g.ndim == 3) ? (double)(2 * kc[2] > u.g.nz ? kc[2] - u.g.nz : kc[2]) * inz : 0.0;
-        const double f2 = fx * fx + fy * fy + fz * fz;
-        const double L = -(4.0 * kPi * kPi) * f2;
-        const double D = kpoly_eval(u.denom, L);
-        const double Ek = kpoly_eval(u.lin, L);
-        double A = 1.0 / (1.0 - u.dt * D);
-        if (u.filter) A *= vandeven(u.filter, sqrt(f2) * 2.0 / kPi);
-        double Lp = 1.0;
-        for (int q = 0; q < u.src_lap; ++q) Lp *= L;
-        const double m1 = A * (1.0 + u.dt * Ek);
-        const double m2 = A * u.dt * Lp;
         const cpx<T> cv = c[e];
-        const cpx<T> gv = gs[e];
         if (valid && u.egrad_partial) {
-            const double w = (kc[0] == 0 || 2 * kc[0] == u.g.nx) ? 1.0 : 2.0;
-            eg += w * (-L) * ((double)cv.x * (double)cv.x + (double)cv.y * (double)cv.y);
+            const double fv = wrapk(lm.t + TT * e, nv) * invv;
+            eg += wk * 4.0 * kPi * kPi * (fixed2 + fv * fv) * ((double)cv.x * (double)cv.x + (double)cv.y * (double)cv.y);
         }
-        if (u.mean && kc[0] == 0 && kc[1] == 0 && kc[2] == 0) {
-            // k = 0 advances in FP64: the arrays hold c - mean, so their k = 0 coefficient stays 0
-            // and mean_new = m1(0) mean + m2(0) G(0) / N
-            if (valid) {
-                const double mu = *u.mean;
-                *u.mean = m1 * mu + m2 * (double)gv.x * u.scale;
-            }
-            c[e] = mk<T>(T(0), T(0));
-        } else {
-            const T s1 = (T)(m1 * u.scale), s2 = (T)(m2 * u.scale);
-            c[e] = mk<T>(s1 * cv.x + s2 * gv.x, s1 * cv.y + s2 * gv.y);
-        }
+        c[e] = mk<T>(s1a[e] * cv.x + gs[e].x, s1a[e] * cv.y + gs[e].y);
+    }
+    if (has_k0) {
+        // k = 0 advances in FP64: the arrays hold c - mean, so their k = 0 coefficient stays 0
+        // and mean_new = m1(0) mean + m2(0) G(0) / N
+        const double mu = *u.mean;
+        *u.mean = m1_0 * mu + m2_0 * g0 * u.scale;
+        c[0] = mk<T>(T(0), T(0));
     }
     if (u.egrad_partial) {
         double* red = reinterpret_cast<double*>(lds + lds_elems(n, a.TW));
         const double tot = block_sum(eg, red);
         if (threadIdx.x == 0) u.egrad_partial[blockIdx.x] = tot;
     }
-    line_fft<T, N, +1>(c, lds + lm.off, lm.es, n, lm.t, a.tw);
+    line_fft<T, N, +1, V>(c, lds + lm.off, lm.es, n, lm.t, a.tw);
     store_line<T, E>(c, carr, base, ti.es, lm.t, TT, valid);
 }
 
@@ -341,9 +385,10 @@ template <typename T, int N>
 __global__ void __launch_bounds__(kMaxBlock) k_rows_r2c(RowR2CArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
-    constexpr int E = LineShape<N>::E;
+    constexpr int V = plain_variant<N>();
+    constexpr int E = LineShape<N, V>::E;
     const int n = (N > 0) ? N : a.nx;
-    const int TT = (N > 0) ? LineShape<N>::T : n;
+    const int TT = (N > 0) ? LineShape<N, V>::T : n;
     const LineMap lm = line_map(true, a.TW, TT, n);
     const long row = (long)blockIdx.x * a.TW + lm.l;
     const bool valid = row < a.rows;
@@ -363,7 +408,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_r2c(RowR2CArgs<T> a) {
             }
             v[e] = mk<T>(x, y);
         }
-        line_fft<T, N, -1>(v, lds + lm.off, 1, n, lm.t, a.tw);
+        line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
         split_store<T, E>(v, lds + lm.off, n, lm.t, TT, row, valid, a.main[q], a.nyq[q],
                           ib ? a.main[q + 1] : nullptr, ib ? a.nyq[q + 1] : nullptr);
     }
@@ -373,9 +418,10 @@ template <typename T, int N>
 __global__ void __launch_bounds__(kMaxBlock) k_rows_c2r(RowC2RArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
-    constexpr int E = LineShape<N>::E;
+    constexpr int V = plain_variant<N>();
+    constexpr int E = LineShape<N, V>::E;
     const int n = (N > 0) ? N : a.nx;
-    const int TT = (N > 0) ? LineShape<N>::T : n;
+    const int TT = (N > 0) ? LineShape<N, V>::T : n;
     const LineMap lm = line_map(true, a.TW, TT, n);
     const long row = (long)blockIdx.x * a.TW + lm.l;
     const bool valid = row < a.rows;
@@ -393,7 +439,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_c2r(RowC2RArgs<T> a) {
 #pragma unroll
             for (int e = 0; e < E; ++e) v[e] = A[e];
         }
-        line_fft<T, N, +1>(v, lds + lm.off, 1, n, lm.t, a.tw);
+        line_fft<T, N, +1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
         if (valid) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
@@ -409,15 +455,16 @@ template <typename T, int N>
 __global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
-    constexpr int E = LineShape<N>::E;
+    constexpr int V = plain_variant<N>();
+    constexpr int E = LineShape<N, V>::E;
     const int n = (N > 0) ? N : a.nx;
-    const int TT = (N > 0) ? LineShape<N>::T : n;
+    const int TT = (N > 0) ? LineShape<N, V>::T : n;
     const LineMap lm = line_map(true, a.TW, TT, n);
     const long row = (long)blockIdx.x * a.TW + lm.l;
     const bool valid = row < a.rows;
     cpx<T> v[E];
     load_herm<T, E>(v, a.s_main, a.s_nyq, row, n, lm.t, TT, valid);
-    line_fft<T, N, +1>(v, lds + lm.off, 1, n, lm.t, a.tw);
+    line_fft<T, N, +1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
     // real space: c = (c - mean) + mean, g(c), optional output and FP64 partial sums
     KSum sb, sm;
     const double mu = a.mean ? *a.mean : 0.0;
@@ -449,13 +496,14 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
             a.mass_partial[blockIdx.x] = ms;
         }
     }
-    line_fft<T, N, -1>(v, lds + lm.off, 1, n, lm.t, a.tw);
+    line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
     split_store<T, E>(v, lds + lm.off, n, lm.t, TT, row, valid, a.c_main, a.c_nyq, a.g_main, a.g_nyq);
 }
 
-int threads_per_line(int n) {
+// v: 0 = plain pass, 1 = update pass
+int threads_per_line(int n, int v = 0) {
     switch (n) {
-#define GPF_T(NN) case NN: return Sched<NN>::T;
+#define GPF_T(NN) case NN: return (v || plain_variant<NN>()) ? Sched<NN, 1>::T : Sched<NN>::T;
         GPF_T(2) GPF_T(4) GPF_T(8) GPF_T(16) GPF_T(32) GPF_T(64) GPF_T(128) GPF_T(256) GPF_T(512)
         GPF_T(1024) GPF_T(12) GPF_T(24) GPF_T(48) GPF_T(96) GPF_T(192) GPF_T(384) GPF_T(768)
 #undef GPF_T
@@ -502,7 +550,7 @@ bool fft_size_fast(int n) {
 
 bool fft_size_supported(int n) { return fft_size_fast(n) || (n >= 1 && n <= 64); }
 
-int fft_threads_per_line(int n) { return threads_per_line(n); }
+int fft_threads_per_line(int n, int variant) { return threads_per_line(n, variant); }
 
 long rows_blocks(long rows, int TW) { return (rows + TW - 1) / TW; }
 
@@ -535,7 +583,7 @@ hipError_t launch_pencil_update(const PencilUpdateArgs<T>& u0, int n, hipStream_
     PencilUpdateArgs<T> u = u0;
     u.p.n = n;
     const long blocks = pencil_blocks(u.p.seg, u.p.nseg, u.p.TW, &u.p.tiles0);
-    const int thr = u.p.TW * threads_per_line(n);
+    const int thr = u.p.TW * threads_per_line(n, 1);
     size_t smem = lds_elems(n, u.p.TW) * sizeof(cpx<T>);
     if (u.egrad_partial) smem += (size_t)thr * sizeof(double);
     return dispatch_size(n, [&](auto NC) -> hipError_t {

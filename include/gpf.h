@@ -140,11 +140,18 @@ int gpf_solver_is_fused(const gpf_solver* s);
  * of each pass recorded with HIP events on the solver's stream */
 int gpf_solver_enable_timing(gpf_solver* s, int on);
 int gpf_solver_pass_times(gpf_solver* s, int maxpass, double* ms, long* launches, char* names, int namelen);
-/* lines per block chosen for the fused passes (rows, y forward, update, y inverse; 0 = default
- * heuristic); returns the number of values written */
+/* launch configuration of the fused step: lines per block of the rows, forward, update and
+ * inverse passes (0 = default heuristic) and, in 3D, the axis of the update pass (2 = z, 1 = y);
+ * returns the number of values written */
 int gpf_solver_tuning(gpf_solver* s, int* tws, int maxn);
 /* the HIP stream the solver launches on (hipStream_t) */
 void* gpf_solver_stream(gpf_solver* s);
+
+/* Device copy bandwidth probe: average milliseconds of one read+write copy of `bytes`
+ * bytes with the library's own copy kernel (no reference counterpart: the measured
+ * copy bandwidth the roofline is also reported against, and the known-size kernel
+ * the HBM traffic counters are calibrated with). */
+int gpf_device_copy_bandwidth(long bytes, int reps, int device, double* ms_per_copy);
 
 /* ---------------- pf.FourierTransform (pfutil.FFTWWrapper) ---------------- */
 
