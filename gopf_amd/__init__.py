@@ -99,6 +99,15 @@ def _declare(L) -> None:
         "gpf_solver_stream": (_vp, [_vp]),
         "gpf_solver_tuning": (C.c_int, [_vp, _ip, C.c_int]),
         "gpf_device_copy_bandwidth": (C.c_int, [C.c_long, C.c_int, C.c_int, _dp]),
+        "gpf_batch_create": (C.c_int, [_vp, C.c_int, _ip, _ip, C.c_double, C.POINTER(_Options), C.POINTER(_vp)]),
+        "gpf_batch_destroy": (None, [_vp]),
+        "gpf_batch_num_sims": (C.c_int, [_vp]),
+        "gpf_batch_set_scalar": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_double]),
+        "gpf_batch_set_field": (C.c_int, [_vp, C.c_int, _dp, C.c_long]),
+        "gpf_batch_get_field": (C.c_int, [_vp, C.c_int, _dp, C.c_long]),
+        "gpf_batch_propagate": (C.c_int, [_vp, C.c_int]),
+        "gpf_batch_synchronize": (C.c_int, [_vp]),
+        "gpf_batch_tuning": (C.c_int, [_vp, _ip, C.c_int]),
         "gpf_fft_create": (C.c_int, [_ip, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
         "gpf_fft_destroy": (None, [_vp]),
         "gpf_fft_forward": (C.c_int, [_vp, _dp, C.c_long]),
@@ -449,6 +458,69 @@ class FourierTransform:
 
     def ConjugateNode(self, i: int) -> int:
         return int(lib().gpf_fft_conjugate_node(_ints(self.Dimensions), len(self.Dimensions), int(i)))
+
+
+class Batch:
+    """Many independent single-field sims of uneven 2D sizes sharing one model.
+
+    The reference runs such a sweep as one pf.Solver per sim (pf/solver.go:40-63);
+    here every size class advances in one launch per pass, steps replay a HIP graph.
+    """
+
+    def __init__(self, model: Model, classes: Sequence[Tuple[Sequence[int], int]], dt: float,
+                 precision: str = "fp32", autotune: bool = True, device: int = 0):
+        o = _Options()
+        lib().gpf_options_default(C.byref(o))
+        o.precision = GPF_FP32 if precision in ("fp32", "float32") else GPF_FP64
+        o.autotune = 1 if autotune else 0
+        o.device = int(device)
+        shapes = []
+        counts = []
+        self.shapes = []
+        for shape, count in classes:
+            shapes += [int(shape[0]), int(shape[1])]
+            counts.append(int(count))
+            self.shapes += [tuple(shape)] * int(count)
+        h = _vp()
+        _check(lib().gpf_batch_create(model._h, len(counts), _ints(shapes), _ints(counts), float(dt), C.byref(o),
+                                      C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().gpf_batch_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def __len__(self) -> int:
+        return int(lib().gpf_batch_num_sims(self._h))
+
+    def SetScalar(self, sim: int, name: str, value: float) -> None:
+        _check(lib().gpf_batch_set_scalar(self._h, int(sim), name.encode(), float(value)))
+
+    def SetField(self, sim: int, data) -> None:
+        n = int(np.prod(self.shapes[sim]))
+        buf = _cbuf(data, n)
+        _check(lib().gpf_batch_set_field(self._h, int(sim), _dptr(buf), n))
+
+    def Field(self, sim: int) -> np.ndarray:
+        n = int(np.prod(self.shapes[sim]))
+        out = np.zeros(n, dtype=np.complex128)
+        _check(lib().gpf_batch_get_field(self._h, int(sim), _dptr(out), n))
+        return out
+
+    def Propagate(self, nsteps: int) -> None:
+        _check(lib().gpf_batch_propagate(self._h, int(nsteps)))
+
+    def Synchronize(self) -> None:
+        _check(lib().gpf_batch_synchronize(self._h))
+
+    def Tuning(self) -> List[int]:
+        v = (C.c_int * 64)()
+        k = lib().gpf_batch_tuning(self._h, v, 64)
+        return [v[i] for i in range(min(k, 64))]
 
 
 def CopyBandwidth(nbytes: int, reps: int = 20, device: int = 0) -> float:

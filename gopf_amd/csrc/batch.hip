@@ -1,0 +1,426 @@
+// batch.hip -- many independent single-field simulations of uneven 2D sizes.
+//
+// A parameter sweep that the reference would run as many pf.Solver instances
+// (each sim stored under its own simID in a FieldDB, pf/database.go:67-80)
+// runs here as size classes: all sims of one grid shape are stacked into one
+// array whose slowest axis is the sim index, so one launch of the fused row
+// pass and one of the fused y-pencil update pass advance every sim of a class.
+// Per-sim scalars become per-sim coefficient tables (the explicit source
+// polynomial, the implicit denominator and the linear explicit terms) and a
+// per-sim FP64 mean.  Steps are captured in a HIP graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <map>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/gpf.h"
+#include "batch.h"
+#include "fft_core.h"
+#include "model.h"
+#include "spectral.h"
+
+namespace gpf {
+
+namespace {
+
+#define BCHK(x) HIPCHK(x)
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t c) { alloc(c); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t c) {
+        if (p) (void)hipFree(p);
+        BCHK(hipMalloc(reinterpret_cast<void**>(&p), (c ? c : 1) * sizeof(T)));
+        n = c;
+    }
+};
+
+template <typename T>
+std::vector<cpx<T>> twiddles(int n) {
+    std::vector<cpx<T>> w(n);
+    for (int q = 0; q < n; ++q) {
+        long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)q / (long double)n;
+        w[q].x = (T)cosl(a);
+        w[q].y = (T)sinl(a);
+    }
+    return w;
+}
+
+int pick_tw(int n, int variant) {
+    const int T = fft_threads_per_line(n, variant);
+    int tw = 1;
+    while (tw * 2 * T <= 256) tw *= 2;
+    return tw > 64 ? 64 : tw;
+}
+
+bool tw_ok(int n, int tw, size_t elem, int variant) {
+    return tw * fft_threads_per_line(n, variant) <= kMaxBlockThreads &&
+           (size_t)(n + n / 16 + 1) * (tw + 1) * elem <= 150 * 1024;
+}
+
+}  // namespace
+
+// one size class: nsim sims of ny x nx
+template <typename T>
+struct BatchClass {
+    int ny = 0, nx = 0, nsim = 0, first = 0;
+    long nmain = 0, nk = 0, rows = 0;
+    DBuf<cpx<T>> X, Y, G, xtw, ytw;
+    DBuf<double> mean, gpoly, denom, lin;
+    int tw_rows = 0, tw_upd = 0;
+};
+
+template <typename T>
+class BatchEngine : public BatchBase {
+public:
+    BatchEngine(const Model& m, int ncls, const int* shapes, const int* counts, double dt, const gpf_options& o)
+        : base_(m), dt_(dt), autotune_(o.autotune != 0) {
+        if (m.field_names.size() != 1) throw BatchError(GPF_EINVAL, "batched sims hold exactly one field");
+        if (o.stepper != GPF_EULER) throw BatchError(GPF_EINVAL, "batched sims use the Euler stepper");
+        BCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+        int first = 0;
+        for (int c = 0; c < ncls; ++c) {
+            auto cl = std::make_unique<BatchClass<T>>();
+            cl->ny = shapes[2 * c];
+            cl->nx = shapes[2 * c + 1];
+            cl->nsim = counts[c];
+            cl->first = first;
+            if (cl->nx % 2 || !fft_size_supported(cl->nx) || !fft_size_supported(cl->ny) || cl->nsim < 1)
+                throw BatchError(GPF_EINVAL, "unsupported batch class shape");
+            cl->rows = (long)cl->nsim * cl->ny;
+            cl->nmain = cl->rows * (cl->nx / 2);
+            cl->nk = cl->nmain + cl->rows;
+            cl->X.alloc(cl->nk);
+            cl->Y.alloc(cl->nk);
+            cl->G.alloc(cl->nk);
+            cl->mean.alloc(cl->nsim);
+            BCHK(hipMemsetAsync(cl->X.p, 0, cl->nk * sizeof(cpx<T>), st_));
+            BCHK(hipMemsetAsync(cl->mean.p, 0, cl->nsim * sizeof(double), st_));
+            auto wx = twiddles<T>(cl->nx), wy = twiddles<T>(cl->ny);
+            cl->xtw.alloc(cl->nx);
+            cl->ytw.alloc(cl->ny);
+            BCHK(hipMemcpy(cl->xtw.p, wx.data(), wx.size() * sizeof(cpx<T>), hipMemcpyHostToDevice));
+            BCHK(hipMemcpy(cl->ytw.p, wy.data(), wy.size() * sizeof(cpx<T>), hipMemcpyHostToDevice));
+            first += cl->nsim;
+            cls_.push_back(std::move(cl));
+        }
+        nsims_ = first;
+        scalars_.assign(nsims_, m.scalars);
+        dirty_ = true;
+        BCHK(hipStreamSynchronize(st_));
+    }
+
+    ~BatchEngine() override {
+        drop_graphs();
+        if (st_) {
+            (void)hipStreamSynchronize(st_);
+            (void)hipStreamDestroy(st_);
+        }
+    }
+
+    int nsims() const override { return nsims_; }
+
+    void set_scalar(int sim, const std::string& name, double v) override {
+        check_sim(sim);
+        if (!base_.scalars.count(name)) throw BatchError(GPF_EINVAL, "unknown scalar " + name);
+        scalars_[sim][name] = {v, 0.0};
+        dirty_ = true;
+    }
+
+    void set_field(int sim, const double* in, long n) override {
+        BatchClass<T>& c = cls_of(sim);
+        const int ls = sim - c.first;
+        if (n != (long)c.ny * c.nx) throw BatchError(GPF_EINVAL, "buffer length does not match the sim grid");
+        double s = 0.0, comp = 0.0;
+        for (long j = 0; j < n; ++j) {
+            const double v = in[2 * j], t = s + v;
+            comp += (std::fabs(s) >= std::fabs(v)) ? ((s - t) + v) : ((v - t) + s);
+            s = t;
+        }
+        const bool split = sizeof(T) == 4;
+        const double mean = split ? (s + comp) / (double)n : 0.0;
+        std::vector<T> h(n);
+        for (long j = 0; j < n; ++j) h[j] = (T)(in[2 * j] - mean);
+        DBuf<T> tmp(n);
+        BCHK(hipMemcpyAsync(tmp.p, h.data(), n * sizeof(T), hipMemcpyHostToDevice, st_));
+        RowR2CArgs<T> a;
+        a.tw = c.xtw.p;
+        a.narr = 1;
+        a.in[0] = tmp.p;
+        a.main[0] = c.X.p + (long)ls * c.ny * (c.nx / 2);
+        a.nyq[0] = c.X.p + c.nmain + (long)ls * c.ny;
+        a.rows = c.ny;
+        a.nx = c.nx;
+        a.TW = pick_tw(c.nx, 0);
+        a.in_scale = T(1) / (T)c.nx;
+        BCHK(launch_rows_r2c<T>(a, st_));
+        BCHK(hipMemcpyAsync(c.mean.p + ls, &mean, sizeof(double), hipMemcpyHostToDevice, st_));
+        BCHK(hipStreamSynchronize(st_));
+    }
+
+    void get_field(int sim, double* out, long n) override {
+        BatchClass<T>& c = cls_of(sim);
+        const int ls = sim - c.first;
+        if (n != (long)c.ny * c.nx) throw BatchError(GPF_EINVAL, "buffer length does not match the sim grid");
+        DBuf<T> tmp(n);
+        RowC2RArgs<T> a;
+        a.tw = c.xtw.p;
+        a.narr = 1;
+        a.main[0] = c.X.p + (long)ls * c.ny * (c.nx / 2);
+        a.nyq[0] = c.X.p + c.nmain + (long)ls * c.ny;
+        a.out[0] = tmp.p;
+        a.rows = c.ny;
+        a.nx = c.nx;
+        a.TW = pick_tw(c.nx, 0);
+        a.scale = T(1);
+        BCHK(launch_rows_c2r<T>(a, st_));
+        std::vector<T> h(n);
+        double mean = 0.0;
+        BCHK(hipMemcpyAsync(h.data(), tmp.p, n * sizeof(T), hipMemcpyDeviceToHost, st_));
+        BCHK(hipMemcpyAsync(&mean, c.mean.p + ls, sizeof(double), hipMemcpyDeviceToHost, st_));
+        BCHK(hipStreamSynchronize(st_));
+        if (sizeof(T) != 4) mean = 0.0;
+        for (long j = 0; j < n; ++j) {
+            out[2 * j] = (double)h[j] + mean;
+            out[2 * j + 1] = 0.0;
+        }
+    }
+
+    void propagate(int nsteps) override {
+        if (dirty_) compile();
+        if (autotune_ && !tuned_) tune();
+        int done = 0;
+        // graphs of kGraphSteps steps (even, so the state buffers return to their places);
+        // one graph per parity of the state-buffer assignment
+        if (nsteps >= kGraphSteps) {
+            if (!graph_exec_[parity_]) capture();
+            while (nsteps - done >= kGraphSteps) {
+                BCHK(hipGraphLaunch(graph_exec_[parity_], st_));
+                done += kGraphSteps;
+            }
+        }
+        for (; done < nsteps; ++done) {
+            enqueue_step();
+            parity_ ^= 1;
+        }
+    }
+
+    void synchronize() override { BCHK(hipStreamSynchronize(st_)); }
+
+    int tuning(int* out, int maxn) override {
+        int k = 0;
+        for (auto& c : cls_) {
+            if (k < maxn) out[k] = c->tw_rows;
+            ++k;
+            if (k < maxn) out[k] = c->tw_upd;
+            ++k;
+        }
+        return k;
+    }
+
+private:
+    static constexpr int kGraphSteps = 10;
+
+    void check_sim(int sim) const {
+        if (sim < 0 || sim >= nsims_) throw BatchError(GPF_EINVAL, "sim index out of range");
+    }
+
+    BatchClass<T>& cls_of(int sim) {
+        check_sim(sim);
+        for (auto& c : cls_)
+            if (sim >= c->first && sim < c->first + c->nsim) return *c;
+        throw BatchError(GPF_EINVAL, "sim index out of range");
+    }
+
+    // per-sim coefficients: parse the model with each sim's scalars
+    void compile() {
+        std::vector<double> gp(kMaxPoly * (size_t)nsims_, 0.0), de(kMaxKTerms * (size_t)nsims_, 0.0),
+            li(kMaxKTerms * (size_t)nsims_, 0.0);
+        bool first = true;
+        for (int s = 0; s < nsims_; ++s) {
+            Model m = base_;
+            m.scalars = scalars_[s];
+            m.init();
+            FusedShape fs;
+            std::string why;
+            if (!fused_shape(m, &fs, &why)) throw BatchError(GPF_EINVAL, "batched sims need the fused class: " + why);
+            if (first) {
+                shape_ = fs;
+                first = false;
+            } else if (!same_structure(shape_, fs)) {
+                throw BatchError(GPF_EINVAL, "every sim must have the same term structure");
+            }
+            for (int q = 0; q < kMaxPoly && q < (int)fs.gpoly.size(); ++q) gp[s * kMaxPoly + q] = fs.gpoly[q];
+            for (int t = 0; t < fs.denom.n; ++t) de[s * kMaxKTerms + t] = fs.denom.coef[t];
+            for (int t = 0; t < fs.lin.n; ++t) li[s * kMaxKTerms + t] = fs.lin.coef[t];
+        }
+        npoly_ = 0;
+        for (int s = 0; s < nsims_; ++s)
+            for (int q = 0; q < kMaxPoly; ++q)
+                if (gp[s * kMaxPoly + q] != 0.0 && q + 1 > npoly_) npoly_ = q + 1;
+        for (auto& c : cls_) {
+            c->gpoly.alloc((size_t)kMaxPoly * c->nsim);
+            c->denom.alloc((size_t)kMaxKTerms * c->nsim);
+            c->lin.alloc((size_t)kMaxKTerms * c->nsim);
+            BCHK(hipMemcpy(c->gpoly.p, gp.data() + (size_t)kMaxPoly * c->first, (size_t)kMaxPoly * c->nsim * sizeof(double),
+                           hipMemcpyHostToDevice));
+            BCHK(hipMemcpy(c->denom.p, de.data() + (size_t)kMaxKTerms * c->first,
+                           (size_t)kMaxKTerms * c->nsim * sizeof(double), hipMemcpyHostToDevice));
+            BCHK(hipMemcpy(c->lin.p, li.data() + (size_t)kMaxKTerms * c->first, (size_t)kMaxKTerms * c->nsim * sizeof(double),
+                           hipMemcpyHostToDevice));
+        }
+        drop_graphs();
+        dirty_ = false;
+    }
+
+    RowFusedArgs<T> rows_args(BatchClass<T>& c, cpx<T>* X, cpx<T>* Y, cpx<T>* G, double* mean, int tw) {
+        RowFusedArgs<T> ra;
+        ra.tw = c.xtw.p;
+        ra.s_main = X;
+        ra.s_nyq = X + c.nmain;
+        ra.c_main = Y;
+        ra.c_nyq = Y + c.nmain;
+        ra.g_main = G;
+        ra.g_nyq = G + c.nmain;
+        ra.rows = c.rows;
+        ra.nx = c.nx;
+        ra.TW = tw > 0 ? tw : pick_tw(c.nx, 0);
+        ra.npoly = npoly_;
+        ra.mean = (sizeof(T) == 4) ? mean : nullptr;
+        ra.rows_per_sim = c.ny;
+        ra.gpoly_sim = c.gpoly.p;
+        return ra;
+    }
+
+    PencilUpdateArgs<T> upd_args(BatchClass<T>& c, cpx<T>* Y, cpx<T>* G, double* mean, int tw) {
+        PencilUpdateArgs<T> u;
+        u.p.tw = c.ytw.p;
+        u.p.narr = shape_.has_src ? 2 : 1;
+        u.p.arr[0][0] = Y;
+        u.p.arr[0][1] = Y + c.nmain;
+        u.p.arr[1][0] = G;
+        u.p.arr[1][1] = G + c.nmain;
+        Seg m;  // y-pencils of every sim: a = sim, b = kx
+        m.na = c.nsim; m.nb = c.nx / 2; m.sa = (long)c.ny * (c.nx / 2); m.sb = 1; m.es = c.nx / 2;
+        m.ksrc[0] = 1; m.ksrc[1] = 2; m.ksrc[2] = 3; m.sim_src = 0;
+        Seg q;  // x-Nyquist column of every sim: contiguous line per sim
+        q.na = 1; q.nb = c.nsim; q.sa = 0; q.sb = c.ny; q.es = 1;
+        q.ksrc[0] = 3; q.kfix[0] = c.nx / 2; q.ksrc[1] = 2; q.ksrc[2] = 3; q.sim_src = 1;
+        u.p.seg[0] = m;
+        u.p.seg[1] = q;
+        u.p.nseg = 2;
+        u.p.TW = tw > 0 ? tw : pick_tw(c.ny, 1);
+        u.g.nx = c.nx;
+        u.g.ny = c.ny;
+        u.g.nz = 1;
+        u.g.ndim = 2;
+        u.dt = dt_;
+        u.scale = 1.0 / ((double)c.nx * c.ny);
+        u.denom = shape_.denom;
+        u.lin = shape_.lin;
+        u.src_lap = shape_.src_lap;
+        u.has_src = shape_.has_src ? 1 : 0;
+        u.mean = (sizeof(T) == 4) ? mean : nullptr;
+        u.denom_sim = c.denom.p;
+        u.lin_sim = c.lin.p;
+        return u;
+    }
+
+    void enqueue_step() {
+        for (auto& cp : cls_) {
+            BatchClass<T>& c = *cp;
+            BCHK(launch_rows_fused<T>(rows_args(c, c.X.p, c.Y.p, c.G.p, c.mean.p, c.tw_rows), st_));
+            BCHK(launch_pencil_update<T>(upd_args(c, c.Y.p, c.G.p, c.mean.p, c.tw_upd), c.ny, st_));
+            std::swap(c.X.p, c.Y.p);
+        }
+    }
+
+    void capture() {
+        hipGraph_t g = nullptr;
+        BCHK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+        for (int s = 0; s < kGraphSteps; ++s) enqueue_step();
+        BCHK(hipStreamEndCapture(st_, &g));
+        BCHK(hipGraphInstantiate(&graph_exec_[parity_], g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+    }
+
+    // batch layout autotuner: lines per block of both passes of every class, timed on scratch copies
+    void tune() {
+        hipEvent_t e0, e1;
+        BCHK(hipEventCreate(&e0));
+        BCHK(hipEventCreate(&e1));
+        const int cand[] = {4, 8, 16, 32, 64};
+        for (auto& cp : cls_) {
+            BatchClass<T>& c = *cp;
+            DBuf<cpx<T>> X(c.nk), Y(c.nk), G(c.nk);
+            DBuf<double> mu(c.nsim);
+            BCHK(hipMemcpyAsync(X.p, c.X.p, c.nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+            BCHK(hipMemcpyAsync(mu.p, c.mean.p, c.nsim * sizeof(double), hipMemcpyDeviceToDevice, st_));
+            auto time_it = [&](auto&& launch) {
+                launch();
+                BCHK(hipEventRecord(e0, st_));
+                for (int r = 0; r < 3; ++r) launch();
+                BCHK(hipEventRecord(e1, st_));
+                BCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                BCHK(hipEventElapsedTime(&ms, e0, e1));
+                return ms;
+            };
+            float best = 1e30f;
+            for (int tw : cand) {
+                if (!tw_ok(c.nx, tw, sizeof(cpx<T>), 0)) continue;
+                const float t = time_it([&] { BCHK(launch_rows_fused<T>(rows_args(c, X.p, Y.p, G.p, mu.p, tw), st_)); });
+                if (t < best) { best = t; c.tw_rows = tw; }
+            }
+            best = 1e30f;
+            for (int tw : cand) {
+                if (!tw_ok(c.ny, tw, sizeof(cpx<T>), 1)) continue;
+                const float t = time_it([&] { BCHK(launch_pencil_update<T>(upd_args(c, Y.p, G.p, mu.p, tw), c.ny, st_)); });
+                if (t < best) { best = t; c.tw_upd = tw; }
+            }
+            BCHK(hipStreamSynchronize(st_));
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        drop_graphs();
+        tuned_ = true;
+    }
+
+    Model base_;
+    double dt_;
+    bool autotune_ = false, tuned_ = false, dirty_ = true;
+    hipStream_t st_ = nullptr;
+    std::vector<std::unique_ptr<BatchClass<T>>> cls_;
+    std::vector<std::map<std::string, std::complex<double>>> scalars_;
+    int nsims_ = 0, npoly_ = 0;
+    FusedShape shape_;
+    hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};
+    int parity_ = 0;
+
+    void drop_graphs() {
+        for (auto& g : graph_exec_) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    }
+};
+
+BatchBase* make_batch(const Model& m, int ncls, const int* shapes, const int* counts, double dt, const gpf_options& o) {
+    if (o.precision == GPF_FP32) return new BatchEngine<float>(m, ncls, shapes, counts, dt, o);
+    return new BatchEngine<double>(m, ncls, shapes, counts, dt, o);
+}
+
+}  // namespace gpf

@@ -29,6 +29,8 @@
 #include <vector>
 
 #include "../../include/gpf.h"
+#include "batch.h"
+#include "common.h"
 #include "fft_core.h"
 #include "model.h"
 #include "pointwise.h"
@@ -36,16 +38,6 @@
 
 namespace gpf {
 
-struct Error : std::runtime_error {
-    int code;
-    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIPCHK(x)                                                                                    \
-    do {                                                                                             \
-        hipError_t e_ = (x);                                                                         \
-        if (e_ != hipSuccess) throw Error(GPF_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
-    } while (0)
 
 static thread_local std::string g_last_error;
 
@@ -294,6 +286,29 @@ static FusedPlan compile_fused(const Model& m, int stepper) {
     fp.gpoly.assign(g.begin(), g.begin() + deg);
     fp.ok = true;
     return fp;
+}
+
+bool fused_shape(const Model& m, FusedShape* out, std::string* why) {
+    FusedPlan fp = compile_fused(m, GPF_EULER);
+    if (!fp.ok) {
+        if (why) *why = fp.why;
+        return false;
+    }
+    out->gpoly = fp.gpoly;
+    out->src_lap = fp.src_lap;
+    out->has_src = fp.has_src;
+    out->denom = fp.denom;
+    out->lin = fp.lin;
+    return true;
+}
+
+bool same_structure(const FusedShape& a, const FusedShape& b) {
+    if (a.has_src != b.has_src || a.src_lap != b.src_lap || a.denom.n != b.denom.n || a.lin.n != b.lin.n) return false;
+    for (int t = 0; t < a.denom.n; ++t)
+        if (a.denom.pow[t] != b.denom.pow[t]) return false;
+    for (int t = 0; t < a.lin.n; ++t)
+        if (a.lin.pow[t] != b.lin.pow[t]) return false;
+    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -1370,6 +1385,11 @@ struct gpf_fft {
     int device = 0;
 };
 
+struct gpf_batch {
+    std::unique_ptr<BatchBase> impl;
+    int device = 0;
+};
+
 template <typename F>
 static int guarded(F&& f) {
     try {
@@ -1695,6 +1715,77 @@ int gpf_solver_tuning(gpf_solver* s, int* tws, int maxn) {
 }
 
 void* gpf_solver_stream(gpf_solver* s) { return (s && s->eng) ? (void*)s->eng->stream() : nullptr; }
+
+int gpf_batch_create(gpf_model* m, int nclasses, const int* shapes, const int* counts, double dt,
+                     const gpf_options* o, gpf_batch** out) {
+    return guarded([&] {
+        if (!m || !shapes || !counts || !out || nclasses < 1) throw Error(GPF_EINVAL, "bad argument");
+        if (!m->initialised) throw Error(GPF_ESTATE, "model is not initialised (call gpf_model_init)");
+        gpf_options opt;
+        gpf_options_default(&opt);
+        if (o) opt = *o;
+        set_device(opt.device);
+        auto b = std::make_unique<gpf_batch>();
+        b->device = opt.device;
+        b->impl.reset(make_batch(m->m, nclasses, shapes, counts, dt, opt));
+        *out = b.release();
+    });
+}
+
+void gpf_batch_destroy(gpf_batch* b) {
+    if (!b) return;
+    try {
+        set_device(b->device);
+    } catch (...) {
+    }
+    delete b;
+}
+
+int gpf_batch_num_sims(const gpf_batch* b) { return (b && b->impl) ? b->impl->nsims() : -1; }
+
+int gpf_batch_set_scalar(gpf_batch* b, int sim, const char* name, double value) {
+    return guarded([&] {
+        if (!b || !name) throw Error(GPF_EINVAL, "bad argument");
+        b->impl->set_scalar(sim, name, value);
+    });
+}
+
+int gpf_batch_set_field(gpf_batch* b, int sim, const double* data, long n) {
+    return guarded([&] {
+        if (!b || !data) throw Error(GPF_EINVAL, "bad argument");
+        set_device(b->device);
+        b->impl->set_field(sim, data, n);
+    });
+}
+
+int gpf_batch_get_field(gpf_batch* b, int sim, double* out, long n) {
+    return guarded([&] {
+        if (!b || !out) throw Error(GPF_EINVAL, "bad argument");
+        set_device(b->device);
+        b->impl->get_field(sim, out, n);
+    });
+}
+
+int gpf_batch_propagate(gpf_batch* b, int nsteps) {
+    return guarded([&] {
+        if (!b || nsteps < 0) throw Error(GPF_EINVAL, "bad argument");
+        set_device(b->device);
+        b->impl->propagate(nsteps);
+    });
+}
+
+int gpf_batch_synchronize(gpf_batch* b) {
+    return guarded([&] {
+        if (!b) throw Error(GPF_EINVAL, "bad argument");
+        set_device(b->device);
+        b->impl->synchronize();
+    });
+}
+
+int gpf_batch_tuning(gpf_batch* b, int* tws, int maxn) {
+    if (!b || !tws) return -GPF_EINVAL;
+    return b->impl->tuning(tws, maxn);
+}
 
 int gpf_fft_create(const int* dims, int ndim, int precision, int device, gpf_fft** out) {
     return guarded([&] {

@@ -30,6 +30,7 @@ struct Seg {
     long sa = 0, sb = 0, es = 0;
     int ksrc[3] = {3, 3, 3};    // source of (kx, ky, kz): 0 = a, 1 = b, 2 = p, 3 = fixed
     int kfix[3] = {0, 0, 0};
+    int sim_src = -1;           // batched sims: index of the sim from 0 = a, 1 = b; -1 none
 };
 
 // k-space multiplier sum  M(k) = sum_t coef_t * L(k)^pow_t,  L = -(2 pi |f|)^2
@@ -73,6 +74,10 @@ struct PencilUpdateArgs {
     double* mean = nullptr;              // FP64 k = 0 mode: the field mean (in/out), or null when the
                                          // k = 0 coefficient is carried by the arrays themselves
     double* egrad_partial = nullptr;     // per-block FP64 sum of w |k|^2 |C|^2 or null
+    // batched sims (Seg::sim_src >= 0): per-sim coefficients of denom / lin ([sim][kMaxKTerms],
+    // powers from denom / lin) and mean[sim]
+    const double* denom_sim = nullptr;
+    const double* lin_sim = nullptr;
 };
 
 // x-row R2C of pairs of real arrays into main/nyq half spectra
@@ -125,6 +130,9 @@ struct RowFusedArgs {
     double bulk[kMaxPoly];
     double* bulk_partial = nullptr;      // per-block FP64 sum of f(c)
     double* mass_partial = nullptr;      // per-block FP64 sum of c
+    // batched sims: rows_per_sim > 0 selects gpoly_sim[sim][kMaxPoly] and mean[sim], sim = row / rows_per_sim
+    long rows_per_sim = 0;
+    const double* gpoly_sim = nullptr;
 };
 
 // launchers (implemented in spectral.hip); n is the line length.

@@ -129,6 +129,7 @@ struct TileInfo {
     int nvalid;   // valid lines in the tile
     long sb, es;  // line and element strides of the segment
     int ksrc0, ksrc1, ksrc2, kfix0, kfix1, kfix2;
+    int sim_src;
 };
 
 // The segment is chosen with selects (no dynamic indexing of the argument
@@ -150,6 +151,7 @@ __device__ __forceinline__ TileInfo decode_tile(const PencilArgs<T>& a) {
     ti.kfix0 = s1 ? a.seg[1].kfix[0] : a.seg[0].kfix[0];
     ti.kfix1 = s1 ? a.seg[1].kfix[1] : a.seg[0].kfix[1];
     ti.kfix2 = s1 ? a.seg[1].kfix[2] : a.seg[0].kfix[2];
+    ti.sim_src = s1 ? a.seg[1].sim_src : a.seg[0].sim_src;
     const int tpa = (nb + a.TW - 1) / a.TW;
     ti.ia = tile / tpa;
     ti.b0 = (tile - ti.ia * tpa) * a.TW;
@@ -269,8 +271,13 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencil_update(PencilUpdateArgs<T>
     const int nv = (vary == 0) ? u.g.nx : (vary == 1) ? u.g.ny : u.g.nz;
     const double invv = (vary == 0) ? inx : (vary == 1) ? iny : inz;
     const double wk = (k0 == 0 || 2 * k0 == u.g.nx) ? 1.0 : 2.0;  // Hermitian weight of the kx plane
+    // batched sims: this line's sim selects its coefficients and mean
+    const int sim = (ti.sim_src == 0) ? ti.ia : (ti.sim_src == 1) ? bl : -1;
+    const double* dsim = (sim >= 0) ? u.denom_sim + sim * kMaxKTerms : nullptr;
+    const double* lsim = (sim >= 0) ? u.lin_sim + sim * kMaxKTerms : nullptr;
+    double* mean = (u.mean && sim >= 0) ? u.mean + sim : u.mean;
     // thread holding k = 0 (p = 0 is element 0 of thread 0 of the line)
-    const bool has_k0 = valid && u.mean && lm.t == 0 && (vary == 0 || k0 == 0) && (vary == 1 || k1 == 0) &&
+    const bool has_k0 = valid && mean && lm.t == 0 && (vary == 0 || k0 == 0) && (vary == 1 || k1 == 0) &&
                         (vary == 2 || k2 == 0);
     T s1a[E], s2a[E];
     double m1_0 = 1.0, m2_0 = 0.0;
@@ -280,11 +287,28 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencil_update(PencilUpdateArgs<T>
         const double fv = wrapk(p, nv) * invv;
         const double f2 = fixed2 + fv * fv;
         const double L = -(4.0 * kPi * kPi) * f2;
-        double A = recip<T>(1.0 - u.dt * kpoly_eval(u.denom, L));
+        double D = 0.0, Ek = 0.0;
+        if (dsim) {
+            // per-sim coefficients (global memory), powers shared (kernel argument)
+            for (int t = 0; t < u.denom.n; ++t) {
+                double v = dsim[t];
+                for (int q = 0; q < u.denom.pow[t]; ++q) v *= L;
+                D += v;
+            }
+            for (int t = 0; t < u.lin.n; ++t) {
+                double v = lsim[t];
+                for (int q = 0; q < u.lin.pow[t]; ++q) v *= L;
+                Ek += v;
+            }
+        } else {
+            D = kpoly_eval(u.denom, L);
+            Ek = kpoly_eval(u.lin, L);
+        }
+        double A = recip<T>(1.0 - u.dt * D);
         if (u.filter) A *= vandeven(u.filter, sqrt(f2) * 2.0 / kPi);
         double Lp = 1.0;
         for (int q = 0; q < u.src_lap; ++q) Lp *= L;
-        const double m1 = A * (1.0 + u.dt * kpoly_eval(u.lin, L));
+        const double m1 = A * (1.0 + u.dt * Ek);
         const double m2 = A * u.dt * Lp;
         if (e == 0) {
             m1_0 = m1;
@@ -323,8 +347,8 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencil_update(PencilUpdateArgs<T>
     if (has_k0) {
         // k = 0 advances in FP64: the arrays hold c - mean, so their k = 0 coefficient stays 0
         // and mean_new = m1(0) mean + m2(0) G(0) / N
-        const double mu = *u.mean;
-        *u.mean = m1_0 * mu + m2_0 * g0 * u.scale;
+        const double mu = *mean;
+        *mean = m1_0 * mu + m2_0 * g0 * u.scale;
         c[0] = mk<T>(T(0), T(0));
     }
     if (u.egrad_partial) {
@@ -467,21 +491,29 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
     line_fft<T, N, +1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
     // real space: c = (c - mean) + mean, g(c), optional output and FP64 partial sums
     KSum sb, sm;
-    const double mu = a.mean ? *a.mean : 0.0;
+    const long sim = (a.rows_per_sim > 0 && valid) ? row / a.rows_per_sim : -1;
+    const double mu = a.mean ? a.mean[sim >= 0 ? sim : 0] : 0.0;
     const T muT = (T)mu;
+    T gp[kMaxPoly];
+#pragma unroll
+    for (int q = 0; q < kMaxPoly; ++q) gp[q] = (T)((sim >= 0) ? a.gpoly_sim[sim * kMaxPoly + q] : a.gpoly[q]);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const T dc = v[e].x;
         const T c = dc + muT;
         T gv = T(0);
-        for (int q = a.npoly - 1; q >= 0; --q) gv = gv * c + (T)a.gpoly[q];
+#pragma unroll
+        for (int q = kMaxPoly - 1; q >= 0; --q)
+            if (q < a.npoly) gv = gv * c + gp[q];
         v[e] = mk<T>(dc, gv);
         if (valid) {
             if (a.c_out) a.c_out[row * n + lm.t + TT * e] = c;
             if (a.bulk_partial) {
                 const double cd = (double)dc + mu;
                 double f = 0.0;
-                for (int q = a.nbulk - 1; q >= 0; --q) f = f * cd + a.bulk[q];
+#pragma unroll
+                for (int q = kMaxPoly - 1; q >= 0; --q)
+                    if (q < a.nbulk) f = f * cd + a.bulk[q];
                 sb.add(f);
                 sm.add(cd);
             }

@@ -51,6 +51,7 @@ extern "C" {
 typedef struct gpf_model gpf_model;
 typedef struct gpf_solver gpf_solver;
 typedef struct gpf_fft gpf_fft;
+typedef struct gpf_batch gpf_batch;
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* gpf_last_error(void);
@@ -165,6 +166,27 @@ int gpf_fft_inverse(gpf_fft* f, double* data, long n);
 int gpf_fft_freq(const int* dims, int ndim, long i, double* out);
 /* FFTWWrapper.ConjugateNode (pfutil/fftWrap.go:78-95) */
 long gpf_fft_conjugate_node(const int* dims, int ndim, long i);
+
+/* ---------------- batched parameter sweeps ---------------- */
+
+/* Many independent single-field simulations of uneven 2D sizes that share one model
+ * (equation strings and scalar names) and differ in scalar values and initial fields:
+ * the sweep the reference runs as one pf.Solver per sim (pf/solver.go:40-63), with each
+ * sim stored under its own simID (pf/database.go:67-80).  shapes holds [D0, D1] per class,
+ * counts the sims per class; sims are numbered class by class.  Fused single-field
+ * Euler models only (the Cahn-Hilliard class). */
+int gpf_batch_create(gpf_model* m, int nclasses, const int* shapes, const int* counts, double dt,
+                     const gpf_options* o, gpf_batch** out);
+void gpf_batch_destroy(gpf_batch* b);
+int gpf_batch_num_sims(const gpf_batch* b);
+int gpf_batch_set_scalar(gpf_batch* b, int sim, const char* name, double value);
+int gpf_batch_set_field(gpf_batch* b, int sim, const double* data, long n);
+int gpf_batch_get_field(gpf_batch* b, int sim, double* out, long n);
+/* nsteps Euler steps of every sim; blocks of 10 steps run as one replayed HIP graph */
+int gpf_batch_propagate(gpf_batch* b, int nsteps);
+int gpf_batch_synchronize(gpf_batch* b);
+/* lines per block of the row and update pass of each class (2 values per class) */
+int gpf_batch_tuning(gpf_batch* b, int* tws, int maxn);
 
 /* ---------------- pf.Vandeven ---------------- */
 
