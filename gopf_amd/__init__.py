@@ -98,6 +98,8 @@ def _declare(L) -> None:
         "gpf_solver_pass_times": (C.c_int, [_vp, C.c_int, _dp, _lp, C.c_char_p, C.c_int]),
         "gpf_solver_stream": (_vp, [_vp]),
         "gpf_solver_tuning": (C.c_int, [_vp, _ip, C.c_int]),
+        "gpf_solver_adjoint": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_long, _dp, _dp, C.c_int,
+                                         C.POINTER(C.c_char_p), _dp]),
         "gpf_device_copy_bandwidth": (C.c_int, [C.c_long, C.c_int, C.c_int, _dp]),
         "gpf_batch_create": (C.c_int, [_vp, C.c_int, _ip, _ip, C.c_double, C.POINTER(_Options), C.POINTER(_vp)]),
         "gpf_batch_destroy": (None, [_vp]),
@@ -415,6 +417,27 @@ class Solver:
 
     def Stream(self) -> int:
         return int(lib().gpf_solver_stream(self._h) or 0)
+
+    def Adjoint(self, nsteps: int, target, scalars: Sequence[str] = (), checkpoint_every: int = 100
+                ) -> Tuple[float, np.ndarray, Dict[str, float]]:
+        """Reverse-mode gradient of J = sum_x (c_T - target)^2 over nsteps Euler steps.
+
+        Returns (J, dJ/dc_0, {scalar: dJ/dscalar}).  No reference counterpart: it
+        differentiates Euler.Step (pf/euler.go:16-47) on the fused single-field class,
+        recomputing states from checkpoints taken every checkpoint_every steps.  The
+        solver is left at c_T, as after StepperPropagate(nsteps).
+        """
+        tgt = np.ascontiguousarray(np.asarray(target).real.astype(np.float64).reshape(-1))
+        if tgt.size != self.n:
+            raise ValueError("length mismatch")
+        names = [str(x) for x in scalars]
+        cn = (C.c_char_p * max(len(names), 1))(*[x.encode() for x in names])
+        gs = np.zeros(max(len(names), 1))
+        gf = np.zeros(self.n)
+        loss = C.c_double()
+        _check(lib().gpf_solver_adjoint(self._h, int(nsteps), int(checkpoint_every), _dptr(tgt), self.n,
+                                        C.byref(loss), _dptr(gf), len(names), cn, _dptr(gs)))
+        return loss.value, gf, {k: float(gs[i]) for i, k in enumerate(names)}
 
 
 # ---------------------------------------------------------------------------

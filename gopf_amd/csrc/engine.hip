@@ -22,6 +22,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -239,7 +240,12 @@ static void kpoly_add(KPoly& p, double c, int pw) {
     ++p.n;
 }
 
-static FusedPlan compile_fused(const Model& m, int stepper) {
+// coef(t, &c) supplies each concrete term's real coefficient: the term's own value, or
+// its derivative with respect to one scalar (the adjoint's parameter tables)
+using CoefFn = std::function<bool(const TermDesc&, double*)>;
+
+static FusedPlan compile_fused(const Model& m, int stepper,
+                               const CoefFn& coef = [](const TermDesc& t, double* c) { return real_coef(t.coef, c); }) {
     FusedPlan fp;
     auto fail = [&](const std::string& w) { fp.ok = false; fp.why = w; return fp; };
     if (m.field_names.size() != 1) return fail("more than one field");
@@ -251,7 +257,7 @@ static FusedPlan compile_fused(const Model& m, int stepper) {
     for (const auto& t : terms) {
         if (t.kind != kConcrete) return fail("registered term");
         double c;
-        if (!real_coef(t.coef, &c)) return fail("complex coefficient");
+        if (!coef(t, &c)) return fail("complex coefficient");
         const int pw = t.has_lap ? t.lap : 0;
         if (pw < 0) return fail("negative LAP power");
         if (t.brick < 0) return fail("constant term");
@@ -275,7 +281,7 @@ static FusedPlan compile_fused(const Model& m, int stepper) {
     for (const auto& t : denum) {
         if (t.kind != kConcrete || t.brick >= 0) return fail("field-dependent implicit term");
         double c;
-        if (!real_coef(t.coef, &c)) return fail("complex coefficient");
+        if (!coef(t, &c)) return fail("complex coefficient");
         kpoly_add(fp.denom, c, t.has_lap ? t.lap : 0);
     }
     fp.has_src = src_lap >= 0;
@@ -330,6 +336,8 @@ public:
     virtual hipStream_t stream() const = 0;
     virtual long size() const = 0;
     virtual int tuning(int* tws, int maxn) = 0;
+    virtual void adjoint(long nsteps, long every, const double* target, double* loss, double* grad_field,
+                         int nscal, const char* const* names, double* grad_scalars) = 0;
 };
 
 template <typename T>
@@ -351,8 +359,11 @@ public:
     hipStream_t stream() const override { return st_; }
     long size() const override { return g_.n; }
     int tuning(int* tws, int maxn) override;
+    void adjoint(long nsteps, long every, const double* target, double* loss, double* grad_field, int nscal,
+                 const char* const* names, double* grad_scalars) override;
 
 private:
+    void state_to_real(const cpx<T>* X, const double* mean, T* out);
     // transforms
     const cpx<T>* tw(int n);
     void pencils(const std::vector<cpx<T>*>& dst, const std::vector<const cpx<T>*>& src, int axis, int dir);
@@ -1275,6 +1286,195 @@ void Engine<T>::on_step_finished() {
 }
 
 // ---------------------------------------------------------------------------
+// adjoint of the fused Euler step (no reference counterpart)
+// ---------------------------------------------------------------------------
+// The fused class advances  c^_{n+1} = m1 c^_n + m2 F[g(c_n)]  with real, even
+// multipliers m1 = A (1 + dt E), m2 = A dt L^p, A = sigma / (1 - dt D) (euler.go:32-44
+// with the fused plan's D, E, g).  Each multiplier is a symmetric operator on real
+// fields, so the transposed step is  lam_n = F^-1[m1 F lam_{n+1}] + g'(c_n) F^-1[m2 F lam_{n+1}].
+// The parameter gradient of step n is <lam_{n+1}, d c_{n+1} / d s>, summed over the half
+// spectrum with Hermitian weights.  States are recomputed per segment from checkpoints
+// of the fused state taken every `every` steps.
+template <typename T>
+void Engine<T>::state_to_real(const cpx<T>* X, const double* mean, T* out) {
+    RowC2RArgs<T> a;
+    a.tw = tw(g_.nx);
+    a.narr = 1;
+    a.main[0] = X;
+    a.nyq[0] = X + g_.nmain;
+    a.out[0] = out;
+    a.rows = g_.rows;
+    a.nx = g_.nx;
+    a.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
+    a.scale = T(1);
+    a.add_mean = split_mean_ ? mean : nullptr;
+    HIPCHK(launch_rows_c2r<T>(a, st_));
+}
+
+template <typename T>
+void Engine<T>::adjoint(long nsteps, long every, const double* target, double* loss, double* grad_field, int nscal,
+                        const char* const* names, double* grad_scalars) {
+    if (!fp_.ok) throw Error(GPF_EINVAL, "the adjoint covers the fused single-field Euler class (" + fp_.why + ")");
+    if (nsteps < 1 || every < 1 || !target || !loss) throw Error(GPF_EINVAL, "bad adjoint arguments");
+    if (nscal < 0 || nscal > kMaxAdjScalars) throw Error(GPF_EINVAL, "at most 8 scalars per adjoint run");
+    if (nscal > 0 && (!names || !grad_scalars)) throw Error(GPF_EINVAL, "scalar names and output are required");
+    if (autotune_ && !tuned_) autotune();
+    const bool mon = mon_;
+    mon_ = false;
+    const long n = g_.n, nk = g_.nk;
+    // parameter tables: derivative of D, E (per LAP power) and of g (per power of c)
+    std::vector<double> dcoef((size_t)std::max(nscal, 1) * 2 * kMaxKTerms, 0.0);
+    std::vector<std::vector<double>> dpoly(nscal);
+    bool need_c1 = false, need_c0 = false;
+    for (int s = 0; s < nscal; ++s) {
+        const std::string name = names[s] ? names[s] : "";
+        auto it = mcopy_.scalars.find(name);
+        if (it == mcopy_.scalars.end()) throw Error(GPF_EINVAL, "unknown scalar " + name);
+        if (it->second.imag() != 0.0) throw Error(GPF_EINVAL, "complex scalar " + name);
+        FusedPlan d = compile_fused(mcopy_, stepper_, [&](const TermDesc& t, double* c) {
+            *c = coef_derivative(t, mcopy_.scalars, name);
+            return true;
+        });
+        if (!d.ok || d.denom.n != fp_.denom.n || d.lin.n != fp_.lin.n)
+            throw Error(GPF_ESTATE, "derivative plan does not match the fused plan");
+        for (int t = 0; t < d.denom.n; ++t) {
+            dcoef[(size_t)s * 2 * kMaxKTerms + t] = d.denom.coef[t];
+            need_c1 |= d.denom.coef[t] != 0.0;
+        }
+        for (int t = 0; t < d.lin.n; ++t) {
+            dcoef[(size_t)s * 2 * kMaxKTerms + kMaxKTerms + t] = d.lin.coef[t];
+            need_c0 |= d.lin.coef[t] != 0.0;
+        }
+        if (fp_.has_src) dpoly[s] = d.gpoly;
+    }
+    // g'(c) coefficients
+    std::vector<double> dg;
+    if (fp_.has_src)
+        for (size_t q = 1; q < fp_.gpoly.size(); ++q) dg.push_back((double)q * fp_.gpoly[q]);
+    Buf<double> d_dcoef(dcoef.size()), d_dg(kMaxPoly), d_dpoly((size_t)kMaxPoly * std::max(nscal, 1));
+    {
+        std::vector<double> h(kMaxPoly, 0.0), hp((size_t)kMaxPoly * std::max(nscal, 1), 0.0);
+        for (size_t q = 0; q < dg.size(); ++q) h[q] = dg[q];
+        for (int s = 0; s < nscal; ++s)
+            for (size_t q = 0; q < dpoly[s].size() && q < (size_t)kMaxPoly; ++q) hp[(size_t)s * kMaxPoly + q] = dpoly[s][q];
+        HIPCHK(hipMemcpyAsync(d_dcoef.get(), dcoef.data(), dcoef.size() * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPCHK(hipMemcpyAsync(d_dg.get(), h.data(), kMaxPoly * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPCHK(hipMemcpyAsync(d_dpoly.get(), hp.data(), hp.size() * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+    }
+    // forward sweep with checkpoints of the fused state
+    const long nck = (nsteps + every - 1) / every;
+    std::vector<Buf<cpx<T>>> ck;
+    Buf<double> ckmean(nck);
+    for (long s = 0; s < nsteps; ++s) {
+        if (s % every == 0) {
+            ck.emplace_back(nk);
+            HIPCHK(hipMemcpyAsync(ck.back().get(), X_.get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+            HIPCHK(hipMemcpyAsync(ckmean.get() + s / every, mean_.get(), sizeof(double), hipMemcpyDeviceToDevice, st_));
+        }
+        fused_step();
+    }
+    Buf<cpx<T>> Xf(nk);
+    Buf<double> meanf(1);
+    HIPCHK(hipMemcpyAsync(Xf.get(), X_.get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+    HIPCHK(hipMemcpyAsync(meanf.get(), mean_.get(), sizeof(double), hipMemcpyDeviceToDevice, st_));
+    // loss and lam_T = 2 (c_T - target), in FP64 on the host
+    std::vector<double> cT(2 * n);
+    get_field(0, cT.data());
+    std::vector<T> lamT(n);
+    double js = 0.0, jc = 0.0;
+    for (long i = 0; i < n; ++i) {
+        const double r = cT[2 * i] - target[i];
+        const double v = r * r;
+        const double t = js + v;
+        jc += (std::fabs(js) >= std::fabs(v)) ? ((js - t) + v) : ((v - t) + js);
+        js = t;
+        lamT[i] = (T)(2.0 * r);
+    }
+    *loss = js + jc;
+    Buf<T> lam(n), pr(n), qr(n);
+    HIPCHK(hipMemcpyAsync(lam.get(), lamT.data(), n * sizeof(T), hipMemcpyHostToDevice, st_));
+    // work arrays
+    const long seglen = std::min(every, nsteps);
+    std::vector<Buf<T>> R;
+    for (long i = 0; i <= seglen; ++i) R.emplace_back(n);
+    std::vector<Buf<T>> Gr;
+    std::vector<Buf<cpx<T>>> Gk;
+    for (int s = 0; s < nscal; ++s) {
+        Gr.emplace_back(dpoly[s].empty() ? 1 : n);
+        Gk.emplace_back(dpoly[s].empty() ? 1 : nk);
+    }
+    Buf<cpx<T>> Lk(nk), C1k(need_c1 ? nk : 1), C0k(need_c0 ? nk : 1), Pk(nk), Qk(nk);
+    const long nblk = std::max(1L, std::min((nk + 255) / 256, 1024L));
+    Buf<double> partial((size_t)std::max(nscal, 1) * nblk), acc(std::max(nscal, 1));
+    HIPCHK(hipMemsetAsync(acc.get(), 0, std::max(nscal, 1) * sizeof(double), st_));
+    AdjArgs<T> a;
+    a.g = g_.kg;
+    a.nmain = g_.nmain;
+    a.nk = nk;
+    a.dt = dt_;
+    a.denom = fp_.denom;
+    a.lin = fp_.lin;
+    a.src_lap = fp_.src_lap;
+    a.filter = has_filter_ ? filter_.get() : nullptr;
+    a.lam = Lk.get();
+    a.P = Pk.get();
+    a.Q = Qk.get();
+    a.nscal = nscal;
+    a.dcoef = d_dcoef.get();
+    a.C1 = need_c1 ? C1k.get() : nullptr;
+    a.C0 = need_c0 ? C0k.get() : nullptr;
+    for (int s = 0; s < nscal; ++s) a.Gs[s] = dpoly[s].empty() ? nullptr : Gk[s].get();
+    a.partial = partial.get();
+    a.inv_n = 1.0 / (double)n;
+    const int ndg = (int)dg.size();
+    for (long seg = nck - 1; seg >= 0; --seg) {
+        const long s0 = seg * every, s1 = std::min(nsteps, s0 + every);
+        HIPCHK(hipMemcpyAsync(X_.get(), ck[seg].get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+        HIPCHK(hipMemcpyAsync(mean_.get(), ckmean.get() + seg, sizeof(double), hipMemcpyDeviceToDevice, st_));
+        for (long s = s0; s < s1; ++s) {
+            state_to_real(X_.get(), mean_.get(), R[s - s0].get());
+            fused_step();
+        }
+        state_to_real(X_.get(), mean_.get(), R[s1 - s0].get());
+        for (long s = s1 - 1; s >= s0; --s) {
+            const T* cn = R[s - s0].get();
+            std::vector<const T*> rin = {lam.get()};
+            std::vector<cpx<T>*> kout = {Lk.get()};
+            if (need_c1) { rin.push_back(R[s + 1 - s0].get()); kout.push_back(C1k.get()); }
+            if (need_c0) { rin.push_back(cn); kout.push_back(C0k.get()); }
+            for (int q = 0; q < nscal; ++q) {
+                if (dpoly[q].empty()) continue;
+                HIPCHK(launch_adj_lambda<T>(nullptr, nullptr, cn, (int)std::min(dpoly[q].size(), (size_t)kMaxPoly),
+                                            d_dpoly.get() + (size_t)q * kMaxPoly, Gr[q].get(), n, st_));
+                rin.push_back(Gr[q].get());
+                kout.push_back(Gk[q].get());
+            }
+            r2c(rin, kout, T(1));
+            if (nscal > 0) {
+                HIPCHK(launch_adj_grad<T>(a, nblk, st_));
+                HIPCHK(launch_accumulate(partial.get(), nscal, nblk, acc.get(), st_));
+            }
+            HIPCHK(launch_adj_mult<T>(a, st_));
+            c2r({Pk.get(), Qk.get()}, {pr.get(), qr.get()}, T(1) / (T)n);
+            HIPCHK(launch_adj_lambda<T>(pr.get(), qr.get(), cn, ndg, d_dg.get(), lam.get(), n, st_));
+        }
+    }
+    // outputs; the solver is left at c_T
+    std::vector<T> hl(n);
+    HIPCHK(hipMemcpyAsync(hl.data(), lam.get(), n * sizeof(T), hipMemcpyDeviceToHost, st_));
+    std::vector<double> hs(std::max(nscal, 1));
+    HIPCHK(hipMemcpyAsync(hs.data(), acc.get(), hs.size() * sizeof(double), hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipMemcpyAsync(X_.get(), Xf.get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+    HIPCHK(hipMemcpyAsync(mean_.get(), meanf.get(), sizeof(double), hipMemcpyDeviceToDevice, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+    if (grad_field)
+        for (long i = 0; i < n; ++i) grad_field[i] = (double)hl[i];
+    for (int s = 0; s < nscal; ++s) grad_scalars[s] = hs[s];
+    mon_ = mon;
+}
+
+// ---------------------------------------------------------------------------
 // full complex FourierTransform (pfutil.FFTWWrapper)
 // ---------------------------------------------------------------------------
 class FFTBase {
@@ -1603,6 +1803,19 @@ int gpf_solver_stepper_propagate(gpf_solver* s, int nsteps) {
             s->eng->step();
             s->current_step++;  // Euler.Step / RK4.Propagate increment
         }
+    });
+}
+
+int gpf_solver_adjoint(gpf_solver* s, int nsteps, int checkpoint_every, const double* target, long n,
+                       double* loss, double* grad_field, int nscalars, const char* const* scalar_names,
+                       double* grad_scalars) {
+    return guarded([&] {
+        if (!s || !target || !loss) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        if (s->stepper != GPF_EULER) throw Error(GPF_EINVAL, "the adjoint covers the Euler stepper");
+        if (n != s->eng->size()) throw Error(GPF_EINVAL, "buffer length does not match the grid");
+        s->eng->adjoint(nsteps, checkpoint_every, target, loss, grad_field, nscalars, scalar_names, grad_scalars);
+        s->current_step += nsteps;
     });
 }
 

@@ -253,6 +253,21 @@ double get_power(const std::string& pattern) {
     return p;
 }
 
+double coef_derivative(const TermDesc& t, const std::map<std::string, std::complex<double>>& scalars,
+                       const std::string& name) {
+    double d = 0.0;
+    for (size_t j = 0; j < t.factors.size(); ++j) {
+        if (t.factors[j].first != name) continue;
+        const double sv = scalars.at(name).real();
+        const double p = t.factors[j].second;
+        double v = t.sign * p * std::pow(sv, p - 1.0);
+        for (size_t i = 0; i < t.factors.size(); ++i)
+            if (i != j) v *= std::pow(scalars.at(t.factors[i].first).real(), t.factors[i].second);
+        d += v;
+    }
+    return d;
+}
+
 // ---------------------------------------------------------------------------
 // Model
 // ---------------------------------------------------------------------------
@@ -394,6 +409,8 @@ TermDesc Model::concrete_term(const SubStringDelimiter& td) {
     std::complex<double> coef(sign, 0.0);
     for (size_t j = 0; j < bricks.size(); ++j) coef = go_cmul(coef, go_cpow(scalars.at(bricks[j]), powers[j]));
     t.coef = coef;
+    t.sign = sign;
+    for (size_t j = 0; j < bricks.size(); ++j) t.factors.push_back({bricks[j], powers[j]});
     if (!fname.empty()) t.brick = brick_index(fname);
     if (term.find("LAP") != std::string::npos) {
         const std::string res = go_find_string("LAP*[^a-zA-Z]*", term);

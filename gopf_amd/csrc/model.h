@@ -29,7 +29,14 @@ struct TermDesc {
     bool has_lap = false;
     int lap = 0;                          // concrete: LaplacianN power; user: total LAP prefix power
     int user = -1;                        // index into the user-term table of its kind
+    double sign = 1.0;                    // concrete: the sign of the term
+    std::vector<std::pair<std::string, double>> factors;  // concrete: (scalar, power) factors of coef
 };
+
+// d coef / d scalar of a concrete term (real scalars): sign * sum over the factors
+// of the scalar of p s^(p-1) times the other factors
+double coef_derivative(const TermDesc& t, const std::map<std::string, std::complex<double>>& scalars,
+                       const std::string& name);
 
 struct DerivedDesc {
     std::string name;

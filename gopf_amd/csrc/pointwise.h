@@ -65,6 +65,40 @@ struct KElasticArgs {
     long nmain = 0, nk = 0;
 };
 
+// adjoint of the fused semi-implicit step (single field)
+constexpr int kMaxAdjScalars = 8;
+template <typename T>
+struct AdjArgs {
+    KGrid g;
+    long nmain = 0, nk = 0;
+    double dt = 0.0;
+    KPoly denom, lin;          // D(k), E(k)
+    int src_lap = 0;
+    const double* filter = nullptr;
+    // k_adj_mult: P = m1 Lam, Q = m2 Lam with m1 = A (1 + dt E), m2 = A dt L^src_lap, A = sigma / (1 - dt D)
+    const cpx<T>* lam = nullptr;
+    cpx<T>* P = nullptr;
+    cpx<T>* Q = nullptr;
+    // k_adj_grad: per scalar s, sum_k w Re(conj(Lam) [Ainv dt dD_s C1 + A dt dE_s C0 + m2 G_s]) / N
+    // with dD_s = sum_t dcoef[s][0][t] L^denom.pow[t] and dE_s = sum_t dcoef[s][1][t] L^lin.pow[t]
+    int nscal = 0;
+    const double* dcoef = nullptr;  // device [nscal][2][kMaxKTerms]
+    const cpx<T>* C1 = nullptr;
+    const cpx<T>* C0 = nullptr;
+    const cpx<T>* Gs[kMaxAdjScalars] = {};
+    double* partial = nullptr;  // [nscal][nblocks]
+    double inv_n = 1.0;
+};
+
+template <typename T> hipError_t launch_adj_mult(const AdjArgs<T>& a, hipStream_t s);
+template <typename T> hipError_t launch_adj_grad(const AdjArgs<T>& a, long nblocks, hipStream_t s);
+// out = p + h(c) q with h(c) = sum_k dpoly[k] c^k; a null p reads as 0 and a null q as 1
+// (the adjoint's lam_n = p + g'(c_n) q, and plain polynomial evaluation)
+template <typename T> hipError_t launch_adj_lambda(const T* p, const T* q, const T* c, int npoly, const double* dpoly,
+                                                   T* out, long n, hipStream_t s);
+// acc[s] += sum_b partial[s][b]  (single block, deterministic order)
+hipError_t launch_accumulate(const double* partial, int nscal, long nblocks, double* acc, hipStream_t s);
+
 template <typename T> hipError_t launch_derived(const DerivedArgs<T>& a, hipStream_t s);
 template <typename T> hipError_t launch_kupdate(const KUpdateArgs<T>& a, hipStream_t s);
 template <typename T> hipError_t launch_kgrad(const KGradArgs<T>& a, int ndim, hipStream_t s);

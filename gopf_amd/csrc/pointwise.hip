@@ -344,6 +344,94 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce2(const double* a, long n
     if (threadIdx.x == 0) out[0] = ca * x + cb * y;
 }
 
+__device__ __forceinline__ void adj_coeffs(const KGrid& g, long i, long nmain, double dt, const KPoly& den,
+                                           const KPoly& lin, int src_lap, const double* filter, double& L, double& A,
+                                           double& Ainv, double& m1, double& m2, double& w) {
+    int kx, ky, kz;
+    kcoords(g, i, nmain, kx, ky, kz);
+    double f[3];
+    ref_freq(g, kx, ky, kz, f);
+    const double f2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+    L = -(4.0 * kPi * kPi) * f2;
+    double D = 0.0, E = 0.0;
+    for (int t = 0; t < den.n; ++t) D += den.coef[t] * ipow(L, den.pow[t]);
+    for (int t = 0; t < lin.n; ++t) E += lin.coef[t] * ipow(L, lin.pow[t]);
+    Ainv = 1.0 / (1.0 - dt * D);
+    A = Ainv;
+    if (filter) A *= vandeven(filter, sqrt(f2) * 2.0 / kPi);
+    m1 = A * (1.0 + dt * E);
+    m2 = A * dt * ipow(L, src_lap);
+    w = (kx == 0 || 2 * kx == g.nx) ? 1.0 : 2.0;
+}
+
+template <typename T>
+__global__ void k_adj_mult(AdjArgs<T> a) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
+        double L, A, Ainv, m1, m2, w;
+        adj_coeffs(a.g, i, a.nmain, a.dt, a.denom, a.lin, a.src_lap, a.filter, L, A, Ainv, m1, m2, w);
+        const cpx<T> l = a.lam[i];
+        a.P[i] = mk<T>((T)(m1 * (double)l.x), (T)(m1 * (double)l.y));
+        a.Q[i] = mk<T>((T)(m2 * (double)l.x), (T)(m2 * (double)l.y));
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_grad(AdjArgs<T> a) {
+    __shared__ double red[256];
+    double acc[kMaxAdjScalars];
+    for (int s = 0; s < kMaxAdjScalars; ++s) acc[s] = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
+        double L, A, Ainv, m1, m2, w;
+        adj_coeffs(a.g, i, a.nmain, a.dt, a.denom, a.lin, a.src_lap, a.filter, L, A, Ainv, m1, m2, w);
+        const cpx<T> l = a.lam[i];
+        const cpx<T> c1 = a.C1 ? a.C1[i] : mk<T>(T(0), T(0));
+        const cpx<T> c0 = a.C0 ? a.C0[i] : mk<T>(T(0), T(0));
+        // Re(conj(l) z) = l.x z.x + l.y z.y
+        const double lc1 = (double)l.x * c1.x + (double)l.y * c1.y;
+        const double lc0 = (double)l.x * c0.x + (double)l.y * c0.y;
+#pragma unroll
+        for (int s = 0; s < kMaxAdjScalars; ++s) {
+            if (s >= a.nscal) break;
+            const double* dc = a.dcoef + (long)s * 2 * kMaxKTerms;
+            double dD = 0.0, dE = 0.0;
+            for (int t = 0; t < a.denom.n; ++t) dD += dc[t] * ipow(L, a.denom.pow[t]);
+            for (int t = 0; t < a.lin.n; ++t) dE += dc[kMaxKTerms + t] * ipow(L, a.lin.pow[t]);
+            double v = Ainv * a.dt * dD * lc1 + A * a.dt * dE * lc0;
+            if (a.Gs[s]) {
+                const cpx<T> gsv = a.Gs[s][i];
+                v += m2 * ((double)l.x * gsv.x + (double)l.y * gsv.y);
+            }
+            acc[s] += w * v;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < kMaxAdjScalars; ++s) {
+        if (s >= a.nscal) break;  // uniform over the block
+        const double tot = block_sum(acc[s], red);
+        if (threadIdx.x == 0) a.partial[(long)s * gridDim.x + blockIdx.x] = tot * a.inv_n;
+    }
+}
+
+template <typename T>
+__global__ void k_adj_lambda(const T* p, const T* q, const T* c, int npoly, const double* dpoly, T* out, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double x = (double)c[i];
+        double d = 0.0;
+        for (int k = npoly - 1; k >= 0; --k) d = d * x + dpoly[k];
+        out[i] = (T)((p ? (double)p[i] : 0.0) + d * (q ? (double)q[i] : 1.0));
+    }
+}
+
+__global__ void __launch_bounds__(256) k_accumulate(const double* partial, int nscal, long nblocks, double* acc) {
+    __shared__ double red[256];
+    for (int s = 0; s < nscal; ++s) {
+        KSum k;
+        for (long b = threadIdx.x; b < nblocks; b += blockDim.x) k.add(partial[(long)s * nblocks + b]);
+        const double tot = block_sum(k.get(), red);
+        if (threadIdx.x == 0) acc[s] += tot;
+    }
+}
+
 inline unsigned grid_for(long n, int thr = 256) {
     long g = (n + thr - 1) / thr;
     if (g > 65536) g = 65536;
@@ -416,6 +504,27 @@ hipError_t launch_copy_bw(const float* src, float* dst, long n, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_adj_mult(const AdjArgs<T>& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_mult<T>, dim3(grid_for(a.nk)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_adj_grad(const AdjArgs<T>& a, long nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_grad<T>, dim3((unsigned)nblocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_adj_lambda(const T* p, const T* q, const T* c, int npoly, const double* dpoly, T* out, long n,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_lambda<T>, dim3(grid_for(n)), dim3(256), 0, s, p, q, c, npoly, dpoly, out, n);
+    return hipGetLastError();
+}
+hipError_t launch_accumulate(const double* partial, int nscal, long nblocks, double* acc, hipStream_t s) {
+    hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(256), 0, s, partial, nscal, nblocks, acc);
+    return hipGetLastError();
+}
+
 hipError_t launch_reduce2(const double* a, long na, double ca, const double* b, long nb, double cb, double* out,
                           hipStream_t s) {
     hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kRedThreads), 0, s, a, na, ca, b, nb, cb, out);
@@ -433,7 +542,11 @@ hipError_t launch_reduce2(const double* a, long na, double ca, const double* b, 
     template hipError_t launch_copy<T>(const T*, T*, long, hipStream_t);                                      \
     template hipError_t launch_real_sums<T>(const T*, long, double, int, const double*, double*, double*,     \
                                             long, hipStream_t);                                               \
-    template hipError_t launch_kgrad_energy<T>(const cpx<T>*, KGrid, long, long, double*, long, hipStream_t);
+    template hipError_t launch_kgrad_energy<T>(const cpx<T>*, KGrid, long, long, double*, long, hipStream_t); \
+    template hipError_t launch_adj_mult<T>(const AdjArgs<T>&, hipStream_t);                                  \
+    template hipError_t launch_adj_grad<T>(const AdjArgs<T>&, long, hipStream_t);                            \
+    template hipError_t launch_adj_lambda<T>(const T*, const T*, const T*, int, const double*, T*, long,     \
+                                             hipStream_t);
 GPF_INST(float)
 GPF_INST(double)
 #undef GPF_INST

@@ -106,6 +106,7 @@ struct RowC2RArgs {
     int nx = 0;
     int TW = 16;
     T scale = T(1);
+    const double* add_mean = nullptr;    // optional FP64 value added to every output (first array)
 };
 
 // fused x-row pass of the single-field fast path:

@@ -449,6 +449,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_c2r(RowC2RArgs<T> a) {
     const LineMap lm = line_map(true, a.TW, TT, n);
     const long row = (long)blockIdx.x * a.TW + lm.l;
     const bool valid = row < a.rows;
+    const double add0 = a.add_mean ? *a.add_mean : 0.0;
 #pragma unroll
     for (int q = 0; q < kMaxArr; q += 2) {
         if (q >= a.narr) break;
@@ -468,7 +469,8 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_c2r(RowC2RArgs<T> a) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const long off = row * n + lm.t + TT * e;
-                a.out[q][off] = v[e].x * a.scale;
+                const T r = v[e].x * a.scale;
+                a.out[q][off] = (q == 0 && a.add_mean) ? (T)((double)r + add0) : r;
                 if (two) a.out[q + 1][off] = v[e].y * a.scale;
             }
         }

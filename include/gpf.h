@@ -148,6 +148,17 @@ int gpf_solver_tuning(gpf_solver* s, int* tws, int maxn);
 /* the HIP stream the solver launches on (hipStream_t) */
 void* gpf_solver_stream(gpf_solver* s);
 
+/* Reverse-mode gradient (no reference counterpart; it replaces nothing in gopf and
+ * differentiates the Euler.Step of pf/euler.go:16-47 on the fused single-field class).
+ * From the solver's current field c_0 it runs nsteps Euler steps, checkpointing the
+ * state every checkpoint_every steps, and returns J = sum_x (c_T(x) - target(x))^2,
+ * dJ/dc_0 (grad_field, n doubles, may be NULL) and dJ/ds for each named scalar s
+ * (at most 8 names).  Earlier states are recomputed from the checkpoints.  On return
+ * the solver holds c_T and its time has advanced by nsteps * dt. */
+int gpf_solver_adjoint(gpf_solver* s, int nsteps, int checkpoint_every, const double* target, long n,
+                       double* loss, double* grad_field, int nscalars, const char* const* scalar_names,
+                       double* grad_scalars);
+
 /* Device copy bandwidth probe: average milliseconds of one read+write copy of `bytes`
  * bytes with the library's own copy kernel (no reference counterpart: the measured
  * copy bandwidth the roofline is also reported against, and the known-size kernel

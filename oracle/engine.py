@@ -87,6 +87,8 @@ def _declare(L) -> None:
     L.or_eval_user_term.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
     L.or_ch_energy.argtypes = [C.c_int, _ip, _dp, C.c_int, _dp, C.c_double]
     L.or_ch_energy.restype = C.c_double
+    L.or_euler_adjoint.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.or_euler_adjoint.restype = C.c_int
 
 
 def _i(a) -> C.Array:
@@ -303,3 +305,27 @@ class OracleSolver:
 
     def get_time(self) -> float:
         return float(lib().or_get_time(self.h))
+
+    # -- reverse mode (no reference counterpart) ---------------------------------
+    def adjoint(self, nsteps: int, target: np.ndarray, scalars: Sequence[str] = ()):
+        """Gradient of J = sum (c_T - target)^2 after nsteps Euler steps from the current field.
+
+        Returns (J, dJ/dc_0, {scalar: dJ/dscalar}); the scalar gradients chain the C
+        oracle's per-term coefficient gradients through each term's (scalar, power)
+        factors.  Leaves c_T in the field.
+        """
+        tgt = np.ascontiguousarray(np.asarray(target).real.astype(np.float64).reshape(self.n))
+        terms, denum = self.model.rhs[0]
+        gt = np.zeros(max(len(terms), 1))
+        gd = np.zeros(max(len(denum), 1))
+        g0 = np.zeros(self.n)
+        loss = C.c_double()
+        rc = lib().or_euler_adjoint(self.h, int(nsteps), tgt.ctypes.data_as(_dp), C.byref(loss),
+                                    g0.ctypes.data_as(_dp), gt.ctypes.data_as(_dp), gd.ctypes.data_as(_dp))
+        if rc != 0:
+            raise ValueError("the oracle adjoint covers one field, Euler, concrete terms")
+        out = {}
+        for name in scalars:
+            out[name] = (sum(gt[i] * t.coef_derivative(self.model.scalars, name) for i, t in enumerate(terms))
+                         + sum(gd[i] * t.coef_derivative(self.model.scalars, name) for i, t in enumerate(denum)))
+        return loss.value, g0, out

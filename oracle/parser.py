@@ -260,6 +260,22 @@ class TermDesc:
     has_lap: bool = False
     lap: int = 0                    # concrete: LaplacianN power; user: total LAP prefix power
     user: int = -1                  # user term table index
+    sign: float = 1.0               # concrete: sign of the term
+    factors: List[Tuple[str, float]] = dc_field(default_factory=list)  # concrete: (scalar, power)
+
+    def coef_derivative(self, scalars: Dict[str, complex], name: str) -> float:
+        """d coef / d scalar for real scalars: sign * p s^(p-1) * the other factors (no reference
+        counterpart; the parameter gradient of the adjoint)."""
+        d = 0.0
+        for j, (nm, p) in enumerate(self.factors):
+            if nm != name:
+                continue
+            v = self.sign * p * scalars[name].real ** (p - 1.0)
+            for i, (other, q) in enumerate(self.factors):
+                if i != j:
+                    v *= scalars[other].real ** q
+            d += v
+        return d
 
 
 @dataclass
@@ -398,7 +414,7 @@ class Model:
         coef = complex(sign, 0.0)
         for bn, p in zip(brick_names, powers):
             coef = go_cmul(coef, go_cpow(self.scalars[bn], p))
-        desc = TermDesc(KIND_CONCRETE, coef=coef)
+        desc = TermDesc(KIND_CONCRETE, coef=coef, sign=sign, factors=list(zip(brick_names, powers)))
         if fname != "":
             desc.brick = self.brick_index(fname)
         if "LAP" in term:

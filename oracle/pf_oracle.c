@@ -1040,3 +1040,158 @@ double or_ch_energy(int ndim, const int *dims, const double *c, int npoly, const
     free(w);
     return bulk + 0.5 * kappa * g / (double)n;
 }
+
+/* ------------------------------------------------------------------ */
+/* Reverse mode of Euler.Step (no reference counterpart)               */
+/* ------------------------------------------------------------------ */
+/*
+ * d ln b / d c of a derived brick, times b: the derivative of a derived field of
+ * field 0 at the value x (product of powers, util.go:38-65, or a real polynomial).
+ */
+static double ipow_real(double x, double p)
+{
+    if (p == (double)(int)p && fabs(p) <= 16) {
+        int ip = (int)fabs(p);
+        double v = 1.0;
+        for (int q = 0; q < ip; ++q) v *= x;
+        return (p < 0) ? 1.0 / v : v;
+    }
+    return pow(x, p);
+}
+
+static double derived_deriv(const or_model *m, int d, double x)
+{
+    double acc = 0.0;
+    int t0 = m->dterm_start[d], t1 = m->dterm_start[d + 1];
+    if (m->dkind[d] == 0) t1 = t0 + 1; /* one product term */
+    for (int t = t0; t < t1; ++t) {
+        double coef = (m->dkind[d] == 0) ? 1.0 : m->dterm_coef[t];
+        int f0 = m->dfac_start[t], f1 = m->dfac_start[t + 1];
+        for (int j = f0; j < f1; ++j) {
+            double v = coef * m->dfac_pow[j] * ipow_real(x, m->dfac_pow[j] - 1.0);
+            for (int i = f0; i < f1; ++i)
+                if (i != j) v *= ipow_real(x, m->dfac_pow[i]);
+            acc += v;
+        }
+    }
+    return acc;
+}
+
+/*
+ * Gradient of J = sum_x (c_T(x) - target(x))^2 after nsteps Euler steps
+ * (euler.go:16-47) from the field held in brick 0, with respect to that initial
+ * field and to the coefficient of every explicit (RHS.Terms) and implicit
+ * (RHS.Denum) term of equation 0.  The transposed step is assembled term by term:
+ * every map x -> IFFT(h FFT x)/N with h real and even is symmetric, so with
+ * s = sigma / (1 - dt D) and Lam = FFT(lam_{n+1})
+ *   lam_n   = Re IFFT(s Lam)/N + sum_t dt coef_t Re IFFT(s L^p_t Lam)/N * d b_t / d c
+ *   dJ/dcoef_t (explicit) += Re sum_k conj(Lam) s dt L^p_t FFT(b_t) / N
+ *   dJ/dcoef_t (implicit) += Re sum_k conj(Lam) dt L^p_t FFT(c_{n+1}) / (1 - dt D) / N
+ * Supported: one field, Euler, concrete terms with real coefficients whose bricks
+ * are the field or derived fields of it, implicit terms without a brick.  Every
+ * state is kept (small grids only).  Leaves c_T in brick 0.  Returns 0, or -1 for
+ * an unsupported model.
+ */
+int or_euler_adjoint(void *h, int nsteps, const double *target, double *loss, double *grad_c0,
+                     double *grad_terms, double *grad_denum)
+{
+    or_model *m = (or_model *)h;
+    long n = m->n;
+    double dt = m->dt, f[3];
+    if (m->stepper != 0 || m->nfields != 1 || m->neq < 1 || nsteps < 1) return -1;
+    int t0 = m->eq_t_start[0], t1 = m->eq_t_start[1];
+    int d0 = m->eq_d_start[0], d1 = m->eq_d_start[1];
+    for (int t = t0; t < t1; ++t)
+        if (m->terms[t].kind != TERM_CONCRETE || m->terms[t].coef_im != 0.0) return -1;
+    for (int t = d0; t < d1; ++t)
+        if (m->denum[t].kind != TERM_CONCRETE || m->denum[t].brick >= 0 || m->denum[t].coef_im != 0.0) return -1;
+    for (int d = 0; d < m->nderived; ++d)
+        for (int j = m->dfac_start[m->dterm_start[d]]; j < m->dfac_start[m->dterm_start[d + 1]]; ++j)
+            if (m->dfac_field[j] != 0) return -1;
+    int nb = m->nfields + m->nderived;
+    double *states = (double *)malloc(sizeof(double) * n * (size_t)(nsteps + 1));
+    for (int s = 0; s < nsteps; ++s) {
+        for (long i = 0; i < n; ++i) states[(size_t)s * n + i] = creal(m->bricks[0][i]);
+        euler_step(m);
+    }
+    for (long i = 0; i < n; ++i) states[(size_t)nsteps * n + i] = creal(m->bricks[0][i]);
+    /* loss and lam_T */
+    double *lam = (double *)malloc(sizeof(double) * n);
+    double js = 0.0, jc = 0.0;
+    for (long i = 0; i < n; ++i) {
+        double r = states[(size_t)nsteps * n + i] - target[i];
+        double v = r * r, t = js + v;
+        if (fabs(js) >= fabs(v)) jc += (js - t) + v;
+        else jc += (v - t) + js;
+        js = t;
+        lam[i] = 2.0 * r;
+    }
+    *loss = js + jc;
+    for (int t = t0; t < t1; ++t) grad_terms[t - t0] = 0.0;
+    for (int t = d0; t < d1; ++t) grad_denum[t - d0] = 0.0;
+    zc *L = (zc *)malloc(sizeof(zc) * n), *C1 = (zc *)malloc(sizeof(zc) * n);
+    zc *den = (zc *)malloc(sizeof(zc) * n), *val = (zc *)malloc(sizeof(zc) * n);
+    zc **mu = (zc **)malloc(sizeof(zc *) * nb);
+    double *sk = (double *)malloc(sizeof(double) * n), *lnew = (double *)malloc(sizeof(double) * n);
+    for (int b = 0; b < nb; ++b) mu[b] = (zc *)malloc(sizeof(zc) * n);
+    for (int s = nsteps - 1; s >= 0; --s) {
+        const double *cs = states + (size_t)s * n, *cs1 = states + (size_t)(s + 1) * n;
+        for (long i = 0; i < n; ++i) m->bricks[0][i] = cs[i];
+        sync_derived(m);
+        for (int b = 0; b < nb; ++b) fftnd_exec(m->ft, m->bricks[b], -1);
+        get_denum(m, 0, den);
+        for (long i = 0; i < n; ++i) {
+            double sv = 1.0 / (1.0 - dt * creal(den[i]));
+            if (m->filter_order > 0) {
+                ref_freq(m->ndim, m->dims, i, f);
+                double frad = sqrt(f[0] * f[0] + f[1] * f[1] + (m->ndim > 2 ? f[2] * f[2] : 0.0));
+                sv *= or_vandeven_eval(m->vand, frad * 2.0 / OR_PI);
+            }
+            sk[i] = sv;
+            L[i] = lam[i];
+            C1[i] = cs1[i];
+        }
+        fftnd_exec(m->ft, L, -1);
+        fftnd_exec(m->ft, C1, -1);
+        for (int b = 0; b < nb; ++b)
+            for (long i = 0; i < n; ++i) mu[b][i] = 0.0;
+        for (long i = 0; i < n; ++i) mu[0][i] = sk[i] * L[i];
+        for (int t = t0; t < t1; ++t) {
+            or_term u = m->terms[t];
+            u.coef_re = 1.0;
+            u.coef_im = 0.0;
+            eval_term(m, &u, val); /* L^p FFT(b_t), or L^p alone for a constant term */
+            double g = 0.0;
+            for (long i = 0; i < n; ++i) g += creal(conj(L[i]) * sk[i] * dt * val[i]);
+            grad_terms[t - t0] += g / (double)n;
+            if (m->terms[t].brick < 0) continue;
+            u.brick = -1;
+            eval_term(m, &u, val); /* L^p */
+            zc *mb = mu[m->terms[t].brick];
+            for (long i = 0; i < n; ++i) mb[i] += dt * m->terms[t].coef_re * sk[i] * creal(val[i]) * L[i];
+        }
+        for (int t = d0; t < d1; ++t) {
+            or_term u = m->denum[t];
+            u.coef_re = 1.0;
+            u.coef_im = 0.0;
+            eval_term(m, &u, val); /* L^p */
+            double g = 0.0;
+            for (long i = 0; i < n; ++i)
+                g += creal(conj(L[i]) * dt * creal(val[i]) * C1[i]) / (1.0 - dt * creal(den[i]));
+            grad_denum[t - d0] += g / (double)n;
+        }
+        fftnd_exec(m->ft, mu[0], +1);
+        for (long i = 0; i < n; ++i) lnew[i] = creal(mu[0][i]) / (double)n;
+        for (int d = 0; d < m->nderived; ++d) {
+            zc *mb = mu[m->nfields + d];
+            fftnd_exec(m->ft, mb, +1);
+            for (long i = 0; i < n; ++i) lnew[i] += creal(mb[i]) / (double)n * derived_deriv(m, d, cs[i]);
+        }
+        memcpy(lam, lnew, sizeof(double) * n);
+    }
+    memcpy(grad_c0, lam, sizeof(double) * n);
+    for (long i = 0; i < n; ++i) m->bricks[0][i] = states[(size_t)nsteps * n + i];
+    for (int b = 0; b < nb; ++b) free(mu[b]);
+    free(mu); free(sk); free(lnew); free(L); free(C1); free(den); free(val); free(lam); free(states);
+    return 0;
+}
