@@ -134,10 +134,19 @@ struct TileInfo {
 
 // The segment is chosen with selects (no dynamic indexing of the argument
 // struct, which would copy it to scratch memory).
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8), and
+// each XCD has its own L2.  Neighbouring tiles of a plane share cache lines when a tile
+// row is narrower than a line, so consecutive tiles go to workgroups of one XCD: XCD x
+// takes the x-th contiguous run of tiles (a bijection of [0, nblk)).
+__device__ __forceinline__ int xcd_tile(int b, int nblk) {
+    const int q = nblk >> 3, r = nblk & 7, x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
 template <typename T>
 __device__ __forceinline__ TileInfo decode_tile(const PencilArgs<T>& a) {
     TileInfo ti;
-    int tile = blockIdx.x;
+    int tile = xcd_tile(blockIdx.x, gridDim.x);
     const bool s1 = tile >= a.tiles0;
     ti.s = s1 ? 1 : 0;
     if (s1) tile -= a.tiles0;
