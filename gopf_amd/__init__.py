@@ -410,10 +410,10 @@ class Solver:
         return out
 
     def Tuning(self) -> Dict[str, int]:
-        v = (C.c_int * 5)()
-        lib().gpf_solver_tuning(self._h, v, 5)
+        v = (C.c_int * 6)()
+        lib().gpf_solver_tuning(self._h, v, 6)
         return {"rows_fused": v[0], "pencils_fwd": v[1], "pencil_update": v[2], "pencils_inv": v[3],
-                "update_axis": "zy"[2 - v[4]] if v[4] in (1, 2) else "y"}
+                "update_axis": "zy"[2 - v[4]] if v[4] in (1, 2) else "y", "rows_schedule": v[5]}
 
     def Stream(self) -> int:
         return int(lib().gpf_solver_stream(self._h) or 0)

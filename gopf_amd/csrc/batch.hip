@@ -7,7 +7,9 @@
 // pass and one of the fused y-pencil update pass advance every sim of a class.
 // Per-sim scalars become per-sim coefficient tables (the explicit source
 // polynomial, the implicit denominator and the linear explicit terms) and a
-// per-sim FP64 mean.  Steps are captured in a HIP graph and replayed.
+// per-sim FP64 mean.  Each class keeps its state as the spectra of its sims and one
+// work array (see the single solver's fused step).  Steps are captured in a HIP graph
+// and replayed.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -78,9 +80,9 @@ template <typename T>
 struct BatchClass {
     int ny = 0, nx = 0, nsim = 0, first = 0;
     long nmain = 0, nk = 0, rows = 0;
-    DBuf<cpx<T>> X, Y, G, xtw, ytw;
+    DBuf<cpx<T>> S, W, xtw, ytw;  // state spectra; work array (S inverse-transformed along y)
     DBuf<double> mean, gpoly, denom, lin;
-    int tw_rows = 0, tw_upd = 0;
+    int tw_rows = 0, tw_upd = 0, rows_var = 0;
 };
 
 template <typename T>
@@ -98,16 +100,16 @@ public:
             cl->nx = shapes[2 * c + 1];
             cl->nsim = counts[c];
             cl->first = first;
-            if (cl->nx % 2 || !fft_size_supported(cl->nx) || !fft_size_supported(cl->ny) || cl->nsim < 1)
-                throw BatchError(GPF_EINVAL, "unsupported batch class shape");
+            if (cl->nx % 2 || cl->ny % 2 || !fft_size_supported(cl->nx) || !fft_size_supported(cl->ny) || cl->nsim < 1)
+                throw BatchError(GPF_EINVAL, "unsupported batch class shape (both sides even, supported lengths)");
             cl->rows = (long)cl->nsim * cl->ny;
             cl->nmain = cl->rows * (cl->nx / 2);
             cl->nk = cl->nmain + cl->rows;
-            cl->X.alloc(cl->nk);
-            cl->Y.alloc(cl->nk);
-            cl->G.alloc(cl->nk);
+            cl->S.alloc(cl->nk);
+            cl->W.alloc(cl->nk);
             cl->mean.alloc(cl->nsim);
-            BCHK(hipMemsetAsync(cl->X.p, 0, cl->nk * sizeof(cpx<T>), st_));
+            BCHK(hipMemsetAsync(cl->S.p, 0, cl->nk * sizeof(cpx<T>), st_));
+            BCHK(hipMemsetAsync(cl->W.p, 0, cl->nk * sizeof(cpx<T>), st_));
             BCHK(hipMemsetAsync(cl->mean.p, 0, cl->nsim * sizeof(double), st_));
             auto wx = twiddles<T>(cl->nx), wy = twiddles<T>(cl->ny);
             cl->xtw.alloc(cl->nx);
@@ -156,17 +158,21 @@ public:
         for (long j = 0; j < n; ++j) h[j] = (T)(in[2 * j] - mean);
         DBuf<T> tmp(n);
         BCHK(hipMemcpyAsync(tmp.p, h.data(), n * sizeof(T), hipMemcpyHostToDevice, st_));
+        cpx<T>* sm = c.S.p + (long)ls * c.ny * (c.nx / 2);
+        cpx<T>* sq = c.S.p + c.nmain + (long)ls * c.ny;
         RowR2CArgs<T> a;
         a.tw = c.xtw.p;
         a.narr = 1;
         a.in[0] = tmp.p;
-        a.main[0] = c.X.p + (long)ls * c.ny * (c.nx / 2);
-        a.nyq[0] = c.X.p + c.nmain + (long)ls * c.ny;
+        a.main[0] = sm;
+        a.nyq[0] = sq;
         a.rows = c.ny;
         a.nx = c.nx;
         a.TW = pick_tw(c.nx, 0);
-        a.in_scale = T(1) / (T)c.nx;
+        a.in_scale = (T)(1.0 / ((double)c.nx * c.ny));
         BCHK(launch_rows_r2c<T>(a, st_));
+        sim_pencils(c, sm, sq, nullptr, nullptr, -1);
+        sim_pencils(c, c.W.p + (sm - c.S.p), c.W.p + (sq - c.S.p), sm, sq, +1);
         BCHK(hipMemcpyAsync(c.mean.p + ls, &mean, sizeof(double), hipMemcpyHostToDevice, st_));
         BCHK(hipStreamSynchronize(st_));
     }
@@ -176,11 +182,14 @@ public:
         const int ls = sim - c.first;
         if (n != (long)c.ny * c.nx) throw BatchError(GPF_EINVAL, "buffer length does not match the sim grid");
         DBuf<T> tmp(n);
+        DBuf<cpx<T>> tk((size_t)c.ny * (c.nx / 2) + c.ny);
+        sim_pencils(c, tk.p, tk.p + (long)c.ny * (c.nx / 2), c.S.p + (long)ls * c.ny * (c.nx / 2),
+                    c.S.p + c.nmain + (long)ls * c.ny, +1);
         RowC2RArgs<T> a;
         a.tw = c.xtw.p;
         a.narr = 1;
-        a.main[0] = c.X.p + (long)ls * c.ny * (c.nx / 2);
-        a.nyq[0] = c.X.p + c.nmain + (long)ls * c.ny;
+        a.main[0] = tk.p;
+        a.nyq[0] = tk.p + (long)c.ny * (c.nx / 2);
         a.out[0] = tmp.p;
         a.rows = c.ny;
         a.nx = c.nx;
@@ -203,19 +212,15 @@ public:
         if (dirty_) compile();
         if (autotune_ && !tuned_) tune();
         int done = 0;
-        // graphs of kGraphSteps steps (even, so the state buffers return to their places);
-        // one graph per parity of the state-buffer assignment
+        // graphs of kGraphSteps steps; the state and work arrays stay in place
         if (nsteps >= kGraphSteps) {
-            if (!graph_exec_[parity_]) capture();
+            if (!graph_exec_) capture();
             while (nsteps - done >= kGraphSteps) {
-                BCHK(hipGraphLaunch(graph_exec_[parity_], st_));
+                BCHK(hipGraphLaunch(graph_exec_, st_));
                 done += kGraphSteps;
             }
         }
-        for (; done < nsteps; ++done) {
-            enqueue_step();
-            parity_ ^= 1;
-        }
+        for (; done < nsteps; ++done) enqueue_step();
     }
 
     void synchronize() override { BCHK(hipStreamSynchronize(st_)); }
@@ -286,18 +291,38 @@ private:
         dirty_ = false;
     }
 
-    RowFusedArgs<T> rows_args(BatchClass<T>& c, cpx<T>* X, cpx<T>* Y, cpx<T>* G, double* mean, int tw) {
+    // y transform of one sim's region (main plane and x-Nyquist column): dst = F_y^dir(src),
+    // in place when src is null
+    void sim_pencils(BatchClass<T>& c, cpx<T>* dm, cpx<T>* dq, const cpx<T>* sm, const cpx<T>* sq, int dir) {
+        PencilArgs<T> a;
+        a.tw = c.ytw.p;
+        a.narr = 1;
+        Seg m;
+        m.na = 1; m.nb = c.nx / 2; m.sa = 0; m.sb = 1; m.es = c.nx / 2;
+        Seg q;
+        q.na = 1; q.nb = 1; q.sa = 0; q.sb = 1; q.es = 1;
+        a.seg[0] = m;
+        a.seg[1] = q;
+        a.nseg = 2;
+        a.TW = pick_tw(c.ny, 0);
+        a.arr[0][0] = dm;
+        a.arr[0][1] = dq;
+        a.src[0][0] = sm;
+        a.src[0][1] = sq;
+        BCHK(launch_pencils<T>(a, c.ny, dir, st_));
+    }
+
+    RowFusedArgs<T> rows_args(BatchClass<T>& c, cpx<T>* W, double* mean, int tw, int var) {
         RowFusedArgs<T> ra;
         ra.tw = c.xtw.p;
-        ra.s_main = X;
-        ra.s_nyq = X + c.nmain;
-        ra.c_main = Y;
-        ra.c_nyq = Y + c.nmain;
-        ra.g_main = G;
-        ra.g_nyq = G + c.nmain;
+        ra.s_main = W;
+        ra.s_nyq = W + c.nmain;
+        ra.g_main = W;
+        ra.g_nyq = W + c.nmain;
         ra.rows = c.rows;
         ra.nx = c.nx;
-        ra.TW = tw > 0 ? tw : pick_tw(c.nx, 0);
+        ra.TW = tw > 0 ? tw : pick_tw(c.nx, var);
+        ra.variant = var;
         ra.npoly = npoly_;
         ra.mean = (sizeof(T) == 4) ? mean : nullptr;
         ra.rows_per_sim = c.ny;
@@ -305,14 +330,14 @@ private:
         return ra;
     }
 
-    PencilUpdateArgs<T> upd_args(BatchClass<T>& c, cpx<T>* Y, cpx<T>* G, double* mean, int tw) {
+    PencilUpdateArgs<T> upd_args(BatchClass<T>& c, cpx<T>* S, cpx<T>* W, double* mean, int tw) {
         PencilUpdateArgs<T> u;
         u.p.tw = c.ytw.p;
-        u.p.narr = shape_.has_src ? 2 : 1;
-        u.p.arr[0][0] = Y;
-        u.p.arr[0][1] = Y + c.nmain;
-        u.p.arr[1][0] = G;
-        u.p.arr[1][1] = G + c.nmain;
+        u.p.narr = 2;
+        u.p.arr[0][0] = S;
+        u.p.arr[0][1] = S + c.nmain;
+        u.p.arr[1][0] = W;
+        u.p.arr[1][1] = W + c.nmain;
         Seg m;  // y-pencils of every sim: a = sim, b = kx
         m.na = c.nsim; m.nb = c.nx / 2; m.sa = (long)c.ny * (c.nx / 2); m.sb = 1; m.es = c.nx / 2;
         m.ksrc[0] = 1; m.ksrc[1] = 2; m.ksrc[2] = 3; m.sim_src = 0;
@@ -342,9 +367,8 @@ private:
     void enqueue_step() {
         for (auto& cp : cls_) {
             BatchClass<T>& c = *cp;
-            BCHK(launch_rows_fused<T>(rows_args(c, c.X.p, c.Y.p, c.G.p, c.mean.p, c.tw_rows), st_));
-            BCHK(launch_pencil_update<T>(upd_args(c, c.Y.p, c.G.p, c.mean.p, c.tw_upd), c.ny, st_));
-            std::swap(c.X.p, c.Y.p);
+            if (shape_.has_src) BCHK(launch_rows_fused<T>(rows_args(c, c.W.p, c.mean.p, c.tw_rows, c.rows_var), st_));
+            BCHK(launch_pencil_update<T>(upd_args(c, c.S.p, c.W.p, c.mean.p, c.tw_upd), c.ny, st_));
         }
     }
 
@@ -353,7 +377,7 @@ private:
         BCHK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
         for (int s = 0; s < kGraphSteps; ++s) enqueue_step();
         BCHK(hipStreamEndCapture(st_, &g));
-        BCHK(hipGraphInstantiate(&graph_exec_[parity_], g, nullptr, nullptr, 0));
+        BCHK(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0));
         (void)hipGraphDestroy(g);
     }
 
@@ -365,9 +389,10 @@ private:
         const int cand[] = {4, 8, 16, 32, 64};
         for (auto& cp : cls_) {
             BatchClass<T>& c = *cp;
-            DBuf<cpx<T>> X(c.nk), Y(c.nk), G(c.nk);
+            DBuf<cpx<T>> S(c.nk), W(c.nk);
             DBuf<double> mu(c.nsim);
-            BCHK(hipMemcpyAsync(X.p, c.X.p, c.nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+            BCHK(hipMemcpyAsync(S.p, c.S.p, c.nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+            BCHK(hipMemcpyAsync(W.p, c.W.p, c.nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
             BCHK(hipMemcpyAsync(mu.p, c.mean.p, c.nsim * sizeof(double), hipMemcpyDeviceToDevice, st_));
             auto time_it = [&](auto&& launch) {
                 launch();
@@ -380,15 +405,16 @@ private:
                 return ms;
             };
             float best = 1e30f;
-            for (int tw : cand) {
-                if (!tw_ok(c.nx, tw, sizeof(cpx<T>), 0)) continue;
-                const float t = time_it([&] { BCHK(launch_rows_fused<T>(rows_args(c, X.p, Y.p, G.p, mu.p, tw), st_)); });
-                if (t < best) { best = t; c.tw_rows = tw; }
-            }
+            for (int var = 0; var < 2; ++var)
+                for (int tw : cand) {
+                    if (!tw_ok(c.nx, tw, sizeof(cpx<T>), var)) continue;
+                    const float t = time_it([&] { BCHK(launch_rows_fused<T>(rows_args(c, W.p, mu.p, tw, var), st_)); });
+                    if (t < best) { best = t; c.tw_rows = tw; c.rows_var = var; }
+                }
             best = 1e30f;
             for (int tw : cand) {
                 if (!tw_ok(c.ny, tw, sizeof(cpx<T>), 1)) continue;
-                const float t = time_it([&] { BCHK(launch_pencil_update<T>(upd_args(c, Y.p, G.p, mu.p, tw), c.ny, st_)); });
+                const float t = time_it([&] { BCHK(launch_pencil_update<T>(upd_args(c, S.p, W.p, mu.p, tw), c.ny, st_)); });
                 if (t < best) { best = t; c.tw_upd = tw; }
             }
             BCHK(hipStreamSynchronize(st_));
@@ -407,14 +433,11 @@ private:
     std::vector<std::map<std::string, std::complex<double>>> scalars_;
     int nsims_ = 0, npoly_ = 0;
     FusedShape shape_;
-    hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};
-    int parity_ = 0;
+    hipGraphExec_t graph_exec_ = nullptr;
 
     void drop_graphs() {
-        for (auto& g : graph_exec_) {
-            if (g) (void)hipGraphExecDestroy(g);
-            g = nullptr;
-        }
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
     }
 };
 

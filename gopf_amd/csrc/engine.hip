@@ -363,12 +363,13 @@ public:
                  const char* const* names, double* grad_scalars) override;
 
 private:
-    void state_to_real(const cpx<T>* X, const double* mean, T* out);
+    void state_to_real(const double* mean, T* out);  // c = inverse transform of the state + mean
     // transforms
     const cpx<T>* tw(int n);
     void pencils(const std::vector<cpx<T>*>& dst, const std::vector<const cpx<T>*>& src, int axis, int dir);
     void r2c(const std::vector<const T*>& in, const std::vector<cpx<T>*>& out, T in_scale);
-    void c2r(const std::vector<const cpx<T>*>& in, const std::vector<T*>& out, T scale);
+    void c2r(const std::vector<const cpx<T>*>& in, const std::vector<T*>& out, T scale,
+             const double* add_mean = nullptr);
     cpx<T>* kscratch(int i);
     T* rscratch(int i);
     // timing
@@ -376,12 +377,14 @@ private:
     void tend(int pass);
     // fused path
     void fused_step();
-    RowFusedArgs<T> rows_args(const cpx<T>* X, cpx<T>* Y, cpx<T>* Gb, double* mean, int tw);
-    PencilUpdateArgs<T> update_args(cpx<T>* Y, cpx<T>* Gb, double* mean, int tw);
+    RowFusedArgs<T> rows_args(cpx<T>* W, double* mean, int tw, int var);
+    PencilUpdateArgs<T> update_args(cpx<T>* S, cpx<T>* W, double* mean, int tw);
+    void prime();
     void pencils_tw(const std::vector<cpx<T>*>& dst, int axis, int dir, int tw);
     void autotune();
     bool autotune_ = false, tuned_ = false;
     int tw_rows_ = 0, tw_pf_ = 0, tw_pu_ = 0, tw_pi_ = 0;
+    int rows_var_ = 0;  // transform schedule of the row pass (0 plain, 1 low-register)
     // 3D: axis of the fused forward-update-inverse pass (2 = z, 1 = y); the other
     // of y / z is transformed forward before and inverse after it
     int upd_axis_ = 2;
@@ -417,8 +420,9 @@ private:
     bool kvalid_ = false;
     std::vector<Buf<cpx<T>>> ks_;
     std::vector<Buf<T>> rs_;
-    // fused-path storage
-    Buf<cpx<T>> X_, Y_, G_;
+    // fused-path storage: the state spectrum S = FFT(c - mean) / N and one work array W,
+    // which between steps holds S inverse-transformed along the update axis
+    Buf<cpx<T>> S_, W_;
     Buf<double> mean_;          // FP32 fused path: FP64 mean of the field (the arrays hold c - mean)
     bool split_mean_ = false;
     double host_mean();
@@ -462,9 +466,8 @@ Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gp
     }
     if (o.fused) fp_ = compile_fused(m, stepper_);
     if (fp_.ok) {
-        X_.alloc(g_.nk);
-        Y_.alloc(g_.nk);
-        G_.alloc(g_.nk);
+        S_.alloc(g_.nk);
+        W_.alloc(g_.nk);
         mean_.alloc(1);
         split_mean_ = (sizeof(T) == 4);
         pass_names_ = {"rows_fused", "pencils_fwd", "pencil_update", "pencils_inv"};
@@ -670,7 +673,8 @@ void Engine<T>::r2c(const std::vector<const T*>& in, const std::vector<cpx<T>*>&
 }
 
 template <typename T>
-void Engine<T>::c2r(const std::vector<const cpx<T>*>& in, const std::vector<T*>& out, T scale) {
+void Engine<T>::c2r(const std::vector<const cpx<T>*>& in, const std::vector<T*>& out, T scale,
+                    const double* add_mean) {
     for (size_t q0 = 0; q0 < in.size(); q0 += kMaxArr) {
         const int na = (int)std::min(in.size() - q0, (size_t)kMaxArr);
         std::vector<cpx<T>*> scr;
@@ -697,6 +701,7 @@ void Engine<T>::c2r(const std::vector<const cpx<T>*>& in, const std::vector<T*>&
         a.nx = g_.nx;
         a.TW = choose_tw(g_.nx, (na + 1) / 2, sizeof(cpx<T>), 1);
         a.scale = scale;
+        a.add_mean = (q0 == 0) ? add_mean : nullptr;
         HIPCHK(launch_rows_c2r<T>(a, st_));
     }
 }
@@ -720,17 +725,8 @@ void Engine<T>::set_field(int idx, const double* in) {
     if (fp_.ok) {
         T* tmp = rscratch(0);
         HIPCHK(hipMemcpyAsync(tmp, h.data(), g_.n * sizeof(T), hipMemcpyHostToDevice, st_));
-        RowR2CArgs<T> a;
-        a.tw = tw(g_.nx);
-        a.narr = 1;
-        a.in[0] = tmp;
-        a.main[0] = X_.get();
-        a.nyq[0] = X_.get() + g_.nmain;
-        a.rows = g_.rows;
-        a.nx = g_.nx;
-        a.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
-        a.in_scale = T(1) / (T)g_.nx;
-        HIPCHK(launch_rows_r2c<T>(a, st_));
+        r2c({tmp}, {S_.get()}, (T)(1.0 / (double)g_.n));
+        prime();
         HIPCHK(hipMemcpyAsync(mean_.get(), &mean, sizeof(double), hipMemcpyHostToDevice, st_));
         HIPCHK(hipStreamSynchronize(st_));
     } else {
@@ -746,17 +742,7 @@ void Engine<T>::get_field(int idx, double* out) {
     std::vector<T> h(g_.n);
     if (fp_.ok) {
         T* tmp = rscratch(0);
-        RowC2RArgs<T> a;
-        a.tw = tw(g_.nx);
-        a.narr = 1;
-        a.main[0] = X_.get();
-        a.nyq[0] = X_.get() + g_.nmain;
-        a.out[0] = tmp;
-        a.rows = g_.rows;
-        a.nx = g_.nx;
-        a.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
-        a.scale = T(1);
-        HIPCHK(launch_rows_c2r<T>(a, st_));
+        c2r({S_.get()}, {tmp}, T(1));
         HIPCHK(hipMemcpyAsync(h.data(), tmp, g_.n * sizeof(T), hipMemcpyDeviceToHost, st_));
     } else {
         HIPCHK(hipMemcpyAsync(h.data(), F_[idx].get(), g_.n * sizeof(T), hipMemcpyDeviceToHost, st_));
@@ -785,17 +771,7 @@ void Engine<T>::energy(int idx, int npoly, const double* poly, double kappa, dou
     const T* c;
     if (fp_.ok) {
         T* tmp = rscratch(0);
-        RowC2RArgs<T> a;
-        a.tw = tw(g_.nx);
-        a.narr = 1;
-        a.main[0] = X_.get();
-        a.nyq[0] = X_.get() + g_.nmain;
-        a.out[0] = tmp;
-        a.rows = g_.rows;
-        a.nx = g_.nx;
-        a.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
-        a.scale = T(1);
-        HIPCHK(launch_rows_c2r<T>(a, st_));
+        c2r({S_.get()}, {tmp}, T(1));
         c = tmp;
     } else {
         c = F_[idx].get();
@@ -847,18 +823,17 @@ long Engine<T>::series(double* e, double* m, long cap) {
 // fused single-field step
 // ---------------------------------------------------------------------------
 template <typename T>
-RowFusedArgs<T> Engine<T>::rows_args(const cpx<T>* X, cpx<T>* Y, cpx<T>* Gb, double* mean, int tw) {
+RowFusedArgs<T> Engine<T>::rows_args(cpx<T>* W, double* mean, int tw, int var) {
     RowFusedArgs<T> ra;
     ra.tw = this->tw(g_.nx);
-    ra.s_main = X;
-    ra.s_nyq = X + g_.nmain;
-    ra.c_main = Y;
-    ra.c_nyq = Y + g_.nmain;
-    ra.g_main = Gb;
-    ra.g_nyq = Gb + g_.nmain;
+    ra.s_main = W;
+    ra.s_nyq = W + g_.nmain;
+    ra.g_main = W;
+    ra.g_nyq = W + g_.nmain;
     ra.rows = g_.rows;
     ra.nx = g_.nx;
-    ra.TW = tw > 0 ? tw : choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
+    ra.TW = tw > 0 ? tw : choose_tw(g_.nx, 1, sizeof(cpx<T>), 1, var);
+    ra.variant = var;
     ra.npoly = (int)fp_.gpoly.size();
     for (int q = 0; q < ra.npoly; ++q) ra.gpoly[q] = fp_.gpoly[q];
     ra.mean = split_mean_ ? mean : nullptr;
@@ -866,16 +841,16 @@ RowFusedArgs<T> Engine<T>::rows_args(const cpx<T>* X, cpx<T>* Y, cpx<T>* Gb, dou
 }
 
 template <typename T>
-PencilUpdateArgs<T> Engine<T>::update_args(cpx<T>* Y, cpx<T>* Gb, double* mean, int tw) {
+PencilUpdateArgs<T> Engine<T>::update_args(cpx<T>* S, cpx<T>* W, double* mean, int tw) {
     PencilUpdateArgs<T> u;
     const int axis = upd_axis();
     const int nu = (axis == 2) ? g_.nz : g_.ny;
     u.p.tw = this->tw(nu);
-    u.p.narr = fp_.has_src ? 2 : 1;
-    u.p.arr[0][0] = Y;
-    u.p.arr[0][1] = Y + g_.nmain;
-    u.p.arr[1][0] = Gb;
-    u.p.arr[1][1] = Gb + g_.nmain;
+    u.p.narr = 2;
+    u.p.arr[0][0] = S;
+    u.p.arr[0][1] = S + g_.nmain;
+    u.p.arr[1][0] = W;
+    u.p.arr[1][1] = W + g_.nmain;
     u.p.seg[0] = (axis == 2) ? seg_z_main(g_) : seg_y_main(g_);
     u.p.seg[1] = (axis == 2) ? seg_z_nyq(g_) : seg_y_nyq(g_);
     u.p.nseg = 2;
@@ -890,6 +865,12 @@ PencilUpdateArgs<T> Engine<T>::update_args(cpx<T>* Y, cpx<T>* Gb, double* mean, 
     u.filter = has_filter_ ? filter_.get() : nullptr;
     u.mean = split_mean_ ? mean : nullptr;
     return u;
+}
+
+// W <- S inverse-transformed along the update axis: the row-pass input of the next step
+template <typename T>
+void Engine<T>::prime() {
+    pencils({W_.get()}, {S_.get()}, upd_axis(), +1);
 }
 
 template <typename T>
@@ -914,9 +895,10 @@ void Engine<T>::pencils_tw(const std::vector<cpx<T>*>& dst, int axis, int dir, i
 template <typename T>
 void Engine<T>::autotune() {
     if (!fp_.ok || tuned_) return;
-    Buf<cpx<T>> X(g_.nk), Y(g_.nk), Gb(g_.nk);
+    Buf<cpx<T>> Sb(g_.nk), Wb(g_.nk);
     Buf<double> mu(1);
-    HIPCHK(hipMemcpyAsync(X.get(), X_.get(), g_.nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+    HIPCHK(hipMemcpyAsync(Sb.get(), S_.get(), g_.nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+    HIPCHK(hipMemcpyAsync(Wb.get(), W_.get(), g_.nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
     HIPCHK(hipMemcpyAsync(mu.get(), mean_.get(), sizeof(double), hipMemcpyDeviceToDevice, st_));
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
@@ -944,7 +926,16 @@ void Engine<T>::autotune() {
         }
         return best;
     };
-    tw_rows_ = best_of(g_.nx, [&](int tw) { HIPCHK(launch_rows_fused<T>(rows_args(X.get(), Y.get(), Gb.get(), mu.get(), tw), st_)); });
+    {
+        // row pass: lines per block and transform schedule
+        float bt = 1e30f;
+        for (int var = 0; var < 2; ++var)
+            for (int tw : cand) {
+                if (!tw_fits(g_.nx, tw, sizeof(cpx<T>), var)) continue;
+                const float t = time_it([&] { HIPCHK(launch_rows_fused<T>(rows_args(Wb.get(), mu.get(), tw, var), st_)); });
+                if (t < bt) { bt = t; tw_rows_ = tw; rows_var_ = var; }
+            }
+    }
     // for each order of the y / z passes, tune the tile widths and keep the faster order
     const int orders = (g_.ndim == 3) ? 2 : 1;
     float best_total = 1e30f;
@@ -955,19 +946,15 @@ void Engine<T>::autotune() {
         float total = 0.f;
         const int no = axis_len(other_axis()), nu = axis_len(upd_axis());
         if (g_.ndim == 3) {
-            pf = best_of(no, [&](int tw) {
-                if (fp_.has_src) pencils_tw({Y.get(), Gb.get()}, other_axis(), -1, tw);
-                else pencils_tw({Y.get()}, other_axis(), -1, tw);
-            });
-            pi = best_of(no, [&](int tw) { pencils_tw({Y.get()}, other_axis(), +1, tw); });
-            total += time_it([&] {
-                if (fp_.has_src) pencils_tw({Y.get(), Gb.get()}, other_axis(), -1, pf);
-                else pencils_tw({Y.get()}, other_axis(), -1, pf);
-            });
-            total += time_it([&] { pencils_tw({Y.get()}, other_axis(), +1, pi); });
+            pi = best_of(no, [&](int tw) { pencils_tw({Wb.get()}, other_axis(), +1, tw); });
+            total += time_it([&] { pencils_tw({Wb.get()}, other_axis(), +1, pi); });
+            if (fp_.has_src) {
+                pf = best_of(no, [&](int tw) { pencils_tw({Wb.get()}, other_axis(), -1, tw); });
+                total += time_it([&] { pencils_tw({Wb.get()}, other_axis(), -1, pf); });
+            }
         }
-        pu = best_of(nu, [&](int tw) { HIPCHK(launch_pencil_update<T>(update_args(Y.get(), Gb.get(), mu.get(), tw), nu, st_)); }, 1);
-        total += time_it([&] { HIPCHK(launch_pencil_update<T>(update_args(Y.get(), Gb.get(), mu.get(), pu), nu, st_)); });
+        pu = best_of(nu, [&](int tw) { HIPCHK(launch_pencil_update<T>(update_args(Sb.get(), Wb.get(), mu.get(), tw), nu, st_)); }, 1);
+        total += time_it([&] { HIPCHK(launch_pencil_update<T>(update_args(Sb.get(), Wb.get(), mu.get(), pu), nu, st_)); });
         if (total < best_total) {
             best_total = total;
             best[0] = pf; best[1] = pu; best[2] = pi; best[3] = upd_axis_;
@@ -981,22 +968,23 @@ void Engine<T>::autotune() {
     HIPCHK(hipEventDestroy(e1));
     HIPCHK(hipStreamSynchronize(st_));
     tuned_ = true;
+    prime();  // the work array follows the chosen update axis
 }
 
 template <typename T>
 int Engine<T>::tuning(int* tws, int maxn) {
-    const int v[5] = {tw_rows_, tw_pf_, tw_pu_, tw_pi_, upd_axis()};
-    for (int i = 0; i < 5 && i < maxn; ++i) tws[i] = v[i];
-    return 5;
+    const int v[6] = {tw_rows_, tw_pf_, tw_pu_, tw_pi_, upd_axis(), rows_var_};
+    for (int i = 0; i < 6 && i < maxn; ++i) tws[i] = v[i];
+    return 6;
 }
 
 template <typename T>
 void Engine<T>::fused_step() {
     if (autotune_ && !tuned_) autotune();
-    RowFusedArgs<T> ra = rows_args(X_.get(), Y_.get(), G_.get(), mean_.get(), tw_rows_);
-    const long nblk_a = rows_blocks(g_.rows, ra.TW);
+    RowFusedArgs<T> ra = rows_args(W_.get(), mean_.get(), tw_rows_, rows_var_);
+    const long nblk_a = rows_fused_blocks(g_.rows, ra.TW);
     const int nu = axis_len(upd_axis());
-    PencilUpdateArgs<T> u = update_args(Y_.get(), G_.get(), mean_.get(), tw_pu_);
+    PencilUpdateArgs<T> u = update_args(S_.get(), W_.get(), mean_.get(), tw_pu_);
     int tiles0 = 0;
     const long nblk_c = pencil_blocks(u.p.seg, u.p.nseg, u.p.TW, &tiles0);
     if (mon_) {
@@ -1017,30 +1005,32 @@ void Engine<T>::fused_step() {
         ra.mass_partial = pa_mass_.get();
         u.egrad_partial = pc_eg_.get();
     }
-    tbegin(0);
-    HIPCHK(launch_rows_fused<T>(ra, st_));
-    tend(0);
-    if (g_.ndim == 3) {
-        tbegin(1);
-        if (fp_.has_src) pencils_tw({Y_.get(), G_.get()}, other_axis(), -1, tw_pf_);
-        else pencils_tw({Y_.get()}, other_axis(), -1, tw_pf_);
-        tend(1);
+    // W holds S inverse-transformed along the update axis; the real-space pass is needed
+    // for the explicit source and for the monitor's bulk sums
+    if (fp_.has_src || mon_) {
+        if (g_.ndim == 3) {
+            tbegin(3);
+            pencils_tw({W_.get()}, other_axis(), +1, tw_pi_);
+            tend(3);
+        }
+        tbegin(0);
+        HIPCHK(launch_rows_fused<T>(ra, st_));
+        tend(0);
+        if (g_.ndim == 3 && fp_.has_src) {
+            tbegin(1);
+            pencils_tw({W_.get()}, other_axis(), -1, tw_pf_);
+            tend(1);
+        }
     }
     tbegin(2);
     HIPCHK(launch_pencil_update<T>(u, nu, st_));
     tend(2);
-    if (g_.ndim == 3) {
-        tbegin(3);
-        pencils_tw({Y_.get()}, other_axis(), +1, tw_pi_);
-        tend(3);
-    }
     if (mon_) {
         HIPCHK(launch_reduce2(pa_bulk_.get(), nblk_a, 1.0, pc_eg_.get(), nblk_c, 0.5 * mon_kappa_ / (double)g_.n,
                               series_.get() + 2 * series_n_, st_));
         HIPCHK(launch_reduce2(pa_mass_.get(), nblk_a, 1.0, pa_mass_.get(), 0, 0.0, series_.get() + 2 * series_n_ + 1, st_));
         ++series_n_;
     }
-    X_.swap(Y_);
 }
 
 // ---------------------------------------------------------------------------
@@ -1296,19 +1286,8 @@ void Engine<T>::on_step_finished() {
 // spectrum with Hermitian weights.  States are recomputed per segment from checkpoints
 // of the fused state taken every `every` steps.
 template <typename T>
-void Engine<T>::state_to_real(const cpx<T>* X, const double* mean, T* out) {
-    RowC2RArgs<T> a;
-    a.tw = tw(g_.nx);
-    a.narr = 1;
-    a.main[0] = X;
-    a.nyq[0] = X + g_.nmain;
-    a.out[0] = out;
-    a.rows = g_.rows;
-    a.nx = g_.nx;
-    a.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
-    a.scale = T(1);
-    a.add_mean = split_mean_ ? mean : nullptr;
-    HIPCHK(launch_rows_c2r<T>(a, st_));
+void Engine<T>::state_to_real(const double* mean, T* out) {
+    c2r({S_.get()}, {out}, T(1), split_mean_ ? mean : nullptr);
 }
 
 template <typename T>
@@ -1369,14 +1348,14 @@ void Engine<T>::adjoint(long nsteps, long every, const double* target, double* l
     for (long s = 0; s < nsteps; ++s) {
         if (s % every == 0) {
             ck.emplace_back(nk);
-            HIPCHK(hipMemcpyAsync(ck.back().get(), X_.get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+            HIPCHK(hipMemcpyAsync(ck.back().get(), S_.get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
             HIPCHK(hipMemcpyAsync(ckmean.get() + s / every, mean_.get(), sizeof(double), hipMemcpyDeviceToDevice, st_));
         }
         fused_step();
     }
-    Buf<cpx<T>> Xf(nk);
+    Buf<cpx<T>> Sf(nk);
     Buf<double> meanf(1);
-    HIPCHK(hipMemcpyAsync(Xf.get(), X_.get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+    HIPCHK(hipMemcpyAsync(Sf.get(), S_.get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
     HIPCHK(hipMemcpyAsync(meanf.get(), mean_.get(), sizeof(double), hipMemcpyDeviceToDevice, st_));
     // loss and lam_T = 2 (c_T - target), in FP64 on the host
     std::vector<double> cT(2 * n);
@@ -1430,13 +1409,14 @@ void Engine<T>::adjoint(long nsteps, long every, const double* target, double* l
     const int ndg = (int)dg.size();
     for (long seg = nck - 1; seg >= 0; --seg) {
         const long s0 = seg * every, s1 = std::min(nsteps, s0 + every);
-        HIPCHK(hipMemcpyAsync(X_.get(), ck[seg].get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+        HIPCHK(hipMemcpyAsync(S_.get(), ck[seg].get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
         HIPCHK(hipMemcpyAsync(mean_.get(), ckmean.get() + seg, sizeof(double), hipMemcpyDeviceToDevice, st_));
+        prime();
         for (long s = s0; s < s1; ++s) {
-            state_to_real(X_.get(), mean_.get(), R[s - s0].get());
+            state_to_real(mean_.get(), R[s - s0].get());
             fused_step();
         }
-        state_to_real(X_.get(), mean_.get(), R[s1 - s0].get());
+        state_to_real(mean_.get(), R[s1 - s0].get());
         for (long s = s1 - 1; s >= s0; --s) {
             const T* cn = R[s - s0].get();
             std::vector<const T*> rin = {lam.get()};
@@ -1465,8 +1445,9 @@ void Engine<T>::adjoint(long nsteps, long every, const double* target, double* l
     HIPCHK(hipMemcpyAsync(hl.data(), lam.get(), n * sizeof(T), hipMemcpyDeviceToHost, st_));
     std::vector<double> hs(std::max(nscal, 1));
     HIPCHK(hipMemcpyAsync(hs.data(), acc.get(), hs.size() * sizeof(double), hipMemcpyDeviceToHost, st_));
-    HIPCHK(hipMemcpyAsync(X_.get(), Xf.get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+    HIPCHK(hipMemcpyAsync(S_.get(), Sf.get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
     HIPCHK(hipMemcpyAsync(mean_.get(), meanf.get(), sizeof(double), hipMemcpyDeviceToDevice, st_));
+    prime();
     HIPCHK(hipStreamSynchronize(st_));
     if (grad_field)
         for (long i = 0; i < n; ++i) grad_field[i] = (double)hl[i];

@@ -59,13 +59,16 @@ struct PencilArgs {
     int n = 0;                           // pencil length (runtime sizes)
 };
 
-// fused "forward -> k-space update -> inverse" pencil pass of the single-field fast path
+// fused update pass of the single-field fast path, along the last axis of the step:
+//   W (source g, transformed along every other axis) --forward--> G^
+//   S <- m1 S + m2 G^ / N            (S: the state spectrum F(c - mean) / N, in place)
+//   W <- inverse transform of the new S along this axis (the next step's row-pass input)
 template <typename T>
 struct PencilUpdateArgs {
-    PencilArgs<T> p;                     // arr[0] = field spectrum (in/out), arr[1] = explicit source
+    PencilArgs<T> p;                     // arr[0] = state spectrum S (in/out), arr[1] = work W (in/out)
     KGrid g;
     double dt = 0.0;
-    double scale = 1.0;                  // 1 / N_total folded into the update
+    double scale = 1.0;                  // 1 / N_total applied to the source
     KPoly denom;                         // implicit part D(k)
     KPoly lin;                           // explicit terms linear in the field E(k)
     int src_lap = 0;                     // LAP power of the explicit source
@@ -73,7 +76,7 @@ struct PencilUpdateArgs {
     const double* filter = nullptr;      // Vandeven table (1000 entries) or null
     double* mean = nullptr;              // FP64 k = 0 mode: the field mean (in/out), or null when the
                                          // k = 0 coefficient is carried by the arrays themselves
-    double* egrad_partial = nullptr;     // per-block FP64 sum of w |k|^2 |C|^2 or null
+    double* egrad_partial = nullptr;     // per-block FP64 sum of w (2 pi |f|)^2 |FFT(c)|^2 or null
     // batched sims (Seg::sim_src >= 0): per-sim coefficients of denom / lin ([sim][kMaxKTerms],
     // powers from denom / lin) and mean[sim]
     const double* denom_sim = nullptr;
@@ -110,28 +113,29 @@ struct RowC2RArgs {
 };
 
 // fused x-row pass of the single-field fast path:
-//   S (x half spectrum) --C2R--> c --poly--> g --R2C--> (C, G) half spectra
+//   x half spectra of c --C2R--> c --poly--> g --R2C--> x half spectra of g
+// Each transform line carries a pair of rows (r, r + 1) as one complex sequence, in
+// both directions.  The output may alias the input (each row is read whole first).
 template <typename T>
 struct RowFusedArgs {
     const cpx<T>* tw = nullptr;
     const cpx<T>* s_main = nullptr;
     const cpx<T>* s_nyq = nullptr;
-    cpx<T>* c_main = nullptr;
-    cpx<T>* c_nyq = nullptr;
     cpx<T>* g_main = nullptr;
     cpx<T>* g_nyq = nullptr;
     T* c_out = nullptr;                  // optional real-space copy of c
     const double* mean = nullptr;        // FP64 mean held outside the arrays (state = c - mean), or null
     long rows = 0;
     int nx = 0;
-    int TW = 16;
+    int TW = 16;                         // transform lines (row pairs) per block
+    int variant = 0;                     // transform schedule: 0 plain, 1 low-register
     int npoly = 0;                       // g(c) = sum_q gpoly[q] c^q
     double gpoly[kMaxPoly];
     int nbulk = 0;                       // optional energy density f(c) = sum_q bulk[q] c^q
     double bulk[kMaxPoly];
     double* bulk_partial = nullptr;      // per-block FP64 sum of f(c)
     double* mass_partial = nullptr;      // per-block FP64 sum of c
-    // batched sims: rows_per_sim > 0 selects gpoly_sim[sim][kMaxPoly] and mean[sim], sim = row / rows_per_sim
+    // batched sims: rows_per_sim > 0 (even) selects gpoly_sim[sim][kMaxPoly] and mean[sim], sim = row / rows_per_sim
     long rows_per_sim = 0;
     const double* gpoly_sim = nullptr;
 };
@@ -149,6 +153,7 @@ int fft_threads_per_line(int n, int variant = 0);  // threads per line (variant 
 
 // blocks a launch uses (for sizing per-block partial-sum buffers)
 long rows_blocks(long rows, int TW);
+long rows_fused_blocks(long rows, int TW);  // the fused row pass: TW row pairs per block
 long pencil_blocks(const Seg* seg, int nseg, int TW, int* tiles0);
 
 }  // namespace gpf

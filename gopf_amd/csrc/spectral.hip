@@ -8,14 +8,20 @@
 //   k_pencils        C2C transform of pencils along a strided or contiguous axis
 //                    (y / z passes over the half spectrum, and every axis of the
 //                    full-complex FourierTransform drop-in, fftWrap.go:26-39)
-//   k_pencil_update  forward transform -> semi-implicit k-space update -> inverse
-//                    transform of the last axis (euler.go:27-39 with
+//   k_pencil_update  forward transform of the explicit source along the last axis ->
+//                    semi-implicit update of the state spectrum (euler.go:27-39 with
 //                    LaplacianN.Eval diffOp.go:25-29, ApplyModalFilter util.go:125-132)
+//                    -> inverse transform of the new state along that axis
 //   k_rows_r2c       x-row transforms of pairs of real arrays into half spectra
 //   k_rows_c2r       x-row inverse of half spectra into real rows (euler.go:42-45)
 //   k_rows_fused     x-row inverse -> pointwise explicit source g(c) (SyncDerivedFields,
-//                    model.go:237-241) -> x-row forward of (c, g); optional FP64
-//                    energy/mass partial sums
+//                    model.go:237-241) -> x-row forward of g, two rows per transform;
+//                    optional FP64 energy/mass partial sums
+//
+// The fused step keeps the state as its spectrum S = FFT(c - mean) / N: the reference
+// transforms the field forward and back every step (euler.go:20-45), which here would
+// add the rounding of two full transforms to the state each step; only the explicit
+// source, a small correction, passes through the transforms.
 #include <type_traits>
 
 #include "fft_core.h"
@@ -250,11 +256,13 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencils(PencilArgs<T> a) {
     }
 }
 
-template <typename T, int N>
+// MON: the energy monitor's partial sums are compiled in (a separate instantiation, so
+// that the common path does not carry their registers)
+template <typename T, int N, bool MON>
 __global__ void __launch_bounds__(kMaxBlock) k_pencil_update(PencilUpdateArgs<T> u) {
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
-    constexpr int V = 1;  // low-register schedule: three lines stay in registers
+    constexpr int V = 1;  // low-register schedule: two lines and the multipliers stay in registers
     constexpr int E = LineShape<N, V>::E;
     const PencilArgs<T>& a = u.p;
     const int n = (N > 0) ? N : a.n;
@@ -263,8 +271,8 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencil_update(PencilUpdateArgs<T>
     const LineMap lm = line_map(ti.es == 1, a.TW, TT, n);
     const bool valid = lm.l < ti.nvalid;
     const long base = ti.base + (long)lm.l * ti.sb;
-    cpx<T>* carr = pick(a.arr[0][0], a.arr[0][1], ti.s);
-    const cpx<T>* garr = pick(a.arr[1][0], a.arr[1][1], ti.s);
+    cpx<T>* sarr = pick(a.arr[0][0], a.arr[0][1], ti.s);
+    cpx<T>* warr = pick(a.arr[1][0], a.arr[1][1], ti.s);
     const double inx = 1.0 / (double)u.g.nx, iny = 1.0 / (double)u.g.ny, inz = 1.0 / (double)u.g.nz;
     // 1) k-space multipliers of this thread's elements, computed before any data is
     //    live to keep register pressure low.  The pencil coordinate p is one of
@@ -323,14 +331,14 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencil_update(PencilUpdateArgs<T>
             m1_0 = m1;
             m2_0 = m2;
         }
-        s1a[e] = (T)(m1 * u.scale);
+        s1a[e] = (T)(m1 - 1.0);  // the update adds a correction to S: m1 itself is never rounded
         s2a[e] = (T)(m2 * u.scale);
     }
-    // 2) explicit source: forward transform, scale by its multiplier
+    // 2) explicit source: forward transform along this axis, scale by its multiplier
     cpx<T> gs[E];
     double g0 = 0.0;
     if (u.has_src) {
-        load_line<T, E>(gs, garr, base, ti.es, lm.t, TT, valid);
+        load_line<T, E>(gs, warr, base, ti.es, lm.t, TT, valid);
         line_fft<T, N, -1, V>(gs, lds + lm.off, lm.es, n, lm.t, a.tw);
         g0 = (double)gs[0].x;
 #pragma unroll
@@ -339,19 +347,18 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencil_update(PencilUpdateArgs<T>
 #pragma unroll
         for (int e = 0; e < E; ++e) gs[e] = mk<T>(T(0), T(0));
     }
-    // 3) field: forward transform, semi-implicit update
+    // 3) semi-implicit update of the state spectrum, which never leaves k-space
     cpx<T> c[E];
-    load_line<T, E>(c, carr, base, ti.es, lm.t, TT, valid);
-    line_fft<T, N, -1, V>(c, lds + lm.off, lm.es, n, lm.t, a.tw);
+    load_line<T, E>(c, sarr, base, ti.es, lm.t, TT, valid);
     double eg = 0.0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const cpx<T> cv = c[e];
-        if (valid && u.egrad_partial) {
+        if (MON && valid) {
             const double fv = wrapk(lm.t + TT * e, nv) * invv;
             eg += wk * 4.0 * kPi * kPi * (fixed2 + fv * fv) * ((double)cv.x * (double)cv.x + (double)cv.y * (double)cv.y);
         }
-        c[e] = mk<T>(s1a[e] * cv.x + gs[e].x, s1a[e] * cv.y + gs[e].y);
+        c[e] = mk<T>(cv.x + (s1a[e] * cv.x + gs[e].x), cv.y + (s1a[e] * cv.y + gs[e].y));
     }
     if (has_k0) {
         // k = 0 advances in FP64: the arrays hold c - mean, so their k = 0 coefficient stays 0
@@ -360,13 +367,16 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencil_update(PencilUpdateArgs<T>
         *mean = m1_0 * mu + m2_0 * g0 * u.scale;
         c[0] = mk<T>(T(0), T(0));
     }
-    if (u.egrad_partial) {
+    store_line<T, E>(c, sarr, base, ti.es, lm.t, TT, valid);
+    if (MON) {
+        // the state is FFT(c) / N: |FFT(c)|^2 = |S|^2 / scale^2
         double* red = reinterpret_cast<double*>(lds + lds_elems(n, a.TW));
         const double tot = block_sum(eg, red);
-        if (threadIdx.x == 0) u.egrad_partial[blockIdx.x] = tot;
+        if (threadIdx.x == 0) u.egrad_partial[blockIdx.x] = tot / (u.scale * u.scale);
     }
+    // 4) the next step's row-pass input: the new state inverse-transformed along this axis
     line_fft<T, N, +1, V>(c, lds + lm.off, lm.es, n, lm.t, a.tw);
-    store_line<T, E>(c, carr, base, ti.es, lm.t, TT, valid);
+    store_line<T, E>(c, warr, base, ti.es, lm.t, TT, valid);
 }
 
 // x-rows: t fastest; the half spectrum split needs Z(k) and Z(n-k) from other threads
@@ -411,6 +421,44 @@ __device__ __forceinline__ void load_herm(cpx<T>* v, const cpx<T>* __restrict__ 
             else x = conjg(m[row * nh + (n - p)]);
         }
         v[e] = x;
+    }
+}
+
+// rows r1 and r1 + 1 (when valid) of a half-spectrum array, Hermitian-extended and
+// packed as A + i B, so that one inverse transform yields both real rows.  Element e of
+// thread t is position p = t + TT e: below nh it is read directly, above nh it is the
+// conjugate of position n - p.  With a compiled length (N > 0, TT = TTc) the region of
+// each e is known at compile time, so the loads use per-row base pointers with immediate
+// offsets; only the element that straddles nh tests at run time.
+template <typename T, int E, int TTc>
+__device__ __forceinline__ void load_herm_pair(cpx<T>* v, const cpx<T>* m, const cpx<T>* q, long r1, int n, int t,
+                                               int TT, bool v1, bool v2) {
+    const int nh = n / 2;
+    const cpx<T>* a1 = m + r1 * nh + t;        // direct: row r1 at position t + TT e
+    const cpx<T>* b1 = m + r1 * nh + (n - t);  // mirror: row r1 at position n - t - TT e
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int lo = (TTc > 0 ? TTc : TT) * e;
+        const int hi = lo + (TTc > 0 ? TTc : TT) - 1;
+        const int p = t + lo;
+        cpx<T> A = mk<T>(T(0), T(0)), B = A;
+        if (TTc > 0 && hi < nh) {
+            if (v1) A = a1[lo];
+            if (v2) B = a1[lo + nh];
+        } else if (TTc > 0 && lo > nh) {
+            if (v1) A = conjg(b1[-lo]);
+            if (v2) B = conjg(b1[nh - lo]);
+        } else if (p < nh) {
+            if (v1) A = a1[lo];
+            if (v2) B = a1[lo + nh];
+        } else if (p == nh) {
+            if (v1) A = q[r1];
+            if (v2) B = q[r1 + 1];
+        } else {
+            if (v1) A = conjg(b1[-lo]);
+            if (v2) B = conjg(b1[nh - lo]);
+        }
+        v[e] = mk<T>(A.x - B.y, A.y + B.x);
     }
 }
 
@@ -486,41 +534,65 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_c2r(RowC2RArgs<T> a) {
     }
 }
 
-template <typename T, int N>
+// V: transform schedule (0 plain, 1 low-register); MON: monitor partial sums compiled in
+template <typename T, int N, int V, bool MON>
 __global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
-    constexpr int V = plain_variant<N>();
     constexpr int E = LineShape<N, V>::E;
     const int n = (N > 0) ? N : a.nx;
     const int TT = (N > 0) ? LineShape<N, V>::T : n;
     const LineMap lm = line_map(true, a.TW, TT, n);
-    const long row = (long)blockIdx.x * a.TW + lm.l;
-    const bool valid = row < a.rows;
-    cpx<T> v[E];
-    load_herm<T, E>(v, a.s_main, a.s_nyq, row, n, lm.t, TT, valid);
-    line_fft<T, N, +1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
-    // real space: c = (c - mean) + mean, g(c), optional output and FP64 partial sums
-    KSum sb, sm;
-    const long sim = (a.rows_per_sim > 0 && valid) ? row / a.rows_per_sim : -1;
+    // this line carries rows r1 and r1 + 1
+    const long r1 = 2 * ((long)blockIdx.x * a.TW + lm.l);
+    const bool v1 = r1 < a.rows, v2 = r1 + 1 < a.rows;
+    const long sim = (a.rows_per_sim > 0 && v1) ? r1 / a.rows_per_sim : -1;
     const double mu = a.mean ? a.mean[sim >= 0 ? sim : 0] : 0.0;
     const T muT = (T)mu;
+    // g about the mean: g(mu + dc) = sum_q gp[q] dc^q, the Taylor shift of the coefficients
+    // formed in FP64 (repeated synthetic division), so that the bulk evaluation rounds
+    // relative to the fluctuation dc rather than to c
     T gp[kMaxPoly];
+    {
+        double gd[kMaxPoly];
 #pragma unroll
-    for (int q = 0; q < kMaxPoly; ++q) gp[q] = (T)((sim >= 0) ? a.gpoly_sim[sim * kMaxPoly + q] : a.gpoly[q]);
+        for (int q = 0; q < kMaxPoly; ++q)
+            gd[q] = (q < a.npoly) ? ((sim >= 0) ? a.gpoly_sim[sim * kMaxPoly + q] : a.gpoly[q]) : 0.0;
+        if (a.mean) {
+#pragma unroll
+            for (int i = 0; i < kMaxPoly - 1; ++i)
+#pragma unroll
+                for (int j = kMaxPoly - 2; j >= i; --j) gd[j] += mu * gd[j + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < kMaxPoly; ++q) gp[q] = (T)gd[q];
+    }
+    cpx<T> v[E];
+    load_herm_pair<T, E, (N > 0 ? LineShape<N, V>::T : 0)>(v, a.s_main, a.s_nyq, r1, n, lm.t, TT, v1, v2);
+    line_fft<T, N, +1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
+    // real space (row r1 in x, row r1 + 1 in y): g, optional output and FP64 partial sums
+    KSum sb, sm;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const T dc = v[e].x;
-        const T c = dc + muT;
-        T gv = T(0);
+        const T d1 = v[e].x, d2 = v[e].y;
+        T g1 = T(0), g2 = T(0);
 #pragma unroll
         for (int q = kMaxPoly - 1; q >= 0; --q)
-            if (q < a.npoly) gv = gv * c + gp[q];
-        v[e] = mk<T>(dc, gv);
-        if (valid) {
-            if (a.c_out) a.c_out[row * n + lm.t + TT * e] = c;
-            if (a.bulk_partial) {
-                const double cd = (double)dc + mu;
+            if (q < a.npoly) {
+                g1 = g1 * d1 + gp[q];
+                g2 = g2 * d2 + gp[q];
+            }
+        v[e] = mk<T>(g1, g2);
+        const long off = r1 * n + lm.t + TT * e;
+        if (a.c_out) {
+            if (v1) a.c_out[off] = d1 + muT;
+            if (v2) a.c_out[off + n] = d2 + muT;
+        }
+        if (MON) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (h == 0 ? !v1 : !v2) continue;
+                const double cd = (double)(h == 0 ? d1 : d2) + mu;
                 double f = 0.0;
 #pragma unroll
                 for (int q = kMaxPoly - 1; q >= 0; --q)
@@ -530,7 +602,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
             }
         }
     }
-    if (a.bulk_partial) {
+    if (MON) {
         double* red = reinterpret_cast<double*>(lds + lds_elems(n, a.TW));
         const double b = block_sum(sb.get(), red);
         const double ms = block_sum(sm.get(), red);
@@ -540,7 +612,8 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
         }
     }
     line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
-    split_store<T, E>(v, lds + lm.off, n, lm.t, TT, row, valid, a.c_main, a.c_nyq, a.g_main, a.g_nyq);
+    split_store<T, E>(v, lds + lm.off, n, lm.t, TT, r1, v1, a.g_main, a.g_nyq, v2 ? a.g_main + n / 2 : nullptr,
+                      v2 ? a.g_nyq + 1 : nullptr);
 }
 
 // v: 0 = plain pass, 1 = update pass
@@ -596,6 +669,7 @@ bool fft_size_supported(int n) { return fft_size_fast(n) || (n >= 1 && n <= 64);
 int fft_threads_per_line(int n, int variant) { return threads_per_line(n, variant); }
 
 long rows_blocks(long rows, int TW) { return (rows + TW - 1) / TW; }
+long rows_fused_blocks(long rows, int TW) { return ((rows + 1) / 2 + TW - 1) / TW; }
 
 long pencil_blocks(const Seg* seg, int nseg, int TW, int* tiles0) {
     long tot = 0;
@@ -631,7 +705,8 @@ hipError_t launch_pencil_update(const PencilUpdateArgs<T>& u0, int n, hipStream_
     if (u.egrad_partial) smem += (size_t)thr * sizeof(double);
     return dispatch_size(n, [&](auto NC) -> hipError_t {
         constexpr int NN = decltype(NC)::value;
-        return launch_k(k_pencil_update<T, NN>, dim3(blocks), dim3(thr), smem, s, u);
+        if (u.egrad_partial) return launch_k(k_pencil_update<T, NN, true>, dim3(blocks), dim3(thr), smem, s, u);
+        return launch_k(k_pencil_update<T, NN, false>, dim3(blocks), dim3(thr), smem, s, u);
     });
 }
 
@@ -659,13 +734,19 @@ hipError_t launch_rows_c2r(const RowC2RArgs<T>& a, hipStream_t s) {
 
 template <typename T>
 hipError_t launch_rows_fused(const RowFusedArgs<T>& a, hipStream_t s) {
-    const long blocks = rows_blocks(a.rows, a.TW);
-    const int thr = a.TW * threads_per_line(a.nx);
+    const long blocks = rows_fused_blocks(a.rows, a.TW);
+    const int thr = a.TW * threads_per_line(a.nx, a.variant);
     size_t smem = lds_elems(a.nx, a.TW) * sizeof(cpx<T>);
     if (a.bulk_partial) smem += (size_t)thr * sizeof(double);
     return dispatch_size(a.nx, [&](auto NC) -> hipError_t {
         constexpr int NN = decltype(NC)::value;
-        return launch_k(k_rows_fused<T, NN>, dim3(blocks), dim3(thr), smem, s, a);
+        constexpr int V0 = plain_variant<NN>();
+        if (a.variant || V0) {
+            if (a.bulk_partial) return launch_k(k_rows_fused<T, NN, 1, true>, dim3(blocks), dim3(thr), smem, s, a);
+            return launch_k(k_rows_fused<T, NN, 1, false>, dim3(blocks), dim3(thr), smem, s, a);
+        }
+        if (a.bulk_partial) return launch_k(k_rows_fused<T, NN, 0, true>, dim3(blocks), dim3(thr), smem, s, a);
+        return launch_k(k_rows_fused<T, NN, 0, false>, dim3(blocks), dim3(thr), smem, s, a);
     });
 }
 

@@ -142,8 +142,9 @@ int gpf_solver_is_fused(const gpf_solver* s);
 int gpf_solver_enable_timing(gpf_solver* s, int on);
 int gpf_solver_pass_times(gpf_solver* s, int maxpass, double* ms, long* launches, char* names, int namelen);
 /* launch configuration of the fused step: lines per block of the rows, forward, update and
- * inverse passes (0 = default heuristic) and, in 3D, the axis of the update pass (2 = z, 1 = y);
- * returns the number of values written */
+ * inverse passes (0 = default heuristic), in 3D the axis of the update pass (2 = z, 1 = y),
+ * and the transform schedule of the row pass (0 plain, 1 low-register); returns the number
+ * of values written */
 int gpf_solver_tuning(gpf_solver* s, int* tws, int maxn);
 /* the HIP stream the solver launches on (hipStream_t) */
 void* gpf_solver_stream(gpf_solver* s);
