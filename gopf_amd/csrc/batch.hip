@@ -255,6 +255,7 @@ private:
         std::vector<double> gp(kMaxPoly * (size_t)nsims_, 0.0), de(kMaxKTerms * (size_t)nsims_, 0.0),
             li(kMaxKTerms * (size_t)nsims_, 0.0);
         bool first = true;
+        dense_ = -1;
         for (int s = 0; s < nsims_; ++s) {
             Model m = base_;
             m.scalars = scalars_[s];
@@ -269,8 +270,22 @@ private:
                 throw BatchError(GPF_EINVAL, "every sim must have the same term structure");
             }
             for (int q = 0; q < kMaxPoly && q < (int)fs.gpoly.size(); ++q) gp[s * kMaxPoly + q] = fs.gpoly[q];
-            for (int t = 0; t < fs.denom.n; ++t) de[s * kMaxKTerms + t] = fs.denom.coef[t];
-            for (int t = 0; t < fs.lin.n; ++t) li[s * kMaxKTerms + t] = fs.lin.coef[t];
+            if (dense_ < 0) {
+                PencilUpdateArgs<T> probe;
+                probe.denom = fs.denom;
+                probe.lin = fs.lin;
+                probe.src_lap = fs.src_lap;
+                dense_ = make_dense(probe);
+            }
+            // per-sim rows: dense coefficients by power, or per-term coefficients
+            for (int t = 0; t < fs.denom.n; ++t) {
+                if (dense_) de[s * kMaxKTerms + fs.denom.pow[t]] += fs.denom.coef[t];
+                else de[s * kMaxKTerms + t] = fs.denom.coef[t];
+            }
+            for (int t = 0; t < fs.lin.n; ++t) {
+                if (dense_) li[s * kMaxKTerms + fs.lin.pow[t]] += fs.lin.coef[t];
+                else li[s * kMaxKTerms + t] = fs.lin.coef[t];
+            }
         }
         npoly_ = 0;
         for (int s = 0; s < nsims_; ++s)
@@ -359,6 +374,7 @@ private:
         u.src_lap = shape_.src_lap;
         u.has_src = shape_.has_src ? 1 : 0;
         u.mean = (sizeof(T) == 4) ? mean : nullptr;
+        u.dense = dense_;
         u.denom_sim = c.denom.p;
         u.lin_sim = c.lin.p;
         return u;
@@ -432,6 +448,7 @@ private:
     std::vector<std::unique_ptr<BatchClass<T>>> cls_;
     std::vector<std::map<std::string, std::complex<double>>> scalars_;
     int nsims_ = 0, npoly_ = 0;
+    int dense_ = 0;  // dense level of the per-sim multiplier rows (make_dense)
     FusedShape shape_;
     hipGraphExec_t graph_exec_ = nullptr;
 

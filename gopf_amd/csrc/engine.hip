@@ -864,6 +864,7 @@ PencilUpdateArgs<T> Engine<T>::update_args(cpx<T>* S, cpx<T>* W, double* mean, i
     u.has_src = fp_.has_src ? 1 : 0;
     u.filter = has_filter_ ? filter_.get() : nullptr;
     u.mean = split_mean_ ? mean : nullptr;
+    make_dense(u);
     return u;
 }
 

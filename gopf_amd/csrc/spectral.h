@@ -23,6 +23,7 @@ constexpr int kMaxArr = 4;
 constexpr int kMaxBlockThreads = 512;  // threads per block of every transform kernel
 constexpr int kMaxPoly = 9;     // polynomial degree < kMaxPoly
 constexpr int kMaxKTerms = 8;   // k-space multiplier terms
+constexpr int kDenseDeg = 4;    // dense multiplier polynomials in L up to this degree
 
 // line grid of one pencil segment
 struct Seg {
@@ -77,11 +78,30 @@ struct PencilUpdateArgs {
     double* mean = nullptr;              // FP64 k = 0 mode: the field mean (in/out), or null when the
                                          // k = 0 coefficient is carried by the arrays themselves
     double* egrad_partial = nullptr;     // per-block FP64 sum of w (2 pi |f|)^2 |FFT(c)|^2 or null
-    // batched sims (Seg::sim_src >= 0): per-sim coefficients of denom / lin ([sim][kMaxKTerms],
-    // powers from denom / lin) and mean[sim]
+    // dense form (dense = 2 or 4: every LAP power of D, E and the source is at most that
+    // degree): D(L) = sum_p dd[p] L^p, E(L) = sum_p dl[p] L^p; 0: the term lists above
+    int dense = 0;
+    double dd[kDenseDeg + 1] = {};
+    double dl[kDenseDeg + 1] = {};
+    // batched sims (Seg::sim_src >= 0): per-sim coefficients of denom / lin and mean[sim],
+    // rows of kMaxKTerms: indexed by power when dense, else by term (powers from denom / lin)
     const double* denom_sim = nullptr;
     const double* lin_sim = nullptr;
 };
+
+// dense form of the update polynomials: the smallest of 2 and 4 that covers every LAP
+// power (0 when some power exceeds kDenseDeg); fills dd / dl
+template <typename T>
+inline int make_dense(PencilUpdateArgs<T>& u) {
+    int maxp = u.src_lap;
+    for (int t = 0; t < u.denom.n; ++t) maxp = maxp > u.denom.pow[t] ? maxp : u.denom.pow[t];
+    for (int t = 0; t < u.lin.n; ++t) maxp = maxp > u.lin.pow[t] ? maxp : u.lin.pow[t];
+    if (maxp > kDenseDeg) return u.dense = 0;
+    for (int p = 0; p <= kDenseDeg; ++p) u.dd[p] = u.dl[p] = 0.0;
+    for (int t = 0; t < u.denom.n; ++t) u.dd[u.denom.pow[t]] += u.denom.coef[t];
+    for (int t = 0; t < u.lin.n; ++t) u.dl[u.lin.pow[t]] += u.lin.coef[t];
+    return u.dense = (maxp <= 2) ? 2 : 4;
+}
 
 // x-row R2C of pairs of real arrays into main/nyq half spectra
 template <typename T>

@@ -256,6 +256,76 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencils(PencilArgs<T> a) {
     }
 }
 
+// per-line constants of the update multipliers
+struct MultLine {
+    int t, TT, nv;
+    double invv, fixed2;
+};
+
+template <int DEG>
+__device__ __forceinline__ double horner(const double* c, double L) {
+    double v = c[DEG];
+#pragma unroll
+    for (int p = DEG - 1; p >= 0; --p) v = v * L + c[p];
+    return v;
+}
+
+// k-space multipliers of a thread's elements: s1 = m1 - 1 and s2 = m2 / N with
+// m1 = A (1 + dt E), m2 = A dt L^src_lap, A = sigma / (1 - dt D).  m1 - 1 is formed as
+// (sigma - 1 + dt (sigma E + D)) / (1 - dt D), free of cancellation.  DEG > 0: D and E
+// are dense polynomials in L of degree DEG (kernel arguments, or per-sim rows indexed by
+// power); DEG = 0: term lists with run-time powers.
+template <typename T, int E, int DEG>
+__device__ __forceinline__ void update_multipliers(const PencilUpdateArgs<T>& u, const MultLine& ml,
+                                                   const double* dsim, const double* lsim, T* s1a, T* s2a,
+                                                   double& m1_0, double& m2_0) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int p = ml.t + ml.TT * e;
+        const double fv = wrapk(p, ml.nv) * ml.invv;
+        const double f2 = ml.fixed2 + fv * fv;
+        const double L = -(4.0 * kPi * kPi) * f2;
+        double D = 0.0, Ek = 0.0, Lp = 1.0;
+        if constexpr (DEG > 0) {
+            // (a select of values: a pointer select between global memory and the argument
+            // struct would copy the struct to scratch)
+            D = dsim ? horner<DEG>(dsim, L) : horner<DEG>(u.dd, L);
+            Ek = lsim ? horner<DEG>(lsim, L) : horner<DEG>(u.dl, L);
+#pragma unroll
+            for (int q = 0; q < DEG; ++q)
+                if (q < u.src_lap) Lp *= L;
+        } else {
+            if (dsim) {
+                // per-sim coefficients (global memory), powers shared (kernel argument)
+                for (int t = 0; t < u.denom.n; ++t) {
+                    double v = dsim[t];
+                    for (int q = 0; q < u.denom.pow[t]; ++q) v *= L;
+                    D += v;
+                }
+                for (int t = 0; t < u.lin.n; ++t) {
+                    double v = lsim[t];
+                    for (int q = 0; q < u.lin.pow[t]; ++q) v *= L;
+                    Ek += v;
+                }
+            } else {
+                D = kpoly_eval(u.denom, L);
+                Ek = kpoly_eval(u.lin, L);
+            }
+            for (int q = 0; q < u.src_lap; ++q) Lp *= L;
+        }
+        const double A0 = recip<T>(1.0 - u.dt * D);
+        const double sig = u.filter ? vandeven(u.filter, sqrt(f2) * 2.0 / kPi) : 1.0;
+        const double m1m1 = (sig - 1.0 + u.dt * (sig * Ek + D)) * A0;
+        const double m2 = sig * A0 * u.dt * Lp;
+        if (e == 0) {
+            m1_0 = 1.0 + m1m1;
+            m2_0 = m2;
+        }
+        s1a[e] = (T)m1m1;  // the update adds a correction to S: m1 itself is never rounded
+        s2a[e] = (T)(m2 * u.scale);
+    }
+}
+
 // MON: the energy monitor's partial sums are compiled in (a separate instantiation, so
 // that the common path does not carry their registers)
 template <typename T, int N, bool MON>
@@ -298,42 +368,10 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencil_update(PencilUpdateArgs<T>
                         (vary == 2 || k2 == 0);
     T s1a[E], s2a[E];
     double m1_0 = 1.0, m2_0 = 0.0;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const int p = lm.t + TT * e;
-        const double fv = wrapk(p, nv) * invv;
-        const double f2 = fixed2 + fv * fv;
-        const double L = -(4.0 * kPi * kPi) * f2;
-        double D = 0.0, Ek = 0.0;
-        if (dsim) {
-            // per-sim coefficients (global memory), powers shared (kernel argument)
-            for (int t = 0; t < u.denom.n; ++t) {
-                double v = dsim[t];
-                for (int q = 0; q < u.denom.pow[t]; ++q) v *= L;
-                D += v;
-            }
-            for (int t = 0; t < u.lin.n; ++t) {
-                double v = lsim[t];
-                for (int q = 0; q < u.lin.pow[t]; ++q) v *= L;
-                Ek += v;
-            }
-        } else {
-            D = kpoly_eval(u.denom, L);
-            Ek = kpoly_eval(u.lin, L);
-        }
-        double A = recip<T>(1.0 - u.dt * D);
-        if (u.filter) A *= vandeven(u.filter, sqrt(f2) * 2.0 / kPi);
-        double Lp = 1.0;
-        for (int q = 0; q < u.src_lap; ++q) Lp *= L;
-        const double m1 = A * (1.0 + u.dt * Ek);
-        const double m2 = A * u.dt * Lp;
-        if (e == 0) {
-            m1_0 = m1;
-            m2_0 = m2;
-        }
-        s1a[e] = (T)(m1 - 1.0);  // the update adds a correction to S: m1 itself is never rounded
-        s2a[e] = (T)(m2 * u.scale);
-    }
+    const MultLine ml{lm.t, TT, nv, invv, fixed2};
+    if (u.dense == 2) update_multipliers<T, E, 2>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
+    else if (u.dense == 4) update_multipliers<T, E, 4>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
+    else update_multipliers<T, E, 0>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
     // 2) explicit source: forward transform along this axis, scale by its multiplier
     cpx<T> gs[E];
     double g0 = 0.0;
@@ -534,6 +572,44 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_c2r(RowC2RArgs<T> a) {
     }
 }
 
+// real space of the row pass (row r1 in x, row r1 + 1 in y): g from the shifted
+// coefficients with DEG of them (those beyond npoly are zero), optional real-space
+// output and the monitor's FP64 partial sums
+template <typename T, int E, int DEG, bool MON>
+__device__ __forceinline__ void rows_pointwise(cpx<T>* v, const T* gp, const RowFusedArgs<T>& a, long r1, int n,
+                                               int t, int TT, bool v1, bool v2, double mu, KSum& sb, KSum& sm) {
+    const T muT = (T)mu;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const T d1 = v[e].x, d2 = v[e].y;
+        T g1 = gp[DEG - 1], g2 = gp[DEG - 1];
+#pragma unroll
+        for (int q = DEG - 2; q >= 0; --q) {
+            g1 = g1 * d1 + gp[q];
+            g2 = g2 * d2 + gp[q];
+        }
+        v[e] = mk<T>(g1, g2);
+        const long off = r1 * n + t + TT * e;
+        if (a.c_out) {
+            if (v1) a.c_out[off] = d1 + muT;
+            if (v2) a.c_out[off + n] = d2 + muT;
+        }
+        if (MON) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (h == 0 ? !v1 : !v2) continue;
+                const double cd = (double)(h == 0 ? d1 : d2) + mu;
+                double f = 0.0;
+#pragma unroll
+                for (int q = kMaxPoly - 1; q >= 0; --q)
+                    if (q < a.nbulk) f = f * cd + a.bulk[q];
+                sb.add(f);
+                sm.add(cd);
+            }
+        }
+    }
+}
+
 // V: transform schedule (0 plain, 1 low-register); MON: monitor partial sums compiled in
 template <typename T, int N, int V, bool MON>
 __global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
@@ -548,7 +624,6 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
     const bool v1 = r1 < a.rows, v2 = r1 + 1 < a.rows;
     const long sim = (a.rows_per_sim > 0 && v1) ? r1 / a.rows_per_sim : -1;
     const double mu = a.mean ? a.mean[sim >= 0 ? sim : 0] : 0.0;
-    const T muT = (T)mu;
     // g about the mean: g(mu + dc) = sum_q gp[q] dc^q, the Taylor shift of the coefficients
     // formed in FP64 (repeated synthetic division), so that the bulk evaluation rounds
     // relative to the fluctuation dc rather than to c
@@ -570,38 +645,9 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
     cpx<T> v[E];
     load_herm_pair<T, E, (N > 0 ? LineShape<N, V>::T : 0)>(v, a.s_main, a.s_nyq, r1, n, lm.t, TT, v1, v2);
     line_fft<T, N, +1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
-    // real space (row r1 in x, row r1 + 1 in y): g, optional output and FP64 partial sums
     KSum sb, sm;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const T d1 = v[e].x, d2 = v[e].y;
-        T g1 = T(0), g2 = T(0);
-#pragma unroll
-        for (int q = kMaxPoly - 1; q >= 0; --q)
-            if (q < a.npoly) {
-                g1 = g1 * d1 + gp[q];
-                g2 = g2 * d2 + gp[q];
-            }
-        v[e] = mk<T>(g1, g2);
-        const long off = r1 * n + lm.t + TT * e;
-        if (a.c_out) {
-            if (v1) a.c_out[off] = d1 + muT;
-            if (v2) a.c_out[off + n] = d2 + muT;
-        }
-        if (MON) {
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                if (h == 0 ? !v1 : !v2) continue;
-                const double cd = (double)(h == 0 ? d1 : d2) + mu;
-                double f = 0.0;
-#pragma unroll
-                for (int q = kMaxPoly - 1; q >= 0; --q)
-                    if (q < a.nbulk) f = f * cd + a.bulk[q];
-                sb.add(f);
-                sm.add(cd);
-            }
-        }
-    }
+    if (a.npoly <= 4) rows_pointwise<T, E, 4, MON>(v, gp, a, r1, n, lm.t, TT, v1, v2, mu, sb, sm);
+    else rows_pointwise<T, E, kMaxPoly, MON>(v, gp, a, r1, n, lm.t, TT, v1, v2, mu, sb, sm);
     if (MON) {
         double* red = reinterpret_cast<double*>(lds + lds_elems(n, a.TW));
         const double b = block_sum(sb.get(), red);
@@ -740,13 +786,14 @@ hipError_t launch_rows_fused(const RowFusedArgs<T>& a, hipStream_t s) {
     if (a.bulk_partial) smem += (size_t)thr * sizeof(double);
     return dispatch_size(a.nx, [&](auto NC) -> hipError_t {
         constexpr int NN = decltype(NC)::value;
-        constexpr int V0 = plain_variant<NN>();
-        if (a.variant || V0) {
-            if (a.bulk_partial) return launch_k(k_rows_fused<T, NN, 1, true>, dim3(blocks), dim3(thr), smem, s, a);
-            return launch_k(k_rows_fused<T, NN, 1, false>, dim3(blocks), dim3(thr), smem, s, a);
+        if constexpr (plain_variant<NN>() == 0) {
+            if (!a.variant) {
+                if (a.bulk_partial) return launch_k(k_rows_fused<T, NN, 0, true>, dim3(blocks), dim3(thr), smem, s, a);
+                return launch_k(k_rows_fused<T, NN, 0, false>, dim3(blocks), dim3(thr), smem, s, a);
+            }
         }
-        if (a.bulk_partial) return launch_k(k_rows_fused<T, NN, 0, true>, dim3(blocks), dim3(thr), smem, s, a);
-        return launch_k(k_rows_fused<T, NN, 0, false>, dim3(blocks), dim3(thr), smem, s, a);
+        if (a.bulk_partial) return launch_k(k_rows_fused<T, NN, 1, true>, dim3(blocks), dim3(thr), smem, s, a);
+        return launch_k(k_rows_fused<T, NN, 1, false>, dim3(blocks), dim3(thr), smem, s, a);
     });
 }
 
