@@ -45,6 +45,19 @@ def pass_bytes(name: str, nx: int, ny: int, nz: int, s: int) -> float:
     return {"rows_fused": 2, "pencils_fwd": 2, "pencil_update": 4, "pencils_inv": 2}[name] * arr
 
 
+def pmc_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (profiles/pmc_traffic.json: FETCH_SIZE scaled by the copy-kernel calibration, plus
+    WRITE_SIZE), or None when the summary does not cover this workload."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return float(d["workloads"][workload]["kernels"][kernel]["hbm_bytes"])
+    except (OSError, KeyError, ValueError, TypeError):
+        return None
+
+
 def build_solver(G, W, n, dims, c0, precision, dt):
     m = G.Model()
     m.AddField("conc", c0)
@@ -89,6 +102,243 @@ def cpu_baseline(n_side: int, dt: float, c0: np.ndarray):
                       f"(3 forward + 1 inverse complex128 3D FFT, cmplx.Pow derived fields), {el:.1f} s"}
 
 
+# ---------------------------------------------------------------------------
+# the other workloads (each prints its own JSON line with --config)
+# ---------------------------------------------------------------------------
+
+def _oracle_ch(n_dims, c0, dt, scal):
+    from oracle import engine as E
+    from oracle import parser as P
+    from gopf_amd import workloads as W
+    E.build()
+    m = P.Model()
+    m.add_field("conc", int(np.prod(n_dims)))
+    for k, v in scal.items():
+        m.add_scalar(k, v)
+    m.add_equation(W.CH_EQUATION)
+    m.init()
+    return E.OracleSolver(m, n_dims, {"conc": c0}, dt)
+
+
+def cpu_baseline_ch2d(N: int, steps: int, dt: float, c0, scal, what: str):
+    """FP64 oracle on one core: `steps` Euler steps of an N^2 CH sim."""
+    o = _oracle_ch([N, N], c0, dt, scal)
+    t0 = time.perf_counter()
+    o.stepper_propagate(steps)
+    el = time.perf_counter() - t0
+    return {"value": N * N * steps / el, "unit": "voxel-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{what}: {steps} Euler steps of one {N}^2 CH sim by the FP64 C oracle, {el:.1f} s"}
+
+
+def run_c1(G, W, args):
+    """128^2 CH, FP64, seed 0, dt 0.1, M = kappa = 1, 1000 steps with the free-energy series."""
+    N, dt, steps = 128, 0.1, args.steps
+    dims = [N, N]
+    c0 = W.ch_initial(0, N * N)
+    m = G.Model()
+    m.AddField("conc", c0)
+    for k, v in W.ch_scalars(1.0, 1.0).items():
+        m.AddScalar(k, v)
+    m.AddEquation(W.CH_EQUATION)
+    m.Init()
+    s = G.Solver(m, dims, dt, precision="fp64", autotune=True)
+    s.StepperPropagate(args.warmup)   # tunes the launch configuration
+    s.SetField(0, c0)
+    e0, m0 = s.Energy(0, W.CH_BULK_POLY, 1.0)
+    s.MonitorEnergy(W.CH_BULK_POLY, 1.0)
+    el = time_steps(s, steps)
+    e, mass = s.EnergySeries()
+    series = np.concatenate([[e0], e])
+    out = {"metric": "steps/s, 128^2 CH FP64 with the per-step free-energy series", "value": steps / el,
+           "unit": "steps/s", "ms_per_step": 1e3 * el / steps, "voxel_steps_per_s": N * N * steps / el,
+           "dtype": "f64", "config": {"workload": "cahn_hilliard_2d_128", "grid": dims, "dt": dt, "steps": steps,
+                                      "init": "0.5 + U(-0.05, 0.05), SplitMix64 seed 0"},
+           "energy": {"values": int(series.size), "first": float(series[0]), "last": float(series[-1]),
+                      "max_increase": float(np.max(np.diff(series))), "mass_drift": float(abs(mass[-1] - m0) / m0)}}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_ch2d(N, min(steps, 200), dt, c0, W.ch_scalars(1.0, 1.0), "sample")
+    return out
+
+
+def c3_sim_inputs(W, s: int, n: int):
+    """Sim s: SplitMix64(1000 + s); (M, kappa) = exp(U[ln 0.1, ln 10]) from the first two
+    draws, the initial field 0.5 + U(-0.05, 0.05) from the next n."""
+    u = W.splitmix64_uniform(1000 + s, n + 2)
+    lo, hi = np.log(0.1), np.log(10.0)
+    M = float(np.exp(lo + (hi - lo) * u[0]))
+    kappa = float(np.exp(lo + (hi - lo) * u[1]))
+    return M, kappa, 0.5 + 0.1 * (u[2:] - 0.5)
+
+
+def run_c3(G, W, args):
+    """384 CH sims: 128 of 128^2, 192 of 256^2, 64 of 512^2, FP32-bulk, dt 0.1, 500 steps."""
+    classes = [((128, 128), 128), ((256, 256), 192), ((512, 512), 64)]
+    dt, steps = 0.1, args.steps
+    m = G.Model()
+    m.AddField("conc", n=16)
+    for k, v in W.ch_scalars().items():
+        m.AddScalar(k, v)
+    m.AddEquation(W.CH_EQUATION)
+    m.Init()
+    b = G.Batch(m, classes, dt, precision="fp32", autotune=True)
+    vox = 0
+    for s in range(len(b)):
+        n = int(np.prod(b.shapes[s]))
+        M, kappa, c0 = c3_sim_inputs(W, s, n)
+        for k, v in W.ch_scalars(M, kappa).items():
+            b.SetScalar(s, k, v)
+        b.SetField(s, c0)
+        vox += n
+    b.Propagate(args.warmup)          # compiles, tunes and captures the step graph
+    b.Synchronize()
+    t0 = time.perf_counter()
+    b.Propagate(steps)
+    b.Synchronize()
+    el = time.perf_counter() - t0
+    out = {"metric": "voxel-steps/s, 384 batched 2D CH sims of uneven sizes, FP32-bulk", "value": vox * steps / el,
+           "unit": "voxel-steps/s", "ms_per_step": 1e3 * el / steps, "dtype": "f32",
+           "config": {"workload": "batched_ch_2d_384", "classes": [[list(c), k] for c, k in classes],
+                      "voxels_per_step": vox, "dt": dt, "steps": steps,
+                      "params": "(M, kappa) = exp(U[ln 0.1, ln 10]), SplitMix64 seed 1000 + sim"},
+           "layout": b.Tuning()}
+    if not args.no_cpu:
+        M, kappa, c0 = c3_sim_inputs(W, 128, 256 * 256)
+        out["cpu_baseline"] = cpu_baseline_ch2d(256, 20, dt, c0, W.ch_scalars(M, kappa),
+                                                "one 256^2 sim of the sweep (sim 128)")
+    return out
+
+
+def precipitate_model(G, W, conc0, phase0):
+    m = G.Model()
+    m.AddField("conc", conc0)
+    m.AddField("phase", phase0)
+    for k, v in W.PRECIPITATE_SCALARS.items():
+        m.AddScalar(k, v)
+    for name, poly in W.precipitate_polynomials().items():
+        m.RegisterPolynomial(name, poly)
+    m.RegisterExplicitTerm("LIN_ELAST", G.HomogeneousModulusLinElast("phase", G.CubicMaterial(110.0, 60.0, 30.0),
+                                                                     0.01 * np.eye(3)))
+    for e in W.PRECIPITATE_EQUATIONS:
+        m.AddEquation(e)
+    m.Init()
+    return m
+
+
+def cpu_baseline_c4(W, N: int, steps: int):
+    """FP64 oracle on one core: the precipitate model at N^3, RK4 with the full 14-FFT
+    elastic evaluation of the reference."""
+    from oracle import engine as E
+    from oracle import parser as P
+    E.build()
+    conc0, phase0 = W.precipitate_initial(N, 20.0 * N / 192.0, seed=4)
+    m = P.Model()
+    n = N ** 3
+    m.add_field("conc", n)
+    m.add_field("phase", n)
+    for k, v in W.PRECIPITATE_SCALARS.items():
+        m.add_scalar(k, v)
+    for name, poly in W.precipitate_polynomials().items():
+        m.register_polynomial(name, poly)
+    import gopf_amd as G
+    m.register_explicit_term("LIN_ELAST", P.KIND_ELAST, field="phase", C=G.CubicMaterial(110.0, 60.0, 30.0),
+                             misfit=0.01 * np.eye(3))
+    for e in W.PRECIPITATE_EQUATIONS:
+        m.add_equation(e)
+    m.init()
+    o = E.OracleSolver(m, [N, N, N], {"conc": conc0, "phase": phase0}, 0.01, "rk4")
+    t0 = time.perf_counter()
+    o.solver_propagate(steps)
+    el = time.perf_counter() - t0
+    return {"value": n * steps / el, "unit": "voxel-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} RK4 steps of the precipitate model at {N}^3 (sphere radius scaled to "
+                      f"{20.0 * N / 192.0:.1f}) by the FP64 C oracle, {el:.1f} s; voxel-steps/s carries over "
+                      f"to 192^3 up to the FFT's log factor"}
+
+
+def run_c4(G, W, args):
+    """Two-field precipitate with the Khachaturyan elastic term, 192^3, RK4, dt 0.01, 500 steps."""
+    N, dt, steps = 192, 0.01, args.steps
+    dims = [N, N, N]
+    conc0, phase0 = W.precipitate_initial(N, 20.0, seed=4)
+    res = {}
+    fields = {}
+    for prec in ("fp64", "fp32"):
+        s = G.Solver(precipitate_model(G, W, conc0, phase0), dims, dt, stepper="rk4", precision=prec)
+        s.Propagate(args.warmup)
+        s.SetField("conc", conc0)
+        s.SetField("phase", phase0)
+        el = 0.0
+        s.Synchronize()
+        t0 = time.perf_counter()
+        s.Propagate(steps)
+        s.Synchronize()
+        el = time.perf_counter() - t0
+        res[prec] = {"value": N ** 3 * steps / el, "ms_per_step": 1e3 * el / steps}
+        fields[prec] = (s.Field("conc").real, s.Field("phase").real)
+        print(f"# c4 {prec}: {1e3 * el / steps:.3f} ms/step", file=sys.stderr, flush=True)
+        del s
+    err = {f: float(np.linalg.norm(fields["fp32"][i] - fields["fp64"][i]) / np.linalg.norm(fields["fp64"][i]))
+           for i, f in enumerate(("conc", "phase"))}
+    out = {"metric": "voxel-steps/s, 192^3 two-field elastic RK4", "value": res["fp32"]["value"],
+           "unit": "voxel-steps/s", "ms_per_step": res["fp32"]["ms_per_step"], "dtype": "f32",
+           "fp64": res["fp64"], "speedup_fp32_vs_fp64": res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"],
+           "err_vs_fp64_rel_l2": err,
+           "config": {"workload": "precipitate_elastic_3d_192", "grid": dims, "dt": dt, "steps": steps,
+                      "stepper": "RK4", "elastic": "cubic C11=110 C12=60 C44=30, misfit 0.01 I",
+                      "init": "sphere r=20 + U(-0.01, 0.01), seed 4; lagged elastic field starts at zero"}}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_c4(W, 32, 2)
+    return out
+
+
+def run_c5(G, W, args):
+    """Adjoint: 512^2 CH, 2000 steps, dt 0.1, checkpoints every 100 steps, target from a
+    forward run at kappa = 1.3 (seed 6); gradients with respect to c0 and kappa."""
+    N, dt, steps = 512, 0.1, args.steps
+    dims = [N, N]
+    n = N * N
+    c0 = W.ch_initial(5, n)
+
+    def solver(prec, scal, init):
+        m = G.Model()
+        m.AddField("conc", init)
+        for k, v in scal.items():
+            m.AddScalar(k, v)
+        m.AddEquation(W.CH_EQUATION)
+        m.Init()
+        return G.Solver(m, dims, dt, precision=prec, autotune=True)
+
+    ts = solver("fp64", W.ch_scalars(1.0, 1.3), W.ch_initial(6, n))
+    ts.StepperPropagate(steps)
+    tgt = ts.Field(0).real
+    del ts
+    res = {}
+    grads = {}
+    for prec in ("fp64", "fp32"):
+        s = solver(prec, W.ch_scalars(1.0, 1.0), c0)
+        s.StepperPropagate(args.warmup)
+        s.SetField(0, c0)
+        tf = time_steps(s, steps)
+        s.SetField(0, c0)
+        s.Synchronize()
+        t0 = time.perf_counter()
+        J, g0, gs = s.Adjoint(steps, tgt, ["mk"], checkpoint_every=100)
+        ta = time.perf_counter() - t0
+        res[prec] = {"adjoint_s": ta, "forward_s": tf, "adjoint_over_forward": ta / tf, "loss": J,
+                     "dJ_dkappa": gs["mk"], "value": n * steps / ta}
+        grads[prec] = g0
+        print(f"# c5 {prec}: forward {tf:.3f} s, adjoint {ta:.3f} s", file=sys.stderr, flush=True)
+        del s
+    rel = float(np.linalg.norm(grads["fp32"] - grads["fp64"]) / np.linalg.norm(grads["fp64"]))
+    out = {"metric": "voxel-steps/s of the adjoint (forward + checkpointed recompute + reverse), 512^2 CH",
+           "value": res["fp64"]["value"], "unit": "voxel-steps/s", "dtype": "f64", "fp64": res["fp64"],
+           "fp32": res["fp32"], "fp32_grad_rel_l2_vs_fp64": rel,
+           "config": {"workload": "adjoint_ch_2d_512", "grid": dims, "dt": dt, "steps": steps,
+                      "checkpoint_every": 100, "loss": "sum (c_T - c_target)^2",
+                      "target": "forward run at kappa = 1.3 from seed 6", "init": "seed 5"}}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +349,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle timing")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--err-steps", type=int, default=1000, help="steps of the FP32-vs-FP64 accuracy run")
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+                    help="c2 (default): 256^3 CH headline; c1 128^2 CH FP64 with energy series; c3 batched "
+                         "sweep; c4 192^3 elastic RK4; c5 512^2 adjoint")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the multi-GPU slab decomposition is "
@@ -106,6 +359,16 @@ def main() -> int:
         return 2
     import gopf_amd as G
     from gopf_amd import workloads as W
+
+    if args.config != "c2":
+        defaults = {"c1": 1000, "c3": 500, "c4": 500, "c5": 2000}
+        if "--steps" not in sys.argv:
+            args.steps = defaults[args.config]
+        out = {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.config](G, W, args)
+        out.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+                    "data": "synthetic", "vs_baseline": None})
+        print(json.dumps(out))
+        return 0
 
     N = args.n
     dims = [N, N, N]
@@ -164,7 +427,8 @@ def main() -> int:
     achieved = algo / avg_s / 1e9
     step_bytes = sum(pass_bytes(k, N, N, N, 4) for k in passes)
     roof = {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": algo,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic(f"cahn_hilliard_3d_{N}_fp32", name),
+            "bytes_per_launch": algo,
             "avg_launch_ms": ms / cnt, "lines_per_block": tuning,
             "passes_ms_per_launch": {k: v[0] / v[1] for k, v in passes.items()},
             "step_effective_GBs": step_bytes * args.steps / el32 / 1e9,

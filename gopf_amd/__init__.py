@@ -98,6 +98,7 @@ def _declare(L) -> None:
         "gpf_solver_pass_times": (C.c_int, [_vp, C.c_int, _dp, _lp, C.c_char_p, C.c_int]),
         "gpf_solver_stream": (_vp, [_vp]),
         "gpf_solver_tuning": (C.c_int, [_vp, _ip, C.c_int]),
+        "gpf_solver_set_filter": (C.c_int, [_vp, _dp, C.c_long]),
         "gpf_solver_adjoint": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_long, _dp, _dp, C.c_int,
                                          C.POINTER(C.c_char_p), _dp]),
         "gpf_device_copy_bandwidth": (C.c_int, [C.c_long, C.c_int, C.c_int, _dp]),
@@ -410,13 +411,23 @@ class Solver:
         return out
 
     def Tuning(self) -> Dict[str, int]:
-        v = (C.c_int * 6)()
-        lib().gpf_solver_tuning(self._h, v, 6)
+        v = (C.c_int * 10)()
+        lib().gpf_solver_tuning(self._h, v, 10)
         return {"rows_fused": v[0], "pencils_fwd": v[1], "pencil_update": v[2], "pencils_inv": v[3],
-                "update_axis": "zy"[2 - v[4]] if v[4] in (1, 2) else "y", "rows_schedule": v[5]}
+                "update_axis": "zy"[2 - v[4]] if v[4] in (1, 2) else "y", "rows_schedule": v[5],
+                "pencils_fwd_lines_per_thread": v[6], "pencils_inv_lines_per_thread": v[7],
+                "pencils_fwd_schedule": v[8], "pencils_inv_schedule": v[9]}
 
     def Stream(self) -> int:
         return int(lib().gpf_solver_stream(self._h) or 0)
+
+    def SetFilter(self, filt) -> None:
+        """TimeStepper.SetFilter (pf/euler.go:62-64) with a Vandeven filter (its table), or None."""
+        if filt is None:
+            _check(lib().gpf_solver_set_filter(self._h, None, 0))
+            return
+        tab = np.ascontiguousarray(np.asarray(filt.Data, dtype=np.float64))
+        _check(lib().gpf_solver_set_filter(self._h, _dptr(tab), tab.size))
 
     def Adjoint(self, nsteps: int, target, scalars: Sequence[str] = (), checkpoint_every: int = 100
                 ) -> Tuple[float, np.ndarray, Dict[str, float]]:

@@ -336,6 +336,7 @@ public:
     virtual hipStream_t stream() const = 0;
     virtual long size() const = 0;
     virtual int tuning(int* tws, int maxn) = 0;
+    virtual void set_filter(const double* table) = 0;
     virtual void adjoint(long nsteps, long every, const double* target, double* loss, double* grad_field,
                          int nscal, const char* const* names, double* grad_scalars) = 0;
 };
@@ -359,6 +360,7 @@ public:
     hipStream_t stream() const override { return st_; }
     long size() const override { return g_.n; }
     int tuning(int* tws, int maxn) override;
+    void set_filter(const double* table) override;
     void adjoint(long nsteps, long every, const double* target, double* loss, double* grad_field, int nscal,
                  const char* const* names, double* grad_scalars) override;
 
@@ -380,11 +382,13 @@ private:
     RowFusedArgs<T> rows_args(cpx<T>* W, double* mean, int tw, int var);
     PencilUpdateArgs<T> update_args(cpx<T>* S, cpx<T>* W, double* mean, int tw);
     void prime();
-    void pencils_tw(const std::vector<cpx<T>*>& dst, int axis, int dir, int tw);
+    void pencils_tw(const std::vector<cpx<T>*>& dst, int axis, int dir, int tw, int lpt = 1, int var = 0);
     void autotune();
     bool autotune_ = false, tuned_ = false;
     int tw_rows_ = 0, tw_pf_ = 0, tw_pu_ = 0, tw_pi_ = 0;
     int rows_var_ = 0;  // transform schedule of the row pass (0 plain, 1 low-register)
+    int lpt_pf_ = 1, lpt_pi_ = 1;  // lines per thread of the forward / inverse pencil passes
+    int var_pf_ = 0, var_pi_ = 0;  // transform schedules of the forward / inverse pencil passes
     // 3D: axis of the fused forward-update-inverse pass (2 = z, 1 = y); the other
     // of y / z is transformed forward before and inverse after it
     int upd_axis_ = 2;
@@ -460,9 +464,7 @@ Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gp
     if (o.filter_order > 0) {
         std::vector<double> tab(1000);
         vandeven_table(o.filter_order, tab.data());
-        filter_.alloc(1000);
-        HIPCHK(hipMemcpy(filter_.get(), tab.data(), 1000 * sizeof(double), hipMemcpyHostToDevice));
-        has_filter_ = true;
+        set_filter(tab.data());
     }
     if (o.fused) fp_ = compile_fused(m, stepper_);
     if (fp_.ok) {
@@ -875,7 +877,7 @@ void Engine<T>::prime() {
 }
 
 template <typename T>
-void Engine<T>::pencils_tw(const std::vector<cpx<T>*>& dst, int axis, int dir, int tw) {
+void Engine<T>::pencils_tw(const std::vector<cpx<T>*>& dst, int axis, int dir, int tw, int lpt, int var) {
     const int n = (axis == 1) ? g_.ny : g_.nz;
     PencilArgs<T> a;
     a.tw = this->tw(n);
@@ -883,7 +885,9 @@ void Engine<T>::pencils_tw(const std::vector<cpx<T>*>& dst, int axis, int dir, i
     a.seg[0] = (axis == 1) ? seg_y_main(g_) : seg_z_main(g_);
     a.seg[1] = (axis == 1) ? seg_y_nyq(g_) : seg_z_nyq(g_);
     a.nseg = 2;
-    a.TW = tw > 0 ? tw : choose_tw(n, a.narr, sizeof(cpx<T>), 8);
+    a.TW = tw > 0 ? tw : choose_tw(n, a.narr, sizeof(cpx<T>), 8, var);
+    a.lpt = lpt;
+    a.variant = var;
     for (int q = 0; q < a.narr; ++q) {
         a.arr[q][0] = dst[q];
         a.arr[q][1] = dst[q] + g_.nmain;
@@ -946,19 +950,32 @@ void Engine<T>::autotune() {
         int pf = 0, pi = 0, pu = 0;
         float total = 0.f;
         const int no = axis_len(other_axis()), nu = axis_len(upd_axis());
+        int lpi = 1, lpf = 1, vpi = 0, vpf = 0;
         if (g_.ndim == 3) {
-            pi = best_of(no, [&](int tw) { pencils_tw({Wb.get()}, other_axis(), +1, tw); });
-            total += time_it([&] { pencils_tw({Wb.get()}, other_axis(), +1, pi); });
-            if (fp_.has_src) {
-                pf = best_of(no, [&](int tw) { pencils_tw({Wb.get()}, other_axis(), -1, tw); });
-                total += time_it([&] { pencils_tw({Wb.get()}, other_axis(), -1, pf); });
-            }
+            // schedule, lines per thread and lines per block of the plain pencil passes
+            auto best_pencil = [&](int dir, int& tw_out, int& lpt_out, int& var_out) {
+                float bt = 1e30f;
+                for (int var = 0; var < 2; ++var)
+                    for (int lpt = 1; lpt <= 2; ++lpt)
+                        for (int tw : cand) {
+                            if (!tw_fits(no, tw, sizeof(cpx<T>), var)) continue;
+                            const float t = time_it([&] { pencils_tw({Wb.get()}, other_axis(), dir, tw, lpt, var); });
+                            if (t < bt) { bt = t; tw_out = tw; lpt_out = lpt; var_out = var; }
+                        }
+                return bt;
+            };
+            total += best_pencil(+1, pi, lpi, vpi);
+            if (fp_.has_src) total += best_pencil(-1, pf, lpf, vpf);
         }
         pu = best_of(nu, [&](int tw) { HIPCHK(launch_pencil_update<T>(update_args(Sb.get(), Wb.get(), mu.get(), tw), nu, st_)); }, 1);
         total += time_it([&] { HIPCHK(launch_pencil_update<T>(update_args(Sb.get(), Wb.get(), mu.get(), pu), nu, st_)); });
         if (total < best_total) {
             best_total = total;
             best[0] = pf; best[1] = pu; best[2] = pi; best[3] = upd_axis_;
+            lpt_pf_ = lpf;
+            lpt_pi_ = lpi;
+            var_pf_ = vpf;
+            var_pi_ = vpi;
         }
     }
     tw_pf_ = best[0];
@@ -972,11 +989,26 @@ void Engine<T>::autotune() {
     prime();  // the work array follows the chosen update axis
 }
 
+// TimeStepper.SetFilter (pf/euler.go:62-64, pf/rk4.go:138-140): a modal filter given as its
+// 1000-point table, evaluated on the device as ModalFilter.Eval of a pf.Vandeven
+// (pf/vandeven.go:31-40); a null table removes the filter
+template <typename T>
+void Engine<T>::set_filter(const double* table) {
+    HIPCHK(hipStreamSynchronize(st_));
+    if (!table) {
+        has_filter_ = false;
+        return;
+    }
+    if (!filter_.get()) filter_.alloc(1000);
+    HIPCHK(hipMemcpy(filter_.get(), table, 1000 * sizeof(double), hipMemcpyHostToDevice));
+    has_filter_ = true;
+}
+
 template <typename T>
 int Engine<T>::tuning(int* tws, int maxn) {
-    const int v[6] = {tw_rows_, tw_pf_, tw_pu_, tw_pi_, upd_axis(), rows_var_};
-    for (int i = 0; i < 6 && i < maxn; ++i) tws[i] = v[i];
-    return 6;
+    const int v[10] = {tw_rows_, tw_pf_, tw_pu_, tw_pi_, upd_axis(), rows_var_, lpt_pf_, lpt_pi_, var_pf_, var_pi_};
+    for (int i = 0; i < 10 && i < maxn; ++i) tws[i] = v[i];
+    return 10;
 }
 
 template <typename T>
@@ -1011,7 +1043,7 @@ void Engine<T>::fused_step() {
     if (fp_.has_src || mon_) {
         if (g_.ndim == 3) {
             tbegin(3);
-            pencils_tw({W_.get()}, other_axis(), +1, tw_pi_);
+            pencils_tw({W_.get()}, other_axis(), +1, tw_pi_, lpt_pi_, var_pi_);
             tend(3);
         }
         tbegin(0);
@@ -1019,7 +1051,7 @@ void Engine<T>::fused_step() {
         tend(0);
         if (g_.ndim == 3 && fp_.has_src) {
             tbegin(1);
-            pencils_tw({W_.get()}, other_axis(), -1, tw_pf_);
+            pencils_tw({W_.get()}, other_axis(), -1, tw_pf_, lpt_pf_, var_pf_);
             tend(1);
         }
     }
@@ -1910,6 +1942,15 @@ int gpf_solver_tuning(gpf_solver* s, int* tws, int maxn) {
 }
 
 void* gpf_solver_stream(gpf_solver* s) { return (s && s->eng) ? (void*)s->eng->stream() : nullptr; }
+
+int gpf_solver_set_filter(gpf_solver* s, const double* table, long n) {
+    return guarded([&] {
+        if (!s) throw Error(GPF_EINVAL, "bad argument");
+        if (table && n != 1000) throw Error(GPF_EINVAL, "a modal filter table has 1000 entries");
+        set_device(s->device);
+        s->eng->set_filter(table);
+    });
+}
 
 int gpf_batch_create(gpf_model* m, int nclasses, const int* shapes, const int* counts, double dt,
                      const gpf_options* o, gpf_batch** out) {

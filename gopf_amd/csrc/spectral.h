@@ -58,6 +58,8 @@ struct PencilArgs {
     int tiles0 = 0;                      // tiles of segment 0 (segment 1 follows)
     int TW = 16;                         // lines per block (power of two)
     int n = 0;                           // pencil length (runtime sizes)
+    int lpt = 1;                         // lines per thread (k_pencils: 1 or 2; a tile is TW * lpt lines)
+    int variant = 0;                     // transform schedule of k_pencils: 0 plain, 1 low-register
 };
 
 // fused update pass of the single-field fast path, along the last axis of the step:

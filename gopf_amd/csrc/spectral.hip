@@ -167,11 +167,12 @@ __device__ __forceinline__ TileInfo decode_tile(const PencilArgs<T>& a) {
     ti.kfix1 = s1 ? a.seg[1].kfix[1] : a.seg[0].kfix[1];
     ti.kfix2 = s1 ? a.seg[1].kfix[2] : a.seg[0].kfix[2];
     ti.sim_src = s1 ? a.seg[1].sim_src : a.seg[0].sim_src;
-    const int tpa = (nb + a.TW - 1) / a.TW;
+    const int tl = a.TW * a.lpt;  // lines per tile
+    const int tpa = (nb + tl - 1) / tl;
     ti.ia = tile / tpa;
-    ti.b0 = (tile - ti.ia * tpa) * a.TW;
+    ti.b0 = (tile - ti.ia * tpa) * tl;
     ti.base = (long)ti.ia * sa + (long)ti.b0 * ti.sb;
-    ti.nvalid = min(a.TW, nb - ti.b0);
+    ti.nvalid = min(tl, nb - ti.b0);
     return ti;
 }
 
@@ -230,28 +231,93 @@ __device__ __forceinline__ void store_line(const cpx<T>* v, cpx<T>* __restrict__
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-template <typename T, int N, int DIR>
+// LPT = 2: thread-line l transforms the adjacent lines 2l and 2l + 1 of a tile of 2 TW
+// lines, both loaded before either is transformed.  When the lines are adjacent in memory
+// (line stride 1) and the pair is 16-byte aligned, FP32 pairs move as one 16-byte vector
+// per element, so eight lanes cover a whole 128-byte cache line.
+template <typename T, int E>
+__device__ __forceinline__ void load_line_pair(cpx<T>* v, cpx<T>* w, const cpx<T>* __restrict__ src, long base, long es,
+                                               int t, int TT, bool vec, bool valid0, bool valid1) {
+    if constexpr (sizeof(T) == 4) {
+        if (vec) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const float4 x = *reinterpret_cast<const float4*>(src + base + (long)(t + TT * e) * es);
+                v[e] = mk<T>(x.x, x.y);
+                w[e] = mk<T>(x.z, x.w);
+            }
+            return;
+        }
+    }
+    load_line<T, E>(v, src, base, es, t, TT, valid0);
+    load_line<T, E>(w, src, base + 1, es, t, TT, valid1);
+}
+
+template <typename T, int E>
+__device__ __forceinline__ void store_line_pair(const cpx<T>* v, const cpx<T>* w, cpx<T>* __restrict__ dst, long base,
+                                                long es, int t, int TT, bool vec, bool valid0, bool valid1) {
+    if constexpr (sizeof(T) == 4) {
+        if (vec) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                float4 x;
+                x.x = v[e].x;
+                x.y = v[e].y;
+                x.z = w[e].x;
+                x.w = w[e].y;
+                *reinterpret_cast<float4*>(dst + base + (long)(t + TT * e) * es) = x;
+            }
+            return;
+        }
+    }
+    store_line<T, E>(v, dst, base, es, t, TT, valid0);
+    store_line<T, E>(w, dst, base + 1, es, t, TT, valid1);
+}
+
+template <typename T, int N, int DIR, int V, int LPT>
 __global__ void __launch_bounds__(kMaxBlock) k_pencils(PencilArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
-    constexpr int V = plain_variant<N>();
     constexpr int E = LineShape<N, V>::E;
     const int n = (N > 0) ? N : a.n;
     const int TT = (N > 0) ? LineShape<N, V>::T : n;
     const TileInfo ti = decode_tile(a);
     const LineMap lm = line_map(ti.es == 1, a.TW, TT, n);
-    const bool valid = lm.l < ti.nvalid;
-    const long base = ti.base + (long)lm.l * ti.sb;
 #pragma unroll
     for (int q = 0; q < kMaxArr; ++q) {
         if (q < a.narr) {
             cpx<T>* out = pick(a.arr[q][0], a.arr[q][1], ti.s);
             const cpx<T>* s = pick(a.src[q][0], a.src[q][1], ti.s);
             const cpx<T>* in = s ? s : out;
-            cpx<T> v[E];
-            load_line<T, E>(v, in, base, ti.es, lm.t, TT, valid);
-            line_fft<T, N, DIR, V>(v, lds + lm.off, lm.es, n, lm.t, a.tw);
-            store_line<T, E>(v, out, base, ti.es, lm.t, TT, valid);
+            if constexpr (LPT == 1) {
+                const bool valid0 = lm.l < ti.nvalid;
+                const long base0 = ti.base + (long)lm.l * ti.sb;
+                cpx<T> v[E];
+                load_line<T, E>(v, in, base0, ti.es, lm.t, TT, valid0);
+                line_fft<T, N, DIR, V>(v, lds + lm.off, lm.es, n, lm.t, a.tw);
+                store_line<T, E>(v, out, base0, ti.es, lm.t, TT, valid0);
+            } else {
+                const int l0 = 2 * lm.l;
+                const bool valid0 = l0 < ti.nvalid, valid1 = l0 + 1 < ti.nvalid;
+                const long base0 = ti.base + (long)l0 * ti.sb;
+                const bool vec = valid1 && ti.sb == 1 && (ti.es & 1) == 0 &&
+                                 ((reinterpret_cast<size_t>(in + base0) | reinterpret_cast<size_t>(out + base0)) & 15) == 0;
+                cpx<T> v[E], w[E];
+                if (ti.sb == 1) {
+                    load_line_pair<T, E>(v, w, in, base0, ti.es, lm.t, TT, vec, valid0, valid1);
+                } else {
+                    load_line<T, E>(v, in, base0, ti.es, lm.t, TT, valid0);
+                    load_line<T, E>(w, in, base0 + ti.sb, ti.es, lm.t, TT, valid1);
+                }
+                line_fft<T, N, DIR, V>(v, lds + lm.off, lm.es, n, lm.t, a.tw);
+                line_fft<T, N, DIR, V>(w, lds + lm.off, lm.es, n, lm.t, a.tw);
+                if (ti.sb == 1) {
+                    store_line_pair<T, E>(v, w, out, base0, ti.es, lm.t, TT, vec, valid0, valid1);
+                } else {
+                    store_line<T, E>(v, out, base0, ti.es, lm.t, TT, valid0);
+                    store_line<T, E>(w, out, base0 + ti.sb, ti.es, lm.t, TT, valid1);
+                }
+            }
         }
     }
 }
@@ -731,13 +797,25 @@ template <typename T>
 hipError_t launch_pencils(const PencilArgs<T>& a0, int n, int dir, hipStream_t s) {
     PencilArgs<T> a = a0;
     a.n = n;
-    const long blocks = pencil_blocks(a.seg, a.nseg, a.TW, &a.tiles0);
-    const int thr = a.TW * threads_per_line(n);
+    if (a.lpt != 1 && a.lpt != 2) return hipErrorInvalidValue;
+    const long blocks = pencil_blocks(a.seg, a.nseg, a.TW * a.lpt, &a.tiles0);
+    const int thr = a.TW * threads_per_line(n, a.variant);
     const size_t smem = lds_elems(n, a.TW) * sizeof(cpx<T>);
     return dispatch_size(n, [&](auto NC) -> hipError_t {
         constexpr int NN = decltype(NC)::value;
-        if (dir < 0) return launch_k(k_pencils<T, NN, -1>, dim3(blocks), dim3(thr), smem, s, a);
-        return launch_k(k_pencils<T, NN, +1>, dim3(blocks), dim3(thr), smem, s, a);
+        auto go = [&](auto VC) -> hipError_t {
+            constexpr int VV = decltype(VC)::value;
+            if (a.lpt == 2) {
+                if (dir < 0) return launch_k(k_pencils<T, NN, -1, VV, 2>, dim3(blocks), dim3(thr), smem, s, a);
+                return launch_k(k_pencils<T, NN, +1, VV, 2>, dim3(blocks), dim3(thr), smem, s, a);
+            }
+            if (dir < 0) return launch_k(k_pencils<T, NN, -1, VV, 1>, dim3(blocks), dim3(thr), smem, s, a);
+            return launch_k(k_pencils<T, NN, +1, VV, 1>, dim3(blocks), dim3(thr), smem, s, a);
+        };
+        if constexpr (plain_variant<NN>() == 0) {
+            if (!a.variant) return go(std::integral_constant<int, 0>{});
+        }
+        return go(std::integral_constant<int, 1>{});
     });
 }
 
@@ -745,6 +823,7 @@ template <typename T>
 hipError_t launch_pencil_update(const PencilUpdateArgs<T>& u0, int n, hipStream_t s) {
     PencilUpdateArgs<T> u = u0;
     u.p.n = n;
+    u.p.lpt = 1;
     const long blocks = pencil_blocks(u.p.seg, u.p.nseg, u.p.TW, &u.p.tiles0);
     const int thr = u.p.TW * threads_per_line(n, 1);
     size_t smem = lds_elems(n, u.p.TW) * sizeof(cpx<T>);

@@ -33,3 +33,49 @@ CH_EQUATION = "dconc/dt = m2*LAP conc - m6*LAP conc^2 + m4*LAP conc^3 - mk*LAP^2
 
 def ch_scalars(M: float = 1.0, kappa: float = 1.0) -> dict:
     return {"m2": 2.0 * M, "m6": 6.0 * M, "m4": 4.0 * M, "mk": M * kappa}
+
+
+# ---------------------------------------------------------------------------
+# two-field strain precipitate (examples/strain_single_precipitate/main.go:14-113,
+# without CONSERVE_PREC_VOL): conc and phase, cubic elasticity, RK4
+# ---------------------------------------------------------------------------
+PRECIPITATE_EQUATIONS = ["dconc/dt = CHEMICALPOT + kappa*LAP conc",
+                         "dphase/dt = DERIV_PHASE_ORDER + LIN_ELAST + kappa*LAP phase"]
+PRECIPITATE_SCALARS = {"kappa": 0.1}
+
+
+def precipitate_initial(N: int, radius: float, seed: int = 4):
+    """conc = phase = 1 inside a centred sphere, 0 outside, plus U(-0.01, 0.01) (numpy PCG64 seed)."""
+    n = N ** 3
+    idx = np.arange(n)
+    col, row, dep = idx % N, (idx // N) % N, idx // (N * N)
+    r2 = (col - N / 2) ** 2 + (row - N / 2) ** 2 + (dep - N / 2) ** 2
+    rng = np.random.default_rng(seed)
+    inside = np.where(r2 < radius * radius, 1.0, 0.0)
+    conc0 = inside + rng.uniform(-0.01, 0.01, n)
+    phase0 = inside + rng.uniform(-0.01, 0.01, n)
+    return conc0, phase0
+
+
+def precipitate_polynomials(A: float = 0.1, B: float = 0.1, W: float = 0.1, M: float = 1.0):
+    """The example's registered functions as polynomials of the fields' real parts:
+    CHEMICALPOT = -M (A c (1 - H) - B (1 - c) H) with H = 3x^2 - 2x^3, and
+    DERIV_PHASE_ORDER = -(-A/2 c^2 H' + B/2 (1 - c)^2 H' + W L'), H' = 6x - 6x^2,
+    L' = 2x - 6x^2 + 4x^3 (x = phase)."""
+    chem = [(-M * A, {"conc": 1}), (3 * M * (A - B), {"conc": 1, "phase": 2}),
+            (-2 * M * (A - B), {"conc": 1, "phase": 3}), (3 * M * B, {"phase": 2}), (-2 * M * B, {"phase": 3})]
+    dpo = []
+    for coef, cp in ((-0.5 * A, {"conc": 2}), (0.5 * B, {}), (-B, {"conc": 1}), (0.5 * B, {"conc": 2})):
+        dpo.append((-coef * 6.0, dict(cp, phase=1)))
+        dpo.append((coef * 6.0, dict(cp, phase=2)))
+    dpo += [(-2 * W, {"phase": 1}), (6 * W, {"phase": 2}), (-4 * W, {"phase": 3})]
+
+    def norm(terms):
+        out = []
+        for c, p in terms:
+            q = {"conc": 0.0, "phase": 0.0}
+            q.update({k: float(v) for k, v in p.items()})
+            out.append((c, q))
+        return out
+
+    return {"CHEMICALPOT": norm(chem), "DERIV_PHASE_ORDER": norm(dpo)}

@@ -107,8 +107,9 @@ typedef struct {
     int filter_order;  /* > 0: Stepper.SetFilter(NewVandeven(order)); 0: no filter */
     int fused;         /* 1: allow the fused single-field path; 0: general path only */
     int device;        /* HIP device ordinal */
-    int autotune;      /* 1: time every pass of the fused step for each admissible tile width
-                          (lines per block) on the first step and keep the fastest */
+    int autotune;      /* 1: on the first step, time every pass of the fused step for each
+                          admissible launch shape (lines per block, lines per thread, transform
+                          schedule, 3D update axis) and keep the fastest */
 } gpf_options;
 
 void gpf_options_default(gpf_options* o);
@@ -143,11 +144,16 @@ int gpf_solver_enable_timing(gpf_solver* s, int on);
 int gpf_solver_pass_times(gpf_solver* s, int maxpass, double* ms, long* launches, char* names, int namelen);
 /* launch configuration of the fused step: lines per block of the rows, forward, update and
  * inverse passes (0 = default heuristic), in 3D the axis of the update pass (2 = z, 1 = y),
- * and the transform schedule of the row pass (0 plain, 1 low-register); returns the number
- * of values written */
+ * the transform schedule of the row pass (0 plain, 1 low-register), the lines per thread
+ * of the forward and inverse pencil passes and their transform schedules; returns the
+ * number of values written */
 int gpf_solver_tuning(gpf_solver* s, int* tws, int maxn);
 /* the HIP stream the solver launches on (hipStream_t) */
 void* gpf_solver_stream(gpf_solver* s);
+/* TimeStepper.SetFilter (pf/euler.go:62-64, pf/rk4.go:138-140) with the filter as the
+ * 1000-entry table of a pf.Vandeven (pf/vandeven.go:8-40; n must be 1000), evaluated as
+ * Vandeven.Eval at |f| 2/pi; table NULL removes the filter */
+int gpf_solver_set_filter(gpf_solver* s, const double* table, long n);
 
 /* Reverse-mode gradient (no reference counterpart; it replaces nothing in gopf and
  * differentiates the Euler.Step of pf/euler.go:16-47 on the fused single-field class).

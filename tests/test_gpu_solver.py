@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import gopf_amd as G
+from gopf_amd import workloads as W
 from oracle import parser as P
 
 pytestmark = pytest.mark.gpu
@@ -284,38 +285,12 @@ def test_elastic_two_field_rk4_parity(oracle_lib):
     N = 16
     dims = [N, N, N]
     n = N ** 3
-    idx = np.arange(n)
-    col, row, dep = idx % N, (idx // N) % N, idx // (N * N)
-    r2 = (col - N / 2) ** 2 + (row - N / 2) ** 2 + (dep - N / 2) ** 2
-    rng = np.random.default_rng(4)
-    conc0 = np.where(r2 < 16.0, 1.0, 0.0) + rng.uniform(-0.01, 0.01, n)
-    phase0 = np.where(r2 < 16.0, 1.0, 0.0) + rng.uniform(-0.01, 0.01, n)
-    A, B, W, M = 0.1, 0.1, 0.1, 1.0
-    # CHEMICALPOT = -M (A c (1 - H) - B (1 - c) H), H = 3x^2 - 2x^3
-    chem = [(-M * A, {"conc": 1}), (3 * M * (A - B), {"conc": 1, "phase": 2}),
-            (-2 * M * (A - B), {"conc": 1, "phase": 3}), (3 * M * B, {"phase": 2}), (-2 * M * B, {"phase": 3})]
-    # DERIV_PHASE_ORDER = -(-A/2 c^2 H' + B/2 (1-c)^2 H' + W L'),  H' = 6x - 6x^2, L' = 2x - 6x^2 + 4x^3
-    dpo = []
-    for coef, cp in ((-0.5 * A, {"conc": 2}), (0.5 * B, {}), (-B, {"conc": 1}), (0.5 * B, {"conc": 2})):
-        dpo.append((-coef * 6.0, dict(cp, phase=1)))
-        dpo.append((coef * 6.0, dict(cp, phase=2)))
-    dpo += [(-2 * W, {"phase": 1}), (6 * W, {"phase": 2}), (-4 * W, {"phase": 3})]
-
-    def norm(terms):
-        out = []
-        for c, p in terms:
-            q = {"conc": 0.0, "phase": 0.0}
-            q.update({k: float(v) for k, v in p.items()})
-            out.append((c, q))
-        return out
-
+    conc0, phase0 = W.precipitate_initial(N, 4.0, seed=4)
     Cm = G.CubicMaterial(110.0, 60.0, 30.0)
     misfit = 0.01 * np.eye(3)
-    pm, om = pair_models({"conc": conc0, "phase": phase0}, {"kappa": 0.1},
-                         ["dconc/dt = CHEMICALPOT + kappa*LAP conc",
-                          "dphase/dt = DERIV_PHASE_ORDER + LIN_ELAST + kappa*LAP phase"], n,
+    pm, om = pair_models({"conc": conc0, "phase": phase0}, W.PRECIPITATE_SCALARS, W.PRECIPITATE_EQUATIONS, n,
                          explicit={"LIN_ELAST": G.HomogeneousModulusLinElast("phase", Cm, misfit)},
-                         polys={"CHEMICALPOT": norm(chem), "DERIV_PHASE_ORDER": norm(dpo)})
+                         polys=W.precipitate_polynomials())
     s = G.Solver(pm, dims, 0.01, stepper="rk4")
     s.Propagate(8)
     from oracle.engine import OracleSolver
@@ -337,3 +312,21 @@ def test_energy_on_demand_matches_oracle(oracle_lib):
         e, m = s.Energy(0, G.CH_BULK_POLY, 1.3)
         assert abs(e - oracle_lib.ch_energy(dims, c0, G.CH_BULK_POLY, 1.3)) <= 1e-12 * abs(e)
         assert abs(m - c0.sum()) <= 1e-12 * c0.sum()
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_set_filter_matches_constructor_filter(fused):
+    """TimeStepper.SetFilter with a Vandeven table equals the filter given at construction;
+    SetFilter(None) removes it."""
+    dims, n = [32, 32], 32 * 32
+    c0 = ch_init(n, 3)
+    runs = []
+    for mode in ("ctor", "set", "none"):
+        pm, _ = pair_models({"conc": c0}, CH_SCALARS, [CH_EQ], n)
+        s = G.Solver(pm, dims, 0.1, fused=fused, filter_order=4 if mode == "ctor" else 0)
+        if mode == "set":
+            s.SetFilter(G.NewVandeven(4))
+        s.StepperPropagate(10)
+        runs.append(s.Field(0).real)
+    assert np.array_equal(runs[0], runs[1])
+    assert not np.array_equal(runs[0], runs[2])
