@@ -365,7 +365,6 @@ public:
                  const char* const* names, double* grad_scalars) override;
 
 private:
-    void state_to_real(const double* mean, T* out);  // c = inverse transform of the state + mean
     // transforms
     const cpx<T>* tw(int n);
     void pencils(const std::vector<cpx<T>*>& dst, const std::vector<const cpx<T>*>& src, int axis, int dir);
@@ -388,6 +387,7 @@ private:
     int tw_rows_ = 0, tw_pf_ = 0, tw_pu_ = 0, tw_pi_ = 0;
     int rows_var_ = 0;  // transform schedule of the row pass (0 plain, 1 low-register)
     int lpt_pf_ = 1, lpt_pi_ = 1;  // lines per thread of the forward / inverse pencil passes
+    T* c_out_ = nullptr;           // when set, the next row pass also writes the real field c here
     int var_pf_ = 0, var_pi_ = 0;  // transform schedules of the forward / inverse pencil passes
     // 3D: axis of the fused forward-update-inverse pass (2 = z, 1 = y); the other
     // of y / z is transformed forward before and inverse after it
@@ -836,6 +836,7 @@ RowFusedArgs<T> Engine<T>::rows_args(cpx<T>* W, double* mean, int tw, int var) {
     ra.nx = g_.nx;
     ra.TW = tw > 0 ? tw : choose_tw(g_.nx, 1, sizeof(cpx<T>), 1, var);
     ra.variant = var;
+    ra.c_out = c_out_;
     ra.npoly = (int)fp_.gpoly.size();
     for (int q = 0; q < ra.npoly; ++q) ra.gpoly[q] = fp_.gpoly[q];
     ra.mean = split_mean_ ? mean : nullptr;
@@ -1315,14 +1316,13 @@ void Engine<T>::on_step_finished() {
 // multipliers m1 = A (1 + dt E), m2 = A dt L^p, A = sigma / (1 - dt D) (euler.go:32-44
 // with the fused plan's D, E, g).  Each multiplier is a symmetric operator on real
 // fields, so the transposed step is  lam_n = F^-1[m1 F lam_{n+1}] + g'(c_n) F^-1[m2 F lam_{n+1}].
-// The parameter gradient of step n is <lam_{n+1}, d c_{n+1} / d s>, summed over the half
-// spectrum with Hermitian weights.  States are recomputed per segment from checkpoints
-// of the fused state taken every `every` steps.
-template <typename T>
-void Engine<T>::state_to_real(const double* mean, T* out) {
-    c2r({S_.get()}, {out}, T(1), split_mean_ ? mean : nullptr);
-}
-
+// The parameter gradient of step n is <lam_{n+1}, d c_{n+1} / d s>: the parts through D and E
+// are summed over the half spectrum with Hermitian weights against the stored state
+// spectra, the part through the source polynomial as the real-space product
+// <F^-1[m2 F lam_{n+1}], dg/ds(c_n)> (Parseval).  States are recomputed per segment from
+// checkpoints of the state spectrum taken every `every` steps; the recompute keeps each
+// spectrum and the real field its row pass writes, so a reverse step costs one forward
+// and one inverse transform.
 template <typename T>
 void Engine<T>::adjoint(long nsteps, long every, const double* target, double* loss, double* grad_field, int nscal,
                         const char* const* names, double* grad_scalars) {
@@ -1406,19 +1406,22 @@ void Engine<T>::adjoint(long nsteps, long every, const double* target, double* l
     *loss = js + jc;
     Buf<T> lam(n), pr(n), qr(n);
     HIPCHK(hipMemcpyAsync(lam.get(), lamT.data(), n * sizeof(T), hipMemcpyHostToDevice, st_));
-    // work arrays
+    // segment storage: the state spectra S_n (and means) and the real fields c_n that the
+    // recomputed row passes write as a by-product
     const long seglen = std::min(every, nsteps);
+    const bool need_real = fp_.has_src;
+    std::vector<Buf<cpx<T>>> SS;
     std::vector<Buf<T>> R;
-    for (long i = 0; i <= seglen; ++i) R.emplace_back(n);
-    std::vector<Buf<T>> Gr;
-    std::vector<Buf<cpx<T>>> Gk;
-    for (int s = 0; s < nscal; ++s) {
-        Gr.emplace_back(dpoly[s].empty() ? 1 : n);
-        Gk.emplace_back(dpoly[s].empty() ? 1 : nk);
-    }
-    Buf<cpx<T>> Lk(nk), C1k(need_c1 ? nk : 1), C0k(need_c0 ? nk : 1), Pk(nk), Qk(nk);
+    for (long i = 0; i <= seglen; ++i) SS.emplace_back(nk);
+    if (need_real)
+        for (long i = 0; i < seglen; ++i) R.emplace_back(n);
+    Buf<double> segmean(seglen + 1);
+    bool any_g = false;
+    for (int q = 0; q < nscal; ++q) any_g |= !dpoly[q].empty();
+    Buf<cpx<T>> Lk(nk), Pk(nk), Qk(nk);
     const long nblk = std::max(1L, std::min((nk + 255) / 256, 1024L));
-    Buf<double> partial((size_t)std::max(nscal, 1) * nblk), acc(std::max(nscal, 1));
+    const long nblk_r = std::max(1L, std::min((n + 255) / 256, 1024L));
+    Buf<double> partial((size_t)std::max(nscal, 1) * std::max(nblk, nblk_r)), acc(std::max(nscal, 1));
     HIPCHK(hipMemsetAsync(acc.get(), 0, std::max(nscal, 1) * sizeof(double), st_));
     AdjArgs<T> a;
     a.g = g_.kg;
@@ -1434,9 +1437,7 @@ void Engine<T>::adjoint(long nsteps, long every, const double* target, double* l
     a.Q = Qk.get();
     a.nscal = nscal;
     a.dcoef = d_dcoef.get();
-    a.C1 = need_c1 ? C1k.get() : nullptr;
-    a.C0 = need_c0 ? C0k.get() : nullptr;
-    for (int s = 0; s < nscal; ++s) a.Gs[s] = dpoly[s].empty() ? nullptr : Gk[s].get();
+    a.cscale = (double)n;
     a.partial = partial.get();
     a.inv_n = 1.0 / (double)n;
     const int ndg = (int)dg.size();
@@ -1446,31 +1447,36 @@ void Engine<T>::adjoint(long nsteps, long every, const double* target, double* l
         HIPCHK(hipMemcpyAsync(mean_.get(), ckmean.get() + seg, sizeof(double), hipMemcpyDeviceToDevice, st_));
         prime();
         for (long s = s0; s < s1; ++s) {
-            state_to_real(mean_.get(), R[s - s0].get());
+            HIPCHK(hipMemcpyAsync(SS[s - s0].get(), S_.get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+            HIPCHK(hipMemcpyAsync(segmean.get() + (s - s0), mean_.get(), sizeof(double), hipMemcpyDeviceToDevice, st_));
+            c_out_ = need_real ? R[s - s0].get() : nullptr;
             fused_step();
+            c_out_ = nullptr;
         }
-        state_to_real(mean_.get(), R[s1 - s0].get());
+        HIPCHK(hipMemcpyAsync(SS[s1 - s0].get(), S_.get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+        HIPCHK(hipMemcpyAsync(segmean.get() + (s1 - s0), mean_.get(), sizeof(double), hipMemcpyDeviceToDevice, st_));
         for (long s = s1 - 1; s >= s0; --s) {
-            const T* cn = R[s - s0].get();
-            std::vector<const T*> rin = {lam.get()};
-            std::vector<cpx<T>*> kout = {Lk.get()};
-            if (need_c1) { rin.push_back(R[s + 1 - s0].get()); kout.push_back(C1k.get()); }
-            if (need_c0) { rin.push_back(cn); kout.push_back(C0k.get()); }
-            for (int q = 0; q < nscal; ++q) {
-                if (dpoly[q].empty()) continue;
-                HIPCHK(launch_adj_lambda<T>(nullptr, nullptr, cn, (int)std::min(dpoly[q].size(), (size_t)kMaxPoly),
-                                            d_dpoly.get() + (size_t)q * kMaxPoly, Gr[q].get(), n, st_));
-                rin.push_back(Gr[q].get());
-                kout.push_back(Gk[q].get());
-            }
-            r2c(rin, kout, T(1));
-            if (nscal > 0) {
+            const T* cn = need_real ? R[s - s0].get() : nullptr;
+            r2c({lam.get()}, {Lk.get()}, T(1));
+            if (nscal > 0 && (need_c1 || need_c0)) {
+                // derivatives of D and E, against the transforms of c_{n+1} and c_n
+                a.C1 = need_c1 ? SS[s + 1 - s0].get() : nullptr;
+                a.C0 = need_c0 ? SS[s - s0].get() : nullptr;
+                a.mean1 = split_mean_ ? segmean.get() + (s + 1 - s0) : nullptr;
+                a.mean0 = split_mean_ ? segmean.get() + (s - s0) : nullptr;
                 HIPCHK(launch_adj_grad<T>(a, nblk, st_));
                 HIPCHK(launch_accumulate(partial.get(), nscal, nblk, acc.get(), st_));
             }
             HIPCHK(launch_adj_mult<T>(a, st_));
             c2r({Pk.get(), Qk.get()}, {pr.get(), qr.get()}, T(1) / (T)n);
-            HIPCHK(launch_adj_lambda<T>(pr.get(), qr.get(), cn, ndg, d_dg.get(), lam.get(), n, st_));
+            if (need_real) {
+                // lam_n = p + g'(c_n) q, and the source derivatives <q, dg/ds(c_n)> in real space
+                HIPCHK(launch_adj_lambda<T>(pr.get(), qr.get(), cn, ndg, d_dg.get(), lam.get(), n, any_g ? nscal : 0,
+                                            d_dpoly.get(), partial.get(), nblk_r, st_));
+                if (any_g) HIPCHK(launch_accumulate(partial.get(), nscal, nblk_r, acc.get(), st_));
+            } else {
+                HIPCHK(hipMemcpyAsync(lam.get(), pr.get(), n * sizeof(T), hipMemcpyDeviceToDevice, st_));
+            }
         }
     }
     // outputs; the solver is left at c_T

@@ -384,11 +384,20 @@ __global__ void __launch_bounds__(256) k_adj_grad(AdjArgs<T> a) {
         double L, A, Ainv, m1, m2, w;
         adj_coeffs(a.g, i, a.nmain, a.dt, a.denom, a.lin, a.src_lap, a.filter, L, A, Ainv, m1, m2, w);
         const cpx<T> l = a.lam[i];
-        const cpx<T> c1 = a.C1 ? a.C1[i] : mk<T>(T(0), T(0));
-        const cpx<T> c0 = a.C0 ? a.C0[i] : mk<T>(T(0), T(0));
+        double c1x = 0.0, c1y = 0.0, c0x = 0.0, c0y = 0.0;
+        if (a.C1) {
+            c1x = a.cscale * (double)a.C1[i].x;
+            c1y = a.cscale * (double)a.C1[i].y;
+            if (i == 0 && a.mean1) c1x += a.cscale * *a.mean1;
+        }
+        if (a.C0) {
+            c0x = a.cscale * (double)a.C0[i].x;
+            c0y = a.cscale * (double)a.C0[i].y;
+            if (i == 0 && a.mean0) c0x += a.cscale * *a.mean0;
+        }
         // Re(conj(l) z) = l.x z.x + l.y z.y
-        const double lc1 = (double)l.x * c1.x + (double)l.y * c1.y;
-        const double lc0 = (double)l.x * c0.x + (double)l.y * c0.y;
+        const double lc1 = (double)l.x * c1x + (double)l.y * c1y;
+        const double lc0 = (double)l.x * c0x + (double)l.y * c0y;
 #pragma unroll
         for (int s = 0; s < kMaxAdjScalars; ++s) {
             if (s >= a.nscal) break;
@@ -396,12 +405,7 @@ __global__ void __launch_bounds__(256) k_adj_grad(AdjArgs<T> a) {
             double dD = 0.0, dE = 0.0;
             for (int t = 0; t < a.denom.n; ++t) dD += dc[t] * ipow(L, a.denom.pow[t]);
             for (int t = 0; t < a.lin.n; ++t) dE += dc[kMaxKTerms + t] * ipow(L, a.lin.pow[t]);
-            double v = Ainv * a.dt * dD * lc1 + A * a.dt * dE * lc0;
-            if (a.Gs[s]) {
-                const cpx<T> gsv = a.Gs[s][i];
-                v += m2 * ((double)l.x * gsv.x + (double)l.y * gsv.y);
-            }
-            acc[s] += w * v;
+            acc[s] += w * (Ainv * a.dt * dD * lc1 + A * a.dt * dE * lc0);
         }
     }
 #pragma unroll
@@ -413,12 +417,31 @@ __global__ void __launch_bounds__(256) k_adj_grad(AdjArgs<T> a) {
 }
 
 template <typename T>
-__global__ void k_adj_lambda(const T* p, const T* q, const T* c, int npoly, const double* dpoly, T* out, long n) {
+__global__ void __launch_bounds__(256) k_adj_lambda(const T* p, const T* q, const T* c, int npoly, const double* dpoly,
+                                                    T* out, long n, int nscal, const double* spoly, double* partial) {
+    __shared__ double red[256];
+    double acc[kMaxAdjScalars];
+    for (int s = 0; s < kMaxAdjScalars; ++s) acc[s] = 0.0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const double x = (double)c[i];
         double d = 0.0;
         for (int k = npoly - 1; k >= 0; --k) d = d * x + dpoly[k];
-        out[i] = (T)((p ? (double)p[i] : 0.0) + d * (q ? (double)q[i] : 1.0));
+        const double qv = q ? (double)q[i] : 1.0;
+        out[i] = (T)((p ? (double)p[i] : 0.0) + d * qv);
+#pragma unroll
+        for (int s = 0; s < kMaxAdjScalars; ++s) {
+            if (s >= nscal) break;
+            const double* sp = spoly + (long)s * kMaxPoly;
+            double h = 0.0;
+            for (int k = kMaxPoly - 1; k >= 0; --k) h = h * x + sp[k];
+            acc[s] += qv * h;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < kMaxAdjScalars; ++s) {
+        if (s >= nscal) break;  // uniform over the block
+        const double tot = block_sum(acc[s], red);
+        if (threadIdx.x == 0) partial[(long)s * gridDim.x + blockIdx.x] = tot;
     }
 }
 
@@ -516,8 +539,10 @@ hipError_t launch_adj_grad(const AdjArgs<T>& a, long nblocks, hipStream_t s) {
 }
 template <typename T>
 hipError_t launch_adj_lambda(const T* p, const T* q, const T* c, int npoly, const double* dpoly, T* out, long n,
-                             hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_lambda<T>, dim3(grid_for(n)), dim3(256), 0, s, p, q, c, npoly, dpoly, out, n);
+                             int nscal, const double* spoly, double* partial, long nblocks, hipStream_t s) {
+    const unsigned grid = nscal > 0 ? (unsigned)nblocks : grid_for(n);
+    hipLaunchKernelGGL(k_adj_lambda<T>, dim3(grid), dim3(256), 0, s, p, q, c, npoly, dpoly, out, n, nscal, spoly,
+                       partial);
     return hipGetLastError();
 }
 hipError_t launch_accumulate(const double* partial, int nscal, long nblocks, double* acc, hipStream_t s) {
@@ -545,8 +570,8 @@ hipError_t launch_reduce2(const double* a, long na, double ca, const double* b, 
     template hipError_t launch_kgrad_energy<T>(const cpx<T>*, KGrid, long, long, double*, long, hipStream_t); \
     template hipError_t launch_adj_mult<T>(const AdjArgs<T>&, hipStream_t);                                  \
     template hipError_t launch_adj_grad<T>(const AdjArgs<T>&, long, hipStream_t);                            \
-    template hipError_t launch_adj_lambda<T>(const T*, const T*, const T*, int, const double*, T*, long,     \
-                                             hipStream_t);
+    template hipError_t launch_adj_lambda<T>(const T*, const T*, const T*, int, const double*, T*, long, int, \
+                                             const double*, double*, long, hipStream_t);
 GPF_INST(float)
 GPF_INST(double)
 #undef GPF_INST
