@@ -510,6 +510,35 @@ __device__ __forceinline__ void split_store(const cpx<T>* v, cpx<T>* L, int n, i
     __syncthreads();
 }
 
+// split_store for the row pass with a compiled line shape: rows r1 (A) and r1 + 1 (B) of the
+// output, the k loop unrolled at compile time (nh a multiple of TTc), stores at immediate
+// offsets from per-row base pointers; the x-Nyquist pair is written by thread 0
+template <typename T, int E, int TTc>
+__device__ __forceinline__ void split_store_rows(const cpx<T>* v, cpx<T>* L, int n, int t, long r1, bool v1, bool v2,
+                                                 cpx<T>* gmain, cpx<T>* gnyq) {
+    const int nh = n / 2;
+#pragma unroll
+    for (int e = 0; e < E; ++e) L[lds_pad(t + TTc * e)] = v[e];
+    __syncthreads();
+    cpx<T>* a1 = gmain + r1 * nh + t;
+    constexpr int J = (E * TTc / 2) / TTc;  // = E / 2 k values per thread below nh
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int k = t + TTc * j;
+        const cpx<T> z = L[lds_pad(k)];
+        const cpx<T> zm = L[lds_pad((n - k) & (n - 1))];
+        // A = (z + conj zm) / 2,  B = (z - conj zm) / (2 i)
+        if (v1) a1[TTc * j] = mk<T>(T(0.5) * (z.x + zm.x), T(0.5) * (z.y - zm.y));
+        if (v2) a1[TTc * j + nh] = mk<T>(T(0.5) * (z.y + zm.y), T(-0.5) * (z.x - zm.x));
+    }
+    if (t == 0) {
+        const cpx<T> z = L[lds_pad(nh)];
+        if (v1) gnyq[r1] = mk<T>(z.x, T(0));
+        if (v2) gnyq[r1 + 1] = mk<T>(z.y, T(0));
+    }
+    __syncthreads();
+}
+
 // Hermitian extension of a half-spectrum row at canonical positions
 template <typename T, int E>
 __device__ __forceinline__ void load_herm(cpx<T>* v, const cpx<T>* __restrict__ m, const cpx<T>* __restrict__ q,
@@ -564,6 +593,48 @@ __device__ __forceinline__ void load_herm_pair(cpx<T>* v, const cpx<T>* m, const
         }
         v[e] = mk<T>(A.x - B.y, A.y + B.x);
     }
+}
+
+// As load_herm_pair, with each half-spectrum value read from HBM once: the direct
+// positions are loaded and staged in the line's LDS (row r1 at [0, nh], row r1 + 1 at
+// [nh + 1, n + 1]), and the mirrored positions are read back from LDS.  The line's LDS
+// region (lds_line(n) >= n + 2 elements for n >= 16) is free again on return.
+template <typename T, int E, int TTc>
+__device__ __forceinline__ void load_herm_pair_lds(cpx<T>* v, const cpx<T>* m, const cpx<T>* q, long r1, int n, int t,
+                                                   bool v1, bool v2, cpx<T>* L) {
+    const int nh = n / 2;
+    const cpx<T>* a1 = m + r1 * nh + t;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int lo = TTc * e, hi = lo + TTc - 1;
+        const int p = t + lo;
+        if (lo > nh) continue;  // mirrored: filled below
+        cpx<T> A = mk<T>(T(0), T(0)), B = A;
+        if (hi < nh || p < nh) {
+            if (v1) A = a1[lo];
+            if (v2) B = a1[lo + nh];
+        } else if (p == nh) {
+            if (v1) A = q[r1];
+            if (v2) B = q[r1 + 1];
+        } else {
+            continue;
+        }
+        L[p] = A;
+        L[nh + 1 + p] = B;
+        v[e] = mk<T>(A.x - B.y, A.y + B.x);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int lo = TTc * e, hi = lo + TTc - 1;
+        const int p = t + lo;
+        if (hi < nh) continue;
+        if (lo > nh || p > nh) {
+            const cpx<T> A = conjg(L[n - p]), B = conjg(L[nh + 1 + n - p]);
+            v[e] = mk<T>(A.x - B.y, A.y + B.x);
+        }
+    }
+    __syncthreads();
 }
 
 template <typename T, int N>
@@ -709,7 +780,10 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
         for (int q = 0; q < kMaxPoly; ++q) gp[q] = (T)gd[q];
     }
     cpx<T> v[E];
-    load_herm_pair<T, E, (N > 0 ? LineShape<N, V>::T : 0)>(v, a.s_main, a.s_nyq, r1, n, lm.t, TT, v1, v2);
+    if constexpr (N >= 16)
+        load_herm_pair_lds<T, E, LineShape<N, V>::T>(v, a.s_main, a.s_nyq, r1, n, lm.t, v1, v2, lds + lm.off);
+    else
+        load_herm_pair<T, E, (N > 0 ? LineShape<N, V>::T : 0)>(v, a.s_main, a.s_nyq, r1, n, lm.t, TT, v1, v2);
     line_fft<T, N, +1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
     KSum sb, sm;
     if (a.npoly <= 4) rows_pointwise<T, E, 4, MON>(v, gp, a, r1, n, lm.t, TT, v1, v2, mu, sb, sm);
@@ -724,8 +798,11 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
         }
     }
     line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
-    split_store<T, E>(v, lds + lm.off, n, lm.t, TT, r1, v1, a.g_main, a.g_nyq, v2 ? a.g_main + n / 2 : nullptr,
-                      v2 ? a.g_nyq + 1 : nullptr);
+    if constexpr (N >= 16 && (N & (N - 1)) == 0 && E % 2 == 0)
+        split_store_rows<T, E, LineShape<N, V>::T>(v, lds + lm.off, n, lm.t, r1, v1, v2, a.g_main, a.g_nyq);
+    else
+        split_store<T, E>(v, lds + lm.off, n, lm.t, TT, r1, v1, a.g_main, a.g_nyq, v2 ? a.g_main + n / 2 : nullptr,
+                          v2 ? a.g_nyq + 1 : nullptr);
 }
 
 // v: 0 = plain pass, 1 = update pass
