@@ -21,6 +21,7 @@
 
 #include <cmath>
 #include <complex>
+#include <algorithm>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -324,6 +325,7 @@ class EngineBase {
 public:
     virtual ~EngineBase() = default;
     virtual void step() = 0;
+    virtual void steps(long n) = 0;  // n steps; the fused path replays captured graphs
     virtual void on_step_finished() = 0;
     virtual void get_field(int idx, double* out) = 0;
     virtual void set_field(int idx, const double* in) = 0;
@@ -348,6 +350,7 @@ public:
     ~Engine() override;
 
     void step() override;
+    void steps(long n) override;
     void on_step_finished() override;
     void get_field(int idx, double* out) override;
     void set_field(int idx, const double* in) override;
@@ -378,6 +381,12 @@ private:
     void tend(int pass);
     // fused path
     void fused_step();
+    void prepare_fused(long upcoming);  // allocations ahead of launches (never during graph capture)
+    void enqueue_fused();               // the launches of one step (capturable)
+    void drop_graph();
+    hipGraphExec_t graph_ = nullptr;
+    bool graph_mon_ = false;  // the monitor setting the graph was captured with
+    static constexpr int kGraphSteps = 10;
     RowFusedArgs<T> rows_args(cpx<T>* W, double* mean, int tw, int var);
     PencilUpdateArgs<T> update_args(cpx<T>* S, cpx<T>* W, double* mean, int tw);
     void prime();
@@ -435,6 +444,7 @@ private:
     std::vector<double> mon_poly_;
     double mon_kappa_ = 0.0;
     Buf<double> pa_bulk_, pa_mass_, pc_eg_, series_, dpoly_;
+    Buf<long> series_count_;  // device slot counter of the series (replayable by a graph)
     long nblk_a_ = 0, nblk_c_ = 0, series_cap_ = 0, series_n_ = 0;
     // timing
     bool timing_ = false;
@@ -552,6 +562,7 @@ Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gp
 
 template <typename T>
 Engine<T>::~Engine() {
+    drop_graph();
     if (st_) {
         (void)hipStreamSynchronize(st_);
         for (auto& p : pending_) {
@@ -804,6 +815,11 @@ void Engine<T>::monitor(int npoly, const double* poly, double kappa) {
     mon_poly_.assign(poly, poly + npoly);
     mon_kappa_ = kappa;
     series_n_ = 0;
+    if (!series_count_.get()) series_count_.alloc(1);
+    const long zero = 0;
+    HIPCHK(hipMemcpyAsync(series_count_.get(), &zero, sizeof(long), hipMemcpyHostToDevice, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+    drop_graph();
 }
 
 template <typename T>
@@ -996,6 +1012,7 @@ void Engine<T>::autotune() {
 template <typename T>
 void Engine<T>::set_filter(const double* table) {
     HIPCHK(hipStreamSynchronize(st_));
+    drop_graph();
     if (!table) {
         has_filter_ = false;
         return;
@@ -1013,26 +1030,38 @@ int Engine<T>::tuning(int* tws, int maxn) {
 }
 
 template <typename T>
-void Engine<T>::fused_step() {
-    if (autotune_ && !tuned_) autotune();
-    RowFusedArgs<T> ra = rows_args(W_.get(), mean_.get(), tw_rows_, rows_var_);
-    const long nblk_a = rows_fused_blocks(g_.rows, ra.TW);
-    const int nu = axis_len(upd_axis());
+void Engine<T>::drop_graph() {
+    if (graph_) (void)hipGraphExecDestroy(graph_);
+    graph_ = nullptr;
+}
+
+template <typename T>
+void Engine<T>::prepare_fused(long upcoming) {
+    if (!mon_) return;
+    const long nblk_a = rows_fused_blocks(g_.rows, rows_args(W_.get(), mean_.get(), tw_rows_, rows_var_).TW);
     PencilUpdateArgs<T> u = update_args(S_.get(), W_.get(), mean_.get(), tw_pu_);
     int tiles0 = 0;
     const long nblk_c = pencil_blocks(u.p.seg, u.p.nseg, u.p.TW, &tiles0);
+    if (nblk_a_ != nblk_a) { pa_bulk_.alloc(nblk_a); pa_mass_.alloc(nblk_a); nblk_a_ = nblk_a; drop_graph(); }
+    if (nblk_c_ != nblk_c) { pc_eg_.alloc(nblk_c); nblk_c_ = nblk_c; drop_graph(); }
+    if (series_n_ + upcoming > series_cap_) {
+        const long cap = std::max({1024L, 2 * series_cap_, series_n_ + upcoming});
+        Buf<double> nb(2 * cap);
+        if (series_n_ > 0)
+            HIPCHK(hipMemcpyAsync(nb.get(), series_.get(), 2 * series_n_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+        series_.swap(nb);
+        series_cap_ = cap;
+        drop_graph();
+    }
+}
+
+template <typename T>
+void Engine<T>::enqueue_fused() {
+    RowFusedArgs<T> ra = rows_args(W_.get(), mean_.get(), tw_rows_, rows_var_);
+    const int nu = axis_len(upd_axis());
+    PencilUpdateArgs<T> u = update_args(S_.get(), W_.get(), mean_.get(), tw_pu_);
     if (mon_) {
-        if (nblk_a_ != nblk_a) { pa_bulk_.alloc(nblk_a); pa_mass_.alloc(nblk_a); nblk_a_ = nblk_a; }
-        if (nblk_c_ != nblk_c) { pc_eg_.alloc(nblk_c); nblk_c_ = nblk_c; }
-        if (series_n_ >= series_cap_) {
-            const long cap = std::max(1024L, 2 * series_cap_);
-            Buf<double> nb(2 * cap);
-            if (series_n_ > 0)
-                HIPCHK(hipMemcpyAsync(nb.get(), series_.get(), 2 * series_n_ * sizeof(double), hipMemcpyDeviceToDevice, st_));
-            series_.swap(nb);
-            HIPCHK(hipStreamSynchronize(st_));
-            series_cap_ = cap;
-        }
         ra.nbulk = (int)mon_poly_.size();
         for (int q = 0; q < ra.nbulk; ++q) ra.bulk[q] = mon_poly_[q];
         ra.bulk_partial = pa_bulk_.get();
@@ -1059,12 +1088,56 @@ void Engine<T>::fused_step() {
     tbegin(2);
     HIPCHK(launch_pencil_update<T>(u, nu, st_));
     tend(2);
-    if (mon_) {
-        HIPCHK(launch_reduce2(pa_bulk_.get(), nblk_a, 1.0, pc_eg_.get(), nblk_c, 0.5 * mon_kappa_ / (double)g_.n,
-                              series_.get() + 2 * series_n_, st_));
-        HIPCHK(launch_reduce2(pa_mass_.get(), nblk_a, 1.0, pa_mass_.get(), 0, 0.0, series_.get() + 2 * series_n_ + 1, st_));
-        ++series_n_;
+    if (mon_)
+        HIPCHK(launch_series_append(pa_bulk_.get(), nblk_a_, pc_eg_.get(), nblk_c_, 0.5 * mon_kappa_ / (double)g_.n,
+                                    pa_mass_.get(), series_.get(), series_count_.get(), st_));
+}
+
+template <typename T>
+void Engine<T>::fused_step() {
+    if (autotune_ && !tuned_) {
+        autotune();
+        drop_graph();
     }
+    prepare_fused(1);
+    enqueue_fused();
+    if (mon_) ++series_n_;
+}
+
+// n fused steps: launch-bound grids replay a captured graph of kGraphSteps steps (kernels
+// only: no memset or allocation is captured; the series slot comes from a device counter)
+template <typename T>
+void Engine<T>::steps(long n) {
+    if (!fp_.ok) {
+        for (long i = 0; i < n; ++i) step();
+        return;
+    }
+    if (n > 0 && autotune_ && !tuned_) {
+        fused_step();
+        --n;
+    }
+    if (timing_ || c_out_ || n < kGraphSteps) {
+        for (long i = 0; i < n; ++i) fused_step();
+        return;
+    }
+    if (graph_ && graph_mon_ != mon_) drop_graph();
+    prepare_fused(n);
+    if (!graph_) {
+        graph_mon_ = mon_;
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < kGraphSteps; ++k) enqueue_fused();
+        HIPCHK(hipStreamEndCapture(st_, &g));
+        const hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(e);
+    }
+    while (n >= kGraphSteps) {
+        HIPCHK(hipGraphLaunch(graph_, st_));
+        n -= kGraphSteps;
+        if (mon_) series_n_ += kGraphSteps;
+    }
+    for (; n > 0; --n) fused_step();
 }
 
 // ---------------------------------------------------------------------------
@@ -1378,13 +1451,11 @@ void Engine<T>::adjoint(long nsteps, long every, const double* target, double* l
     const long nck = (nsteps + every - 1) / every;
     std::vector<Buf<cpx<T>>> ck;
     Buf<double> ckmean(nck);
-    for (long s = 0; s < nsteps; ++s) {
-        if (s % every == 0) {
-            ck.emplace_back(nk);
-            HIPCHK(hipMemcpyAsync(ck.back().get(), S_.get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
-            HIPCHK(hipMemcpyAsync(ckmean.get() + s / every, mean_.get(), sizeof(double), hipMemcpyDeviceToDevice, st_));
-        }
-        fused_step();
+    for (long s = 0; s < nsteps; s += every) {
+        ck.emplace_back(nk);
+        HIPCHK(hipMemcpyAsync(ck.back().get(), S_.get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+        HIPCHK(hipMemcpyAsync(ckmean.get() + s / every, mean_.get(), sizeof(double), hipMemcpyDeviceToDevice, st_));
+        steps(std::min(every, nsteps - s));
     }
     Buf<cpx<T>> Sf(nk);
     Buf<double> meanf(1);
@@ -1807,6 +1878,12 @@ int gpf_solver_propagate(gpf_solver* s, int nsteps) {
     return guarded([&] {
         if (!s || nsteps < 0) throw Error(GPF_EINVAL, "bad argument");
         set_device(s->device);
+        if (s->eng->fused()) {
+            // the fused class has no term with OnStepFinished state
+            s->eng->steps(nsteps);
+            if (s->stepper == GPF_EULER) s->current_step += nsteps;  // Euler.Step increments (euler.go:46)
+            return;
+        }
         for (int i = 0; i < nsteps; ++i) {
             s->eng->step();
             if (s->stepper == GPF_EULER) s->current_step++;  // Euler.Step increments (euler.go:46)
@@ -1819,10 +1896,8 @@ int gpf_solver_stepper_propagate(gpf_solver* s, int nsteps) {
     return guarded([&] {
         if (!s || nsteps < 0) throw Error(GPF_EINVAL, "bad argument");
         set_device(s->device);
-        for (int i = 0; i < nsteps; ++i) {
-            s->eng->step();
-            s->current_step++;  // Euler.Step / RK4.Propagate increment
-        }
+        s->eng->steps(nsteps);
+        s->current_step += nsteps;  // Euler.Step / RK4.Propagate increment
     });
 }
 

@@ -129,5 +129,10 @@ hipError_t launch_copy_bw(const float* src, float* dst, long n, hipStream_t s);
 // single-block reduction of partials; out[0] = sum a * ca + sum b * cb
 hipError_t launch_reduce2(const double* a, long na, double ca, const double* b, long nb, double cb,
                           double* out, hipStream_t s);
+// one entry of the energy series at the slot a device counter names (so that a captured
+// graph can replay it): series[2 i] = sum bulk + ceg sum eg, series[2 i + 1] = sum mass,
+// i = *counter, then *counter = i + 1
+hipError_t launch_series_append(const double* bulk, long nbulk, const double* eg, long neg, double ceg,
+                                const double* mass, double* series, long* counter, hipStream_t s);
 
 }  // namespace gpf
