@@ -544,6 +544,7 @@ Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gp
                 for (const auto& fac : dd.terms[t].second) a.pw[t][fac.first] += fac.second;
             }
             for (int f = 0; f < nf_; ++f) a.f[f] = F_[f].get();
+            make_dense_derived(a);
             dargs_.push_back(a);
         }
         pass_names_ = {"general_step"};
@@ -1286,10 +1287,10 @@ template <typename T>
 void Engine<T>::general_step_rk4() {
     ensure_k_fields();
     const size_t kb = g_.nk * sizeof(cpx<T>);
-    for (int i = 0; i < nf_; ++i) {
+    // initial = field (rk4.go:35-38); final starts as the field too and is formed by the
+    // first stage's update reading the field directly
+    for (int i = 0; i < nf_; ++i)
         HIPCHK(hipMemcpyAsync(INIT_[i].get(), K_[i].get(), kb, hipMemcpyDeviceToDevice, st_));
-        HIPCHK(hipMemcpyAsync(FINAL_[i].get(), K_[i].get(), kb, hipMemcpyDeviceToDevice, st_));
-    }
     sync_derived();
     eval_elastic();  // the lagged field is constant within a step (homoLinElast.go:130-134)
     for (int i = 0; i < nf_; ++i) {
@@ -1300,10 +1301,11 @@ void Engine<T>::general_step_rk4() {
     const double fac_corr[3] = {0.5, 0.5, 1.0};
     for (int stage = 0; stage < 4; ++stage) {
         for (int i = 0; i < nf_; ++i) {
-            // final += fac dt kFactor; field = initial  (rk4.go:77-84)
+            // final += fac dt kFactor; field = initial  (rk4.go:77-84).  The field is not copied
+            // back: the correction below reads the initial field directly
             KUpdateArgs<T> a;
             a.dst = FINAL_[i].get();
-            a.cur = FINAL_[i].get();
+            a.cur = stage == 0 ? K_[i].get() : FINAL_[i].get();
             a.brick[0] = KF_[i].get();
             a.coef[0] = 1.0;
             a.pw[0] = 0;
@@ -1315,7 +1317,6 @@ void Engine<T>::general_step_rk4() {
             a.nmain = g_.nmain;
             a.nk = g_.nk;
             HIPCHK(launch_kupdate<T>(a, st_));
-            HIPCHK(hipMemcpyAsync(K_[i].get(), INIT_[i].get(), kb, hipMemcpyDeviceToDevice, st_));
         }
         if (stage == 3) break;
         // correction(factor) (rk4.go:101-127)
@@ -1323,7 +1324,7 @@ void Engine<T>::general_step_rk4() {
         std::vector<const cpx<T>*> in;
         std::vector<T*> out;
         for (int i = 0; i < nf_; ++i) {
-            rhs_update(i, K_[i].get(), K_[i].get(), 1.0, f * dt_, f * dt_, false, KF_[i].get());
+            rhs_update(i, K_[i].get(), INIT_[i].get(), 1.0, f * dt_, f * dt_, false, KF_[i].get());
             in.push_back(K_[i].get());
             out.push_back(F_[i].get());
         }

@@ -21,7 +21,31 @@ struct DerivedArgs {
     double coef[kMaxDTerms];
     double pw[kMaxDTerms][kMaxFields];
     long n = 0;
+    // dense form for at most two fields and integer powers 0..3:
+    // out = sum_ij dc[i][j] f0^i f1^j (f1 = 0 when nf == 1)
+    int dense = 0;
+    double dc[4][4] = {};
 };
+
+// fill DerivedArgs::dc when the term list has the dense form; returns a.dense
+template <typename T>
+inline int make_dense_derived(DerivedArgs<T>& a) {
+    a.dense = 0;
+    if (a.nf < 1 || a.nf > 2) return 0;
+    double dc[4][4] = {};
+    for (int t = 0; t < a.nterm; ++t) {
+        int e[2] = {0, 0};
+        for (int f = 0; f < a.nf; ++f) {
+            const double p = a.pw[t][f];
+            if (p < 0.0 || p > 3.0 || p != (double)(int)p) return 0;
+            e[f] = (int)p;
+        }
+        dc[e[0]][e[1]] += a.coef[t];
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) a.dc[i][j] = dc[i][j];
+    return a.dense = 1;
+}
 
 // half-spectrum k-space update
 //   dst = filt * (alpha * cur + beta * sum_t coef_t L^pow_t B_t) / (1 - gamma * D(k))

@@ -81,6 +81,21 @@ __device__ __forceinline__ T powT(T x, double p) {
 
 template <typename T>
 __global__ void k_derived(DerivedArgs<T> a) {
+    if (a.dense) {
+        // Horner in f0 of polynomials in f1
+        for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.n; i += (long)gridDim.x * blockDim.x) {
+            const T x0 = a.f[0][i];
+            const T x1 = (a.nf > 1) ? a.f[1][i] : T(0);
+            T acc = T(0);
+#pragma unroll
+            for (int p = 3; p >= 0; --p) {
+                const T inner = (((T)a.dc[p][3] * x1 + (T)a.dc[p][2]) * x1 + (T)a.dc[p][1]) * x1 + (T)a.dc[p][0];
+                acc = acc * x0 + inner;
+            }
+            a.out[i] = acc;
+        }
+        return;
+    }
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.n; i += (long)gridDim.x * blockDim.x) {
         T x[kMaxFields];
         for (int f = 0; f < a.nf; ++f) x[f] = a.f[f] ? a.f[f][i] : T(0);
