@@ -1405,8 +1405,9 @@ void Engine<T>::adjoint(long nsteps, long every, const double* target, double* l
     if (nscal < 0 || nscal > kMaxAdjScalars) throw Error(GPF_EINVAL, "at most 8 scalars per adjoint run");
     if (nscal > 0 && (!names || !grad_scalars)) throw Error(GPF_EINVAL, "scalar names and output are required");
     if (autotune_ && !tuned_) autotune();
-    const bool mon = mon_;
+    const bool mon = mon_, timing = timing_;
     mon_ = false;
+    timing_ = false;  // the segments are captured as graphs: no per-pass events
     const long n = g_.n, nk = g_.nk;
     // parameter tables: derivative of D, E (per LAP power) and of g (per power of c)
     std::vector<double> dcoef((size_t)std::max(nscal, 1) * 2 * kMaxKTerms, 0.0);
@@ -1493,8 +1494,13 @@ void Engine<T>::adjoint(long nsteps, long every, const double* target, double* l
     Buf<cpx<T>> Lk(nk), Pk(nk), Qk(nk);
     const long nblk = std::max(1L, std::min((nk + 255) / 256, 1024L));
     const long nblk_r = std::max(1L, std::min((n + 255) / 256, 1024L));
-    Buf<double> partial((size_t)std::max(nscal, 1) * std::max(nblk, nblk_r)), acc(std::max(nscal, 1));
+    // per-block partial sums, accumulated in place across the reverse sweep (each block owns
+    // its slot; launches are ordered on the stream) and reduced once at the end
+    Buf<double> partial_k((size_t)std::max(nscal, 1) * nblk), partial_r((size_t)std::max(nscal, 1) * nblk_r),
+        acc(std::max(nscal, 1));
     HIPCHK(hipMemsetAsync(acc.get(), 0, std::max(nscal, 1) * sizeof(double), st_));
+    HIPCHK(hipMemsetAsync(partial_k.get(), 0, std::max(nscal, 1) * nblk * sizeof(double), st_));
+    HIPCHK(hipMemsetAsync(partial_r.get(), 0, std::max(nscal, 1) * nblk_r * sizeof(double), st_));
     AdjArgs<T> a;
     a.g = g_.kg;
     a.nmain = g_.nmain;
@@ -1510,47 +1516,86 @@ void Engine<T>::adjoint(long nsteps, long every, const double* target, double* l
     a.nscal = nscal;
     a.dcoef = d_dcoef.get();
     a.cscale = (double)n;
-    a.partial = partial.get();
+    a.partial = partial_k.get();
     a.inv_n = 1.0 / (double)n;
     const int ndg = (int)dg.size();
-    for (long seg = nck - 1; seg >= 0; --seg) {
-        const long s0 = seg * every, s1 = std::min(nsteps, s0 + every);
-        HIPCHK(hipMemcpyAsync(S_.get(), ck[seg].get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
-        HIPCHK(hipMemcpyAsync(mean_.get(), ckmean.get() + seg, sizeof(double), hipMemcpyDeviceToDevice, st_));
-        prime();
-        for (long s = s0; s < s1; ++s) {
-            HIPCHK(hipMemcpyAsync(SS[s - s0].get(), S_.get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
-            HIPCHK(hipMemcpyAsync(segmean.get() + (s - s0), mean_.get(), sizeof(double), hipMemcpyDeviceToDevice, st_));
-            c_out_ = need_real ? R[s - s0].get() : nullptr;
+    // one segment's recompute and reverse sweep; device pointers depend only on the step's
+    // offset in its segment, so a full segment can be captured once and replayed (copies
+    // are kernels, nothing is allocated)
+    auto copy_k = [&](cpx<T>* dst, const cpx<T>* src) {
+        HIPCHK(launch_copy<T>(reinterpret_cast<const T*>(src), reinterpret_cast<T*>(dst), 2 * nk, st_));
+    };
+    auto recompute = [&](long len) {
+        for (long k = 0; k < len; ++k) {
+            copy_k(SS[k].get(), S_.get());
+            if (split_mean_) HIPCHK(launch_copy<double>(mean_.get(), segmean.get() + k, 1, st_));
+            c_out_ = need_real ? R[k].get() : nullptr;
             fused_step();
             c_out_ = nullptr;
         }
-        HIPCHK(hipMemcpyAsync(SS[s1 - s0].get(), S_.get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
-        HIPCHK(hipMemcpyAsync(segmean.get() + (s1 - s0), mean_.get(), sizeof(double), hipMemcpyDeviceToDevice, st_));
-        for (long s = s1 - 1; s >= s0; --s) {
-            const T* cn = need_real ? R[s - s0].get() : nullptr;
+        copy_k(SS[len].get(), S_.get());
+        if (split_mean_) HIPCHK(launch_copy<double>(mean_.get(), segmean.get() + len, 1, st_));
+    };
+    auto reverse = [&](long len) {
+        for (long k = len - 1; k >= 0; --k) {
+            const T* cn = need_real ? R[k].get() : nullptr;
             r2c({lam.get()}, {Lk.get()}, T(1));
             if (nscal > 0 && (need_c1 || need_c0)) {
-                // derivatives of D and E, against the transforms of c_{n+1} and c_n
-                a.C1 = need_c1 ? SS[s + 1 - s0].get() : nullptr;
-                a.C0 = need_c0 ? SS[s - s0].get() : nullptr;
-                a.mean1 = split_mean_ ? segmean.get() + (s + 1 - s0) : nullptr;
-                a.mean0 = split_mean_ ? segmean.get() + (s - s0) : nullptr;
+                // P, Q and the derivatives of D and E against the transforms of c_{n+1} and c_n
+                a.C1 = need_c1 ? SS[k + 1].get() : nullptr;
+                a.C0 = need_c0 ? SS[k].get() : nullptr;
+                a.mean1 = split_mean_ ? segmean.get() + (k + 1) : nullptr;
+                a.mean0 = split_mean_ ? segmean.get() + k : nullptr;
                 HIPCHK(launch_adj_grad<T>(a, nblk, st_));
-                HIPCHK(launch_accumulate(partial.get(), nscal, nblk, acc.get(), st_));
+            } else {
+                HIPCHK(launch_adj_mult<T>(a, st_));
             }
-            HIPCHK(launch_adj_mult<T>(a, st_));
             c2r({Pk.get(), Qk.get()}, {pr.get(), qr.get()}, T(1) / (T)n);
             if (need_real) {
                 // lam_n = p + g'(c_n) q, and the source derivatives <q, dg/ds(c_n)> in real space
                 HIPCHK(launch_adj_lambda<T>(pr.get(), qr.get(), cn, ndg, d_dg.get(), lam.get(), n, any_g ? nscal : 0,
-                                            d_dpoly.get(), partial.get(), nblk_r, st_));
-                if (any_g) HIPCHK(launch_accumulate(partial.get(), nscal, nblk_r, acc.get(), st_));
+                                            d_dpoly.get(), partial_r.get(), nblk_r, st_));
             } else {
-                HIPCHK(hipMemcpyAsync(lam.get(), pr.get(), n * sizeof(T), hipMemcpyDeviceToDevice, st_));
+                HIPCHK(launch_copy<T>(pr.get(), lam.get(), n, st_));
             }
         }
+    };
+    auto capture = [&](auto&& body) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIPCHK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+        body();
+        HIPCHK(hipStreamEndCapture(st_, &g));
+        const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(e);
+        return ge;
+    };
+    hipGraphExec_t g_rec = nullptr, g_rev = nullptr;
+    bool warm = false;  // the first segment runs eagerly: it creates every lazily built table
+    for (long seg = nck - 1; seg >= 0; --seg) {
+        const long s0 = seg * every, s1 = std::min(nsteps, s0 + every), len = s1 - s0;
+        HIPCHK(hipMemcpyAsync(S_.get(), ck[seg].get(), nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+        HIPCHK(hipMemcpyAsync(mean_.get(), ckmean.get() + seg, sizeof(double), hipMemcpyDeviceToDevice, st_));
+        prime();
+        if (warm && len == every && len >= 2) {
+            if (!g_rec) g_rec = capture([&] { recompute(len); });
+            HIPCHK(hipGraphLaunch(g_rec, st_));
+            if (!g_rev) g_rev = capture([&] { reverse(len); });
+            HIPCHK(hipGraphLaunch(g_rev, st_));
+        } else {
+            recompute(len);
+            reverse(len);
+        }
+        warm = true;
     }
+    if (nscal > 0) {
+        HIPCHK(launch_accumulate(partial_k.get(), nscal, nblk, acc.get(), st_));
+        HIPCHK(launch_accumulate(partial_r.get(), nscal, nblk_r, acc.get(), st_));
+    }
+    HIPCHK(hipStreamSynchronize(st_));
+    if (g_rec) (void)hipGraphExecDestroy(g_rec);
+    if (g_rev) (void)hipGraphExecDestroy(g_rev);
     // outputs; the solver is left at c_T
     std::vector<T> hl(n);
     HIPCHK(hipMemcpyAsync(hl.data(), lam.get(), n * sizeof(T), hipMemcpyDeviceToHost, st_));
@@ -1564,6 +1609,7 @@ void Engine<T>::adjoint(long nsteps, long every, const double* target, double* l
         for (long i = 0; i < n; ++i) grad_field[i] = (double)hl[i];
     for (int s = 0; s < nscal; ++s) grad_scalars[s] = hs[s];
     mon_ = mon;
+    timing_ = timing;
 }
 
 // ---------------------------------------------------------------------------

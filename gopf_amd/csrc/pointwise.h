@@ -103,7 +103,8 @@ struct AdjArgs {
     const cpx<T>* lam = nullptr;
     cpx<T>* P = nullptr;
     cpx<T>* Q = nullptr;
-    // k_adj_grad: per scalar s, sum_k w Re(conj(Lam) [dt dD_s C1 / (1 - dt D) + A dt dE_s C0]) / N, with
+    // k_adj_grad: P and Q as k_adj_mult, and per scalar s the block sums of
+    // w Re(conj(Lam) [dt dD_s C1 / (1 - dt D) + A dt dE_s C0]) / N added to partial[s][block], with
     // dD_s = sum_t dcoef[s][0][t] L^denom.pow[t], dE_s = sum_t dcoef[s][1][t] L^lin.pow[t] and the
     // transforms C1 = FFT(c_{n+1}), C0 = FFT(c_n) given as state spectra: C = cscale S, plus
     // N mean at k = 0 (mean1 / mean0 may be null)
@@ -114,7 +115,7 @@ struct AdjArgs {
     const double* mean1 = nullptr;
     const double* mean0 = nullptr;
     double cscale = 1.0;            // N
-    double* partial = nullptr;      // [nscal][nblocks]
+    double* partial = nullptr;      // [nscal][nblocks], accumulated across launches
     double inv_n = 1.0;
 };
 
@@ -122,7 +123,7 @@ template <typename T> hipError_t launch_adj_mult(const AdjArgs<T>& a, hipStream_
 template <typename T> hipError_t launch_adj_grad(const AdjArgs<T>& a, long nblocks, hipStream_t s);
 // out = p + h(c) q with h(c) = sum_k dpoly[k] c^k; a null p reads as 0 and a null q as 1
 // (the adjoint's lam_n = p + g'(c_n) q, and plain polynomial evaluation).  With nscal > 0
-// it also forms, per scalar s, the block sums of q(x) sum_k spoly[s][k] c(x)^k into
+// it also adds, per scalar s, the block sums of q(x) sum_k spoly[s][k] c(x)^k to
 // partial[s][block] (spoly: device [nscal][kMaxPoly]); launched with nblocks blocks.
 template <typename T> hipError_t launch_adj_lambda(const T* p, const T* q, const T* c, int npoly, const double* dpoly,
                                                    T* out, long n, int nscal, const double* spoly, double* partial,

@@ -399,6 +399,8 @@ __global__ void __launch_bounds__(256) k_adj_grad(AdjArgs<T> a) {
         double L, A, Ainv, m1, m2, w;
         adj_coeffs(a.g, i, a.nmain, a.dt, a.denom, a.lin, a.src_lap, a.filter, L, A, Ainv, m1, m2, w);
         const cpx<T> l = a.lam[i];
+        a.P[i] = mk<T>((T)(m1 * (double)l.x), (T)(m1 * (double)l.y));
+        a.Q[i] = mk<T>((T)(m2 * (double)l.x), (T)(m2 * (double)l.y));
         double c1x = 0.0, c1y = 0.0, c0x = 0.0, c0y = 0.0;
         if (a.C1) {
             c1x = a.cscale * (double)a.C1[i].x;
@@ -427,7 +429,7 @@ __global__ void __launch_bounds__(256) k_adj_grad(AdjArgs<T> a) {
     for (int s = 0; s < kMaxAdjScalars; ++s) {
         if (s >= a.nscal) break;  // uniform over the block
         const double tot = block_sum(acc[s], red);
-        if (threadIdx.x == 0) a.partial[(long)s * gridDim.x + blockIdx.x] = tot * a.inv_n;
+        if (threadIdx.x == 0) a.partial[(long)s * gridDim.x + blockIdx.x] += tot * a.inv_n;
     }
 }
 
@@ -456,7 +458,7 @@ __global__ void __launch_bounds__(256) k_adj_lambda(const T* p, const T* q, cons
     for (int s = 0; s < kMaxAdjScalars; ++s) {
         if (s >= nscal) break;  // uniform over the block
         const double tot = block_sum(acc[s], red);
-        if (threadIdx.x == 0) partial[(long)s * gridDim.x + blockIdx.x] = tot;
+        if (threadIdx.x == 0) partial[(long)s * gridDim.x + blockIdx.x] += tot;
     }
 }
 
