@@ -974,7 +974,7 @@ void Engine<T>::autotune() {
             auto best_pencil = [&](int dir, int& tw_out, int& lpt_out, int& var_out) {
                 float bt = 1e30f;
                 for (int var = 0; var < 2; ++var)
-                    for (int lpt = 1; lpt <= 2; ++lpt)
+                    for (int lpt = 1; lpt <= (sizeof(T) == 4 ? 2 : 1); ++lpt)
                         for (int tw : cand) {
                             if (!tw_fits(no, tw, sizeof(cpx<T>), var)) continue;
                             const float t = time_it([&] { pencils_tw({Wb.get()}, other_axis(), dir, tw, lpt, var); });
