@@ -358,3 +358,28 @@ def test_graph_replayed_steps_equal_single_steps(precision):
     assert np.array_equal(out[0][1], out[1][1]) and out[0][1].size == 13
     assert np.array_equal(out[0][2], out[1][2])
     assert out[0][3] == out[1][3]
+
+
+@pytest.mark.parametrize("dims,steps", [([7, 16], 30), ([24, 96], 30), ([12, 24], 30), ([10, 20], 30),
+                                        ([96, 192], 10), ([24, 24, 24], 10), ([12, 12, 12], 10),
+                                        ([48, 48, 48], 5)])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fused_ch_sizes_match_oracle(oracle_lib, dims, steps, precision):
+    """The fused step on row counts that leave an unpaired row, radix-3 lengths, lengths
+    transformed by the brute-force DFT, and 3D cubes: FP64 within 1e-10 max relative,
+    FP32-bulk within 1e-5 relative L2, of the oracle."""
+    n = int(np.prod(dims))
+    c0 = ch_init(n, 7)
+    pm, om = pair_models({"conc": c0}, CH_SCALARS, [CH_EQ], n)
+    print("fused sizes", dims, steps, precision, flush=True)
+    s = G.Solver(pm, dims, 0.05, precision=precision, autotune=True)
+    assert s.fused
+    s.StepperPropagate(steps)
+    from oracle.engine import OracleSolver
+    o = OracleSolver(om, dims, {"conc": c0}, 0.05)
+    o.stepper_propagate(steps)
+    got, ref = s.Field(0).real, o.field("conc").real
+    if precision == "fp64":
+        assert maxrel(got, ref) <= 1e-10
+    else:
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-5
