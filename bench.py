@@ -339,6 +339,68 @@ def run_c5(G, W, args):
     return out
 
 
+def run_ie(G, W, args):
+    """ImplicitEuler on the 128^2 CH workload (dt 0.1, M = kappa = 1, seed 0) with the
+    reference's default NewtonKrylov settings (Maxiter 50, StepSize 1e-3, Tol 1e-7, Stencil 6)."""
+    N, dt, steps = 128, 0.1, args.steps
+    dims = [N, N]
+    n = N * N
+    c0 = W.ch_initial(0, n)
+    m = G.Model()
+    m.AddField("conc", c0)
+    for k, v in W.ch_scalars(1.0, 1.0).items():
+        m.AddScalar(k, v)
+    m.AddEquation(W.CH_EQUATION)
+    m.Init()
+    s = G.Solver(m, dims, dt, stepper="implicitEuler")
+    s.Propagate(args.warmup)
+    s.SetField(0, c0)
+    s.Synchronize()
+    newton, krylov, conv = [], [], 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.Propagate(1)
+        st = s.NewtonStats()
+        newton.append(st["newton_iters"])
+        krylov.append(st["krylov_iters"])
+        conv += int(st["converged"])
+    s.Synchronize()
+    el = time.perf_counter() - t0
+    # residual evaluations: one per Newton check, Stencil per Arnoldi step
+    evals = sum(newton) + steps + 6 * sum(krylov)
+    print(f"# ie: {1e3 * el / steps:.2f} ms/step, {evals / steps:.1f} residual evaluations/step",
+          file=sys.stderr, flush=True)
+    out = {"metric": "steps/s, 128^2 CH FP64 ImplicitEuler (Newton-Krylov, GMRES on the device)",
+           "value": steps / el, "unit": "steps/s", "ms_per_step": 1e3 * el / steps, "dtype": "f64",
+           "newton_iters_per_step": float(np.mean(newton)), "krylov_iters_per_step": float(np.mean(krylov)),
+           "residual_evals_per_step": evals / steps, "us_per_residual_eval": 1e6 * el / evals,
+           "converged_steps": conv,
+           "config": {"workload": "implicit_euler_ch_2d_128", "grid": dims, "dt": dt, "steps": steps,
+                      "newton_krylov": {"Maxiter": 50, "StepSize": 1e-3, "Tol": 1e-7, "Stencil": 6, "Restart": 30},
+                      "init": "0.5 + U(-0.05, 0.05), SplitMix64 seed 0"}}
+    if not args.no_cpu:
+        from oracle import engine as E
+        from oracle import parser as P
+        from oracle.implicit import ImplicitEulerOracle
+        E.build()
+        om = P.Model()
+        om.add_field("conc", n)
+        for k, v in W.ch_scalars(1.0, 1.0).items():
+            om.add_scalar(k, v)
+        om.add_equation(W.CH_EQUATION)
+        om.init()
+        o = ImplicitEulerOracle(om, dims, {"conc": c0.astype(complex)}, dt, f_tol=1e-7)
+        k = 5
+        t0 = time.perf_counter()
+        for _ in range(k):
+            o.step()
+        oel = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": k / oel, "unit": "steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{k} ImplicitEuler steps of the same sim by the oracle (C oracle "
+                                         f"bricks, numpy FFTs, SciPy Newton-Krylov to max|F| < 1e-7), {oel:.1f} s"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -349,9 +411,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle timing")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--err-steps", type=int, default=1000, help="steps of the FP32-vs-FP64 accuracy run")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "ie"],
                     help="c2 (default): 256^3 CH headline; c1 128^2 CH FP64 with energy series; c3 batched "
-                         "sweep; c4 192^3 elastic RK4; c5 512^2 adjoint")
+                         "sweep; c4 192^3 elastic RK4; c5 512^2 adjoint; ie 128^2 CH ImplicitEuler")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the multi-GPU slab decomposition is "
@@ -361,10 +423,10 @@ def main() -> int:
     from gopf_amd import workloads as W
 
     if args.config != "c2":
-        defaults = {"c1": 1000, "c3": 500, "c4": 500, "c5": 2000}
+        defaults = {"c1": 1000, "c3": 500, "c4": 500, "c5": 2000, "ie": 50}
         if "--steps" not in sys.argv:
             args.steps = defaults[args.config]
-        out = {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.config](G, W, args)
+        out = {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5, "ie": run_ie}[args.config](G, W, args)
         out.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
                     "data": "synthetic", "vs_baseline": None})
         print(json.dumps(out))

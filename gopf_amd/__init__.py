@@ -24,7 +24,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgopf_amd.so")
 
 GPF_FP64, GPF_FP32 = 0, 1
-GPF_EULER, GPF_RK4 = 0, 1
+GPF_EULER, GPF_RK4, GPF_IMPLICIT_EULER = 0, 1, 2
 KIND_CONCRETE, KIND_SQGRAD, KIND_ELAST = 0, 1, 2
 
 
@@ -99,6 +99,8 @@ def _declare(L) -> None:
         "gpf_solver_stream": (_vp, [_vp]),
         "gpf_solver_tuning": (C.c_int, [_vp, _ip, C.c_int]),
         "gpf_solver_set_filter": (C.c_int, [_vp, _dp, C.c_long]),
+        "gpf_solver_set_newton_krylov": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int]),
+        "gpf_solver_newton_stats": (C.c_int, [_vp, _ip, _ip, _ip, _dp]),
         "gpf_solver_adjoint": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_long, _dp, _dp, C.c_int,
                                          C.POINTER(C.c_char_p), _dp]),
         "gpf_device_copy_bandwidth": (C.c_int, [C.c_long, C.c_int, C.c_int, _dp]),
@@ -316,9 +318,10 @@ class Solver:
         o = _Options()
         lib().gpf_options_default(C.byref(o))
         o.precision = GPF_FP32 if precision in ("fp32", "float32") else GPF_FP64
-        if stepper not in ("euler", "rk4"):
+        steppers = {"euler": GPF_EULER, "rk4": GPF_RK4, "implicitEuler": GPF_IMPLICIT_EULER}
+        if stepper not in steppers:
             raise GpfError(1, "Unknown stepper scheme")
-        o.stepper = GPF_RK4 if stepper == "rk4" else GPF_EULER
+        o.stepper = steppers[stepper]
         o.filter_order = int(filter_order)
         o.fused = 1 if fused else 0
         o.device = int(device)
@@ -428,6 +431,20 @@ class Solver:
             return
         tab = np.ascontiguousarray(np.asarray(filt.Data, dtype=np.float64))
         _check(lib().gpf_solver_set_filter(self._h, _dptr(tab), tab.size))
+
+    def SetNewtonKrylov(self, Maxiter: int = 50, StepSize: float = 1e-3, Tol: float = 1e-7, Stencil: int = 6,
+                        Restart: int = 30) -> None:
+        """ImplicitEuler.NonlinSolver = &nonlin.NewtonKrylov{...} (pf/implicitEuler.go:26-28, 221-229);
+        Restart is the GMRES restart length."""
+        _check(lib().gpf_solver_set_newton_krylov(self._h, int(Maxiter), float(StepSize), float(Tol), int(Stencil),
+                                                  int(Restart)))
+
+    def NewtonStats(self) -> Dict[str, float]:
+        """Newton and GMRES iterations, convergence and max-norm residual of the last ImplicitEuler step."""
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        r = C.c_double()
+        _check(lib().gpf_solver_newton_stats(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(r)))
+        return {"newton_iters": a.value, "krylov_iters": b.value, "converged": bool(c.value), "residual": r.value}
 
     def Adjoint(self, nsteps: int, target, scalars: Sequence[str] = (), checkpoint_every: int = 100
                 ) -> Tuple[float, np.ndarray, Dict[str, float]]:

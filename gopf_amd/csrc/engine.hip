@@ -12,21 +12,25 @@
 //    1/nx, so the x-inverse of one step and the x-forward of the next share
 //    one pass.
 //
-//  * general path (any number of fields, Euler or RK4, ConcreteTerms on
-//    fields / derived fields / constants, SquaredGradient,
+//  * general path (any number of fields, Euler, RK4 or ImplicitEuler,
+//    ConcreteTerms on fields / derived fields / constants, SquaredGradient,
 //    HomogeneousModulusLinElast, Vandeven filter): derived fields in real
 //    space, R2C / C2R transforms, and one k-space update kernel per field in
 //    the reference's sequential order (euler.go:27-39, rk4.go:29-127).
+//    ImplicitEuler solves its step's residual equation with a Newton-Krylov
+//    iteration whose vectors stay on the device (implicitEuler.go:68-208).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <complex>
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -34,6 +38,7 @@
 #include "batch.h"
 #include "common.h"
 #include "fft_core.h"
+#include "krylov.h"
 #include "model.h"
 #include "pointwise.h"
 #include "spectral.h"
@@ -341,6 +346,8 @@ public:
     virtual void set_filter(const double* table) = 0;
     virtual void adjoint(long nsteps, long every, const double* target, double* loss, double* grad_field,
                          int nscal, const char* const* names, double* grad_scalars) = 0;
+    virtual void set_newton_krylov(int maxiter, double step_size, double tol, int stencil, int restart) = 0;
+    virtual void newton_stats(int* newton_iters, int* krylov_iters, int* converged, double* residual) const = 0;
 };
 
 template <typename T>
@@ -366,6 +373,8 @@ public:
     void set_filter(const double* table) override;
     void adjoint(long nsteps, long every, const double* target, double* loss, double* grad_field, int nscal,
                  const char* const* names, double* grad_scalars) override;
+    void set_newton_krylov(int maxiter, double step_size, double tol, int stencil, int restart) override;
+    void newton_stats(int* newton_iters, int* krylov_iters, int* converged, double* residual) const override;
 
 private:
     // transforms
@@ -413,6 +422,29 @@ private:
     void eval_elastic();
     void rhs_update(int eq, cpx<T>* dst, const cpx<T>* cur, double alpha, double beta, double gamma, bool filt,
                     const cpx<T>* extra);
+    template <class A>
+    void terms_into(int eq, A& a);
+    // ImplicitEuler (FP64): the residual F(x) of the exponential-integrator step, its
+    // finite-difference directional derivative and a restarted GMRES solve of J dx = -F
+    void implicit_step();
+    void implicit_residual(const double* x, double* out);
+    void implicit_jv(const double* v, double* out);
+    int implicit_gmres();
+    double dev_scalar(const double* p);
+    struct NewtonKrylov {
+        int maxiter = 50;       // DefaultNonLinSolver (implicitEuler.go:221-229)
+        double step = 1e-3;
+        double tol = 1e-7;
+        int stencil = 6;
+        int restart = 30;       // GMRES restart length (device choice)
+    } nk_;
+    int nk_newton_ = 0, nk_krylov_ = 0, nk_conv_ = 0;
+    double nk_res_ = 0.0;
+    std::vector<Buf<cpx<T>>> ORIG_, RP_, RES_;
+    // Newton-Krylov vectors of length nf n: iterate, residual, update, perturbed point,
+    // residual there, Arnoldi work vector, GMRES basis (restart + 1 vectors)
+    Buf<double> nx_, nfx_, ndx_, nxp_, nfp_, nw_, nv_, npart_, ndev_;
+    std::vector<double> nhost_;
 
     Geometry g_;
     hipStream_t st_ = nullptr;
@@ -529,6 +561,23 @@ Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gp
                 FINAL_.emplace_back(g_.nk);
                 KF_.emplace_back(g_.nk);
             }
+        if (stepper_ == GPF_IMPLICIT_EULER) {
+            if (sizeof(T) != sizeof(double)) throw Error(GPF_EINVAL, "ImplicitEuler runs in FP64 (GPF_FP64)");
+            for (int i = 0; i < nf_; ++i) {
+                ORIG_.emplace_back(g_.nk);
+                RP_.emplace_back(g_.nk);
+                RES_.emplace_back(g_.nk);
+            }
+            const long M = (long)nf_ * g_.n;
+            nx_.alloc(M);
+            nfx_.alloc(M);
+            ndx_.alloc(M);
+            nxp_.alloc(M);
+            nfp_.alloc(M);
+            nw_.alloc(M);
+            npart_.alloc(krylov_blocks(M) * (kMaxKrylov + 1));
+            ndev_.alloc(4 * (kMaxKrylov + 1));
+        }
         // derived-field programs
         for (int d = 0; d < nd_; ++d) {
             const DerivedDesc& dd = m.derived[d];
@@ -1253,16 +1302,24 @@ void Engine<T>::rhs_update(int eq, cpx<T>* dst, const cpx<T>* cur, double alpha,
         a.pw[0] = 0;
         a.nterm = 1;
     } else {
-        for (const auto& t : eqs_[eq].terms) {
-            const int q = a.nterm++;
-            a.coef[q] = t.coef;
-            a.pw[q] = t.pw;
-            if (t.kind == kConcrete) a.brick[q] = (t.brick >= 0) ? K_[t.brick].get() : nullptr;
-            else if (t.kind == kSquaredGradient) a.brick[q] = Tsq_[t.user].get();
-            else a.brick[q] = Tel_[t.user].get();
-        }
+        terms_into(eq, a);
     }
     HIPCHK(launch_kupdate<T>(a, st_));
+}
+
+// the explicit terms of equation eq (GetRHS, model.go:283-303) as k-space bricks
+template <typename T>
+template <class A>
+void Engine<T>::terms_into(int eq, A& a) {
+    a.nterm = 0;
+    for (const auto& t : eqs_[eq].terms) {
+        const int q = a.nterm++;
+        a.coef[q] = t.coef;
+        a.pw[q] = t.pw;
+        if (t.kind == kConcrete) a.brick[q] = (t.brick >= 0) ? K_[t.brick].get() : nullptr;
+        else if (t.kind == kSquaredGradient) a.brick[q] = Tsq_[t.user].get();
+        else a.brick[q] = Tel_[t.user].get();
+    }
 }
 
 template <typename T>
@@ -1357,6 +1414,271 @@ void Engine<T>::general_step_rk4() {
     c2r(in, out, T(1) / (T)g_.n);
 }
 
+// ---------------------------------------------------------------------------
+// ImplicitEuler (implicitEuler.go:68-229)
+// ---------------------------------------------------------------------------
+// Step: orig^ = FFT(c_n) and R^prev = GetRHS at c_n; an explicit Euler step is the initial
+// guess; then x = c_{n+1} solves F(x) = 0 with
+//   F(x) = Re IFFT(x^ - (orig^ e^{D dt} + I(D, R^(x), R^prev))) / N,
+// by Newton's method: each update dx solves J dx = -F(x) with restarted GMRES, J v taken
+// as a central difference of F along v (the NewtonKrylov StepSize and Stencil).  The
+// Krylov basis lives in HBM; per Arnoldi step the host reads back only the Gram-Schmidt
+// coefficients and the new norm (FP64, fixed summation order) for the Givens updates.
+template <typename T>
+double Engine<T>::dev_scalar(const double* p) {
+    double v = 0.0;
+    HIPCHK(hipMemcpyAsync(&v, p, sizeof(double), hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+    return v;
+}
+
+template <typename T>
+void Engine<T>::implicit_residual(const double* x, double* out) {
+    if constexpr (std::is_same<T, double>::value) {
+        const long n = g_.n;
+        std::vector<const T*> in;
+        std::vector<cpx<T>*> ko;
+        for (int i = 0; i < nf_; ++i) {
+            HIPCHK(launch_copy<T>(x + i * n, F_[i].get(), n, st_));
+            in.push_back(F_[i].get());
+            ko.push_back(K_[i].get());
+        }
+        r2c(in, ko, T(1));
+        sync_derived();
+        std::vector<const cpx<T>*> rin;
+        std::vector<T*> rout;
+        for (int i = 0; i < nf_; ++i) {
+            eval_user_terms_for(i);
+            EtdArgs<T> a;
+            terms_into(i, a);
+            a.dst = RES_[i].get();
+            a.cur = K_[i].get();
+            a.orig = ORIG_[i].get();
+            a.rprev = RP_[i].get();
+            a.denom = eqs_[i].denom;
+            a.dt = dt_;
+            a.g = g_.kg;
+            a.nmain = g_.nmain;
+            a.nk = g_.nk;
+            HIPCHK(launch_etd_residual<T>(a, st_));
+            rin.push_back(RES_[i].get());
+            rout.push_back(out + i * n);
+        }
+        c2r(rin, rout, T(1) / (T)n);
+    } else {
+        (void)x;
+        (void)out;
+        throw Error(GPF_EINVAL, "ImplicitEuler runs in FP64 (GPF_FP64)");
+    }
+}
+
+// central difference weights of the first derivative on 2, 4, 6, 8 points (Fornberg)
+struct FdStencil {
+    int n;
+    int off[8];
+    double w[8];
+};
+static const FdStencil kFdStencils[4] = {
+    {2, {-1, 1}, {-0.5, 0.5}},
+    {4, {-2, -1, 1, 2}, {1.0 / 12.0, -2.0 / 3.0, 2.0 / 3.0, -1.0 / 12.0}},
+    {6, {-3, -2, -1, 1, 2, 3}, {-1.0 / 60.0, 3.0 / 20.0, -3.0 / 4.0, 3.0 / 4.0, -3.0 / 20.0, 1.0 / 60.0}},
+    {8, {-4, -3, -2, -1, 1, 2, 3, 4},
+     {1.0 / 280.0, -4.0 / 105.0, 1.0 / 5.0, -4.0 / 5.0, 4.0 / 5.0, -1.0 / 5.0, 4.0 / 105.0, -1.0 / 280.0}},
+};
+
+// out = J(x) v for the current iterate x (in nx_, with F(x) in nfx_)
+template <typename T>
+void Engine<T>::implicit_jv(const double* v, double* out) {
+    const long M = (long)nf_ * g_.n;
+    const double h = nk_.step;
+    if (nk_.stencil == 1) {
+        HIPCHK(launch_axpby(1.0, nx_.get(), h, v, nxp_.get(), M, st_));
+        implicit_residual(nxp_.get(), nfp_.get());
+        HIPCHK(launch_axpby(1.0 / h, nfp_.get(), -1.0 / h, nfx_.get(), out, M, st_));
+        return;
+    }
+    const FdStencil& fd = kFdStencils[nk_.stencil / 2 - 1];
+    for (int k = 0; k < fd.n; ++k) {
+        HIPCHK(launch_axpby(1.0, nx_.get(), fd.off[k] * h, v, nxp_.get(), M, st_));
+        implicit_residual(nxp_.get(), nfp_.get());
+        HIPCHK(launch_axpby(1.0, k == 0 ? nullptr : out, fd.w[k] / h, nfp_.get(), out, M, st_));
+    }
+}
+
+// restarted GMRES for J dx = -F(x), dx = 0 initially; returns the Arnoldi steps taken
+template <typename T>
+int Engine<T>::implicit_gmres() {
+    const long M = (long)nf_ * g_.n;
+    const int m = nk_.restart;
+    const long nb = krylov_blocks(M);
+    constexpr int kStride = kMaxKrylov + 1;
+    double* V = nv_.get();
+    double* dev = ndev_.get();  // [0, kStride) CGS pass 1, [kStride, 2 kStride) pass 2, [2 kStride] |w|^2,
+                                // [3 kStride, 4 kStride) solution coefficients
+    HIPCHK(hipMemsetAsync(ndx_.get(), 0, M * sizeof(double), st_));
+    HIPCHK(launch_axpby(0.0, nullptr, -1.0, nfx_.get(), nw_.get(), M, st_));
+    HIPCHK(launch_multidot(nw_.get(), M, 1, nw_.get(), M, npart_.get(), st_));
+    HIPCHK(launch_reduce_rows(npart_.get(), 1, nb, dev + 2 * kStride, st_));
+    HIPCHK(launch_normalize(nw_.get(), dev + 2 * kStride, V, M, st_));
+    double beta = std::sqrt(std::max(dev_scalar(dev + 2 * kStride), 0.0));
+    if (!(beta > 0.0) || !std::isfinite(beta)) return 0;
+    // the linear model needs no residual far below the Newton tolerance (max norm <= 2-norm)
+    const double target = std::max(1e-10 * beta, 0.1 * nk_.tol);
+    std::vector<double> H((size_t)(m + 1) * m), Hu((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m + 1);
+    auto at = [m](std::vector<double>& A, int i, int j) -> double& { return A[(size_t)i * m + j]; };
+    nhost_.assign(2 * kStride + 1, 0.0);
+    int total = 0;
+    const int max_cycles = 10;
+    for (int cycle = 0; cycle < max_cycles; ++cycle) {
+        std::fill(H.begin(), H.end(), 0.0);
+        std::fill(Hu.begin(), Hu.end(), 0.0);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int jend = 0;
+        bool done = false;
+        for (int j = 0; j < m; ++j) {
+            double* w = nw_.get();
+            implicit_jv(V + (long)j * M, w);
+            // classical Gram-Schmidt, twice
+            HIPCHK(launch_multidot(V, M, j + 1, w, M, npart_.get(), st_));
+            HIPCHK(launch_reduce_rows(npart_.get(), j + 1, nb, dev, st_));
+            HIPCHK(launch_combine(V, M, j + 1, dev, -1.0, w, M, st_));
+            HIPCHK(launch_multidot(V, M, j + 1, w, M, npart_.get(), st_));
+            HIPCHK(launch_reduce_rows(npart_.get(), j + 1, nb, dev + kStride, st_));
+            HIPCHK(launch_combine(V, M, j + 1, dev + kStride, -1.0, w, M, st_));
+            HIPCHK(launch_multidot(w, M, 1, w, M, npart_.get(), st_));
+            HIPCHK(launch_reduce_rows(npart_.get(), 1, nb, dev + 2 * kStride, st_));
+            HIPCHK(launch_normalize(w, dev + 2 * kStride, V + (long)(j + 1) * M, M, st_));
+            HIPCHK(hipMemcpyAsync(nhost_.data(), dev, (2 * kStride + 1) * sizeof(double), hipMemcpyDeviceToHost,
+                                  st_));
+            HIPCHK(hipStreamSynchronize(st_));
+            for (int i = 0; i <= j; ++i) at(Hu, i, j) = at(H, i, j) = nhost_[i] + nhost_[kStride + i];
+            const double hn = std::sqrt(std::max(nhost_[2 * kStride], 0.0));
+            at(Hu, j + 1, j) = at(H, j + 1, j) = hn;
+            // Givens rotations of the least-squares problem min |g - H y|
+            for (int i = 0; i < j; ++i) {
+                const double a = at(H, i, j), b = at(H, i + 1, j);
+                at(H, i, j) = cs[i] * a + sn[i] * b;
+                at(H, i + 1, j) = -sn[i] * a + cs[i] * b;
+            }
+            const double a = at(H, j, j), b = at(H, j + 1, j);
+            const double r = std::hypot(a, b);
+            cs[j] = (r > 0.0) ? a / r : 1.0;
+            sn[j] = (r > 0.0) ? b / r : 0.0;
+            at(H, j, j) = r;
+            at(H, j + 1, j) = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            ++total;
+            jend = j + 1;
+            if (!std::isfinite(g[j + 1])) return total;
+            if (std::fabs(g[j + 1]) <= target || !(hn > 1e-14 * beta)) {
+                done = true;
+                break;
+            }
+        }
+        // back substitution for y, then dx += V y
+        for (int i = jend - 1; i >= 0; --i) {
+            double t = g[i];
+            for (int k = i + 1; k < jend; ++k) t -= at(H, i, k) * y[k];
+            y[i] = (at(H, i, i) != 0.0) ? t / at(H, i, i) : 0.0;
+        }
+        HIPCHK(hipMemcpyAsync(dev + 3 * kStride, y.data(), jend * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPCHK(launch_combine(V, M, jend, dev + 3 * kStride, 1.0, ndx_.get(), M, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+        if (done || cycle + 1 == max_cycles) break;
+        // restart from the Arnoldi residual r = V_{m+1} (beta e1 - Hu y)
+        std::vector<double> z(jend + 1, 0.0);
+        z[0] = beta;
+        for (int i = 0; i <= jend; ++i)
+            for (int k = 0; k < jend; ++k) z[i] -= at(Hu, i, k) * y[k];
+        HIPCHK(hipMemcpyAsync(dev + 3 * kStride, z.data(), (jend + 1) * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPCHK(hipMemsetAsync(nw_.get(), 0, M * sizeof(double), st_));
+        HIPCHK(launch_combine(V, M, jend + 1, dev + 3 * kStride, 1.0, nw_.get(), M, st_));
+        HIPCHK(launch_multidot(nw_.get(), M, 1, nw_.get(), M, npart_.get(), st_));
+        HIPCHK(launch_reduce_rows(npart_.get(), 1, nb, dev + 2 * kStride, st_));
+        HIPCHK(launch_normalize(nw_.get(), dev + 2 * kStride, V, M, st_));
+        beta = std::sqrt(std::max(dev_scalar(dev + 2 * kStride), 0.0));
+        if (!(beta > target) || !std::isfinite(beta)) break;
+    }
+    return total;
+}
+
+template <typename T>
+void Engine<T>::implicit_step() {
+    const long n = g_.n;
+    const long M = (long)nf_ * n;
+    if (!nv_.get() || nv_.size() != (size_t)(nk_.restart + 1) * M) nv_.alloc((size_t)(nk_.restart + 1) * M);
+    // orig^ and R^prev at the current fields (implicitEuler.go:172-179)
+    ensure_k_fields();
+    sync_derived();
+    eval_elastic();  // the lagged elastic field is constant within a step
+    const size_t kb = g_.nk * sizeof(cpx<T>);
+    for (int i = 0; i < nf_; ++i) {
+        eval_user_terms_for(i);
+        rhs_update(i, RP_[i].get(), nullptr, 0.0, 1.0, 0.0, false, nullptr);
+        HIPCHK(hipMemcpyAsync(ORIG_[i].get(), K_[i].get(), kb, hipMemcpyDeviceToDevice, st_));
+    }
+    // explicit Euler step as the initial guess (implicitEuler.go:182-190; euler.go:32-44 with
+    // the stepper's filter), from the right-hand side just formed
+    std::vector<const cpx<T>*> in;
+    std::vector<T*> out;
+    for (int i = 0; i < nf_; ++i) {
+        rhs_update(i, K_[i].get(), K_[i].get(), 1.0, dt_, dt_, true, RP_[i].get());
+        in.push_back(K_[i].get());
+        out.push_back(F_[i].get());
+    }
+    c2r(in, out, T(1) / (T)n);
+    for (int i = 0; i < nf_; ++i)
+        HIPCHK(hipMemcpyAsync(nx_.get() + i * n, F_[i].get(), n * sizeof(T), hipMemcpyDeviceToDevice, st_));
+    // Newton iteration (nonlin.NewtonKrylov.Solve)
+    nk_newton_ = 0;
+    nk_krylov_ = 0;
+    nk_conv_ = 0;
+    double* dev = ndev_.get();
+    for (int it = 0; it < nk_.maxiter; ++it) {
+        implicit_residual(nx_.get(), nfx_.get());
+        HIPCHK(launch_absmax(nfx_.get(), M, npart_.get(), dev, st_));
+        nk_res_ = dev_scalar(dev);
+        if (nk_res_ < nk_.tol) {
+            nk_conv_ = 1;
+            break;
+        }
+        if (!std::isfinite(nk_res_)) break;
+        nk_krylov_ += implicit_gmres();
+        HIPCHK(launch_axpby(1.0, nx_.get(), 1.0, ndx_.get(), nx_.get(), M, st_));
+        ++nk_newton_;
+    }
+    if (!nk_conv_) std::fprintf(stderr, "Warning: Iterative solver did not converge\n");
+    // the fields take the solution (implicitEuler.go:205); after a converged check the
+    // k-space bricks of the fields are FFT(x) already
+    for (int i = 0; i < nf_; ++i)
+        HIPCHK(hipMemcpyAsync(F_[i].get(), nx_.get() + i * n, n * sizeof(T), hipMemcpyDeviceToDevice, st_));
+    kvalid_ = nk_conv_ != 0;
+}
+
+template <typename T>
+void Engine<T>::set_newton_krylov(int maxiter, double step_size, double tol, int stencil, int restart) {
+    if (maxiter < 1) throw Error(GPF_EINVAL, "Maxiter must be positive");
+    if (!(step_size > 0.0) || !(tol > 0.0)) throw Error(GPF_EINVAL, "StepSize and Tol must be positive");
+    if (stencil != 1 && stencil != 2 && stencil != 4 && stencil != 6 && stencil != 8)
+        throw Error(GPF_EINVAL, "Stencil must be 1, 2, 4, 6 or 8");
+    if (restart < 1 || restart > kMaxKrylov) throw Error(GPF_EINVAL, "GMRES restart must be in 1..64");
+    nk_.maxiter = maxiter;
+    nk_.step = step_size;
+    nk_.tol = tol;
+    nk_.stencil = stencil;
+    nk_.restart = restart;
+}
+
+template <typename T>
+void Engine<T>::newton_stats(int* newton_iters, int* krylov_iters, int* converged, double* residual) const {
+    if (newton_iters) *newton_iters = nk_newton_;
+    if (krylov_iters) *krylov_iters = nk_krylov_;
+    if (converged) *converged = nk_conv_;
+    if (residual) *residual = nk_res_;
+}
+
 template <typename T>
 void Engine<T>::step() {
     if (fp_.ok) {
@@ -1365,7 +1687,8 @@ void Engine<T>::step() {
     }
     tbegin(0);
     if (stepper_ == GPF_EULER) general_step_euler();
-    else general_step_rk4();
+    else if (stepper_ == GPF_RK4) general_step_rk4();
+    else implicit_step();
     tend(0);
 }
 
@@ -1899,7 +2222,8 @@ int gpf_solver_create(gpf_model* m, const int* dims, int ndim, double dt, const 
         gpf_options opt;
         gpf_options_default(&opt);
         if (o) opt = *o;
-        if (opt.stepper != GPF_EULER && opt.stepper != GPF_RK4) throw Error(GPF_EINVAL, "Unknown stepper scheme");
+        if (opt.stepper != GPF_EULER && opt.stepper != GPF_RK4 && opt.stepper != GPF_IMPLICIT_EULER)
+            throw Error(GPF_EINVAL, "Unknown stepper scheme");
         set_device(opt.device);
         auto s = std::make_unique<gpf_solver>();
         s->stepper = opt.stepper;
@@ -1933,7 +2257,9 @@ int gpf_solver_propagate(gpf_solver* s, int nsteps) {
         }
         for (int i = 0; i < nsteps; ++i) {
             s->eng->step();
-            if (s->stepper == GPF_EULER) s->current_step++;  // Euler.Step increments (euler.go:46)
+            // Euler.Step and ImplicitEuler.Step increment (euler.go:46, implicitEuler.go:206);
+            // RK4.Step does not (rk4.go:29-74)
+            if (s->stepper != GPF_RK4) s->current_step++;
             s->eng->on_step_finished();
         }
     });
@@ -2070,6 +2396,24 @@ int gpf_solver_tuning(gpf_solver* s, int* tws, int maxn) {
 }
 
 void* gpf_solver_stream(gpf_solver* s) { return (s && s->eng) ? (void*)s->eng->stream() : nullptr; }
+
+int gpf_solver_set_newton_krylov(gpf_solver* s, int maxiter, double step_size, double tol, int stencil,
+                                 int restart) {
+    return guarded([&] {
+        if (!s) throw Error(GPF_EINVAL, "bad argument");
+        if (s->stepper != GPF_IMPLICIT_EULER) throw Error(GPF_EINVAL, "the solver's stepper is not ImplicitEuler");
+        s->eng->set_newton_krylov(maxiter, step_size, tol, stencil, restart);
+    });
+}
+
+int gpf_solver_newton_stats(gpf_solver* s, int* newton_iters, int* krylov_iters, int* converged,
+                            double* residual) {
+    return guarded([&] {
+        if (!s) throw Error(GPF_EINVAL, "bad argument");
+        if (s->stepper != GPF_IMPLICIT_EULER) throw Error(GPF_EINVAL, "the solver's stepper is not ImplicitEuler");
+        s->eng->newton_stats(newton_iters, krylov_iters, converged, residual);
+    });
+}
 
 int gpf_solver_set_filter(gpf_solver* s, const double* table, long n) {
     return guarded([&] {

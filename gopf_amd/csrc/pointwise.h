@@ -65,6 +65,26 @@ struct KUpdateArgs {
     long nmain = 0, nk = 0;            // main elements; total (main + nyq)
 };
 
+// residual of the exponential-integrator step of ImplicitEuler on a half spectrum:
+//   dst = cur - (orig e^{D dt} + I(D, R, rprev)),  R = sum_t coef_t L^pow_t B_t
+// with I the integral over the step of e^{D (dt - s)} times the explicit part interpolated
+// linearly from rprev (s = 0) to R (s = dt); B_t == null means the brick is the constant 1
+template <typename T>
+struct EtdArgs {
+    cpx<T>* dst = nullptr;
+    const cpx<T>* cur = nullptr;
+    const cpx<T>* orig = nullptr;
+    const cpx<T>* rprev = nullptr;
+    int nterm = 0;
+    const cpx<T>* brick[kMaxUTerms] = {};
+    double coef[kMaxUTerms];
+    int pw[kMaxUTerms];
+    KPoly denom;
+    double dt = 0.0;
+    KGrid g;
+    long nmain = 0, nk = 0;
+};
+
 // spectral gradient components: out_d = i 2 pi f_d' src   (f_d' = 0 at f_d = +0.5)
 template <typename T>
 struct KGradArgs {
@@ -133,6 +153,7 @@ hipError_t launch_accumulate(const double* partial, int nscal, long nblocks, dou
 
 template <typename T> hipError_t launch_derived(const DerivedArgs<T>& a, hipStream_t s);
 template <typename T> hipError_t launch_kupdate(const KUpdateArgs<T>& a, hipStream_t s);
+template <typename T> hipError_t launch_etd_residual(const EtdArgs<T>& a, hipStream_t s);
 template <typename T> hipError_t launch_kgrad(const KGradArgs<T>& a, int ndim, hipStream_t s);
 template <typename T> hipError_t launch_kelastic(const KElasticArgs<T>& a, hipStream_t s);
 // out = factor * sum_d w_d^2

@@ -5,6 +5,7 @@
 //   k_kupdate     GetRHS / GetDenum sums of ConcreteTerm closures (rhsBuilder.go:125-190,
 //                 model.go:283-314) fused with the Euler / RK4 stage algebra
 //                 (euler.go:32-38, rk4.go:77-127) and ApplyModalFilter (util.go:125-132)
+//   k_etd_residual  the spectral part of ImplicitEuler's residual (implicitEuler.go:68-94)
 //   k_kgrad       i 2 pi f_d u(k) with f_d = +0.5 zeroed (squareGradientTerm.go:44-50)
 //   k_kelastic    scalar Khachaturyan kernel K(k): force (effectiveForce.go:26-35), 3x3
 //                 displacement solve (linearElasticity.go:33-63), strains
@@ -141,6 +142,49 @@ __global__ void k_kupdate(KUpdateArgs<T> a) {
             cy = (double)c.y;
         }
         a.dst[i] = mk<T>((T)(s * (a.alpha * cx + a.beta * rx)), (T)(s * (a.alpha * cy + a.beta * ry)));
+    }
+}
+
+// ImplicitEuler.updateEquation's spectral part (implicitEuler.go:79-88) with
+// nonlinearIntegral (implicitEuler.go:151-162); D is real for the device's implicit
+// terms (scalar * LAP^p), so e^{D dt} is real
+template <typename T>
+__global__ void k_etd_residual(EtdArgs<T> a) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
+        int kx, ky, kz;
+        kcoords(a.g, i, a.nmain, kx, ky, kz);
+        double f[3];
+        ref_freq(a.g, kx, ky, kz, f);
+        const double L = -(4.0 * kPi * kPi) * (f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        double rx = 0.0, ry = 0.0;
+        for (int t = 0; t < a.nterm; ++t) {
+            const double m = a.coef[t] * ipow(L, a.pw[t]);
+            if (a.brick[t]) {
+                const cpx<T> b = a.brick[t][i];
+                rx += m * (double)b.x;
+                ry += m * (double)b.y;
+            } else {
+                rx += m;
+            }
+        }
+        double D = 0.0;
+        for (int t = 0; t < a.denom.n; ++t) D += a.denom.coef[t] * ipow(L, a.denom.pow[t]);
+        const double e = exp(D * a.dt);
+        const cpx<T> p = a.rprev[i];
+        const double px = (double)p.x, py = (double)p.y;
+        double ix, iy;
+        if (fabs(D) < 1e-5) {
+            ix = 0.5 * a.dt * (rx + px * e);
+            iy = 0.5 * a.dt * (ry + py * e);
+        } else {
+            const double c1 = (e - 1.0) / D;
+            const double c2 = (e - D * a.dt - 1.0) / (D * D);
+            ix = px * c1 + (rx - px) / a.dt * c2;
+            iy = py * c1 + (ry - py) / a.dt * c2;
+        }
+        const cpx<T> o = a.orig[i];
+        const cpx<T> c = a.cur[i];
+        a.dst[i] = mk<T>((T)((double)c.x - ((double)o.x * e + ix)), (T)((double)c.y - ((double)o.y * e + iy)));
     }
 }
 
@@ -492,6 +536,11 @@ hipError_t launch_kupdate(const KUpdateArgs<T>& a, hipStream_t s) {
     return hipGetLastError();
 }
 template <typename T>
+hipError_t launch_etd_residual(const EtdArgs<T>& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_etd_residual<T>, dim3(grid_for(a.nk)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+template <typename T>
 hipError_t launch_kgrad(const KGradArgs<T>& a, int ndim, hipStream_t s) {
     hipLaunchKernelGGL(k_kgrad<T>, dim3(grid_for(a.nk)), dim3(256), 0, s, a, ndim);
     return hipGetLastError();
@@ -604,6 +653,7 @@ hipError_t launch_reduce2(const double* a, long na, double ca, const double* b, 
 #define GPF_INST(T)                                                                                            \
     template hipError_t launch_derived<T>(const DerivedArgs<T>&, hipStream_t);                                \
     template hipError_t launch_kupdate<T>(const KUpdateArgs<T>&, hipStream_t);                                \
+    template hipError_t launch_etd_residual<T>(const EtdArgs<T>&, hipStream_t);                               \
     template hipError_t launch_kgrad<T>(const KGradArgs<T>&, int, hipStream_t);                               \
     template hipError_t launch_kelastic<T>(const KElasticArgs<T>&, hipStream_t);                              \
     template hipError_t launch_sqsum<T>(const T* const*, int, T, T*, long, hipStream_t);                      \

@@ -47,6 +47,7 @@ extern "C" {
 
 #define GPF_EULER 0      /* pf.Euler */
 #define GPF_RK4 1        /* pf.RK4 */
+#define GPF_IMPLICIT_EULER 2  /* pf.ImplicitEuler (FP64 only) */
 
 typedef struct gpf_model gpf_model;
 typedef struct gpf_solver gpf_solver;
@@ -103,7 +104,9 @@ int gpf_model_get_term(const gpf_model* m, int eq, int denum, int idx, gpf_term*
 
 typedef struct {
     int precision;     /* GPF_FP64 or GPF_FP32 */
-    int stepper;       /* GPF_EULER or GPF_RK4 (Solver.SetStepper, pf/solver.go:88-103) */
+    int stepper;       /* GPF_EULER or GPF_RK4 (Solver.SetStepper, pf/solver.go:88-103), or
+                          GPF_IMPLICIT_EULER (solver.Stepper = &pf.ImplicitEuler{...},
+                          pf/implicitEuler.go:20-29) */
     int filter_order;  /* > 0: Stepper.SetFilter(NewVandeven(order)); 0: no filter */
     int fused;         /* 1: allow the fused single-field path; 0: general path only */
     int device;        /* HIP device ordinal */
@@ -154,6 +157,20 @@ void* gpf_solver_stream(gpf_solver* s);
  * 1000-entry table of a pf.Vandeven (pf/vandeven.go:8-40; n must be 1000), evaluated as
  * Vandeven.Eval at |f| 2/pi; table NULL removes the filter */
 int gpf_solver_set_filter(gpf_solver* s, const double* table, long n);
+
+/* The nonlinear solver of an ImplicitEuler stepper: ImplicitEuler.NonlinSolver, a
+ * nonlin.NewtonKrylov (pf/implicitEuler.go:26-28, 221-229; defaults Maxiter 50,
+ * StepSize 1e-3, Tol 1e-7, Stencil 6).  stencil is the number of points of the central
+ * difference that approximates the Jacobian-vector product (2, 4, 6 or 8; 1 selects a
+ * forward difference); restart is the GMRES restart length (1..64, default 30).  A step
+ * has converged when max_i |F_i| < tol. */
+int gpf_solver_set_newton_krylov(gpf_solver* s, int maxiter, double step_size, double tol, int stencil,
+                                 int restart);
+/* Statistics of the last ImplicitEuler step: Newton iterations, GMRES iterations, whether
+ * it converged (the reference logs "Iterative solver did not converge" otherwise,
+ * pf/implicitEuler.go:201-204) and the last max-norm residual.  Any pointer may be NULL. */
+int gpf_solver_newton_stats(gpf_solver* s, int* newton_iters, int* krylov_iters, int* converged,
+                            double* residual);
 
 /* Reverse-mode gradient (no reference counterpart; it replaces nothing in gopf and
  * differentiates the Euler.Step of pf/euler.go:16-47 on the fused single-field class).

@@ -43,8 +43,9 @@ class ImplicitEulerOracle:
     """pf.ImplicitEuler stepping a pf.Model on the C oracle's bricks."""
 
     def __init__(self, model: P.Model, dims: Sequence[int], fields: Dict[str, np.ndarray], dt: float,
-                 f_tol: float = 1e-12) -> None:
-        self.o = OracleSolver(model, dims, fields, dt, stepper="euler")
+                 f_tol: float = 1e-12, filter_order: int = 0) -> None:
+        # the explicit Euler step that makes the initial guess carries the stepper's filter
+        self.o = OracleSolver(model, dims, fields, dt, stepper="euler", filter_order=filter_order)
         self.model = model
         self.dt = dt
         self.n = self.o.n
