@@ -25,6 +25,7 @@ LIB_PATH = os.path.join(_HERE, "libgopf_amd.so")
 
 GPF_FP64, GPF_FP32 = 0, 1
 GPF_EULER, GPF_RK4, GPF_IMPLICIT_EULER = 0, 1, 2
+GPF_IO_FLOAT64, GPF_IO_UINT8 = 0, 1
 KIND_CONCRETE, KIND_SQGRAD, KIND_ELAST = 0, 1, 2
 
 
@@ -101,6 +102,15 @@ def _declare(L) -> None:
         "gpf_solver_set_filter": (C.c_int, [_vp, _dp, C.c_long]),
         "gpf_solver_set_newton_krylov": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int]),
         "gpf_solver_newton_stats": (C.c_int, [_vp, _ip, _ip, _ip, _dp]),
+        "gpf_solver_save_fields": (C.c_int, [_vp, C.c_char_p, C.c_int, C.c_int]),
+        "gpf_solver_flush_io": (C.c_int, [_vp]),
+        "gpf_solver_save_csv": (C.c_int, [_vp, C.c_char_p, C.c_int]),
+        "gpf_save_float64": (C.c_int, [C.c_char_p, _dp, C.c_long]),
+        "gpf_load_float64": (C.c_int, [C.c_char_p, _dp, C.c_long, _lp]),
+        "gpf_save_csv": (C.c_int, [C.c_char_p, C.POINTER(C.c_char_p), C.c_int, _dp, C.c_long, _ip, C.c_int]),
+        "gpf_write_xdmf": (C.c_int, [C.c_char_p, C.POINTER(C.c_char_p), C.c_int, C.c_char_p, C.c_int, _ip, C.c_int]),
+        "gpf_create_xdmf": (C.c_int, [C.POINTER(C.c_char_p), C.c_int, C.c_char_p, C.c_int, _ip, C.c_int,
+                                      C.c_char_p, C.c_char_p, C.c_long, _lp]),
         "gpf_solver_adjoint": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_long, _dp, _dp, C.c_int,
                                          C.POINTER(C.c_char_p), _dp]),
         "gpf_device_copy_bandwidth": (C.c_int, [C.c_long, C.c_int, C.c_int, _dp]),
@@ -304,6 +314,10 @@ class Model:
     def FieldIndex(self, name: str) -> int:
         return self._fields.index(name)
 
+    def FieldNames(self) -> List[str]:
+        """The names of Model.Fields in order."""
+        return list(self._fields)
+
 
 # ---------------------------------------------------------------------------
 # pf.Solver
@@ -349,11 +363,17 @@ class Solver:
         _check(lib().gpf_solver_stepper_propagate(self._h, int(nsteps)))
 
     def Solve(self, nepochs: int, nsteps: int, callbacks=()) -> None:
-        """Solver.Solve (solver.go:106-120) with host callbacks cb(solver, epoch)."""
+        """Solver.Solve (solver.go:106-120) with host callbacks cb(solver, epoch); files of
+        asynchronous snapshot callbacks (Float64IO, Uint8IO) are written when it returns."""
         for epoch in range(nepochs):
             self.Propagate(nsteps)
             for cb in callbacks:
                 cb(self, epoch)
+        self.FlushIO()
+
+    def FlushIO(self) -> None:
+        """Waits until every queued field snapshot is on disk."""
+        _check(lib().gpf_solver_flush_io(self._h))
 
     def GetTime(self) -> float:
         t = C.c_double()
@@ -624,3 +644,7 @@ def cahn_hilliard_model(n: int, M: float = 1.0, kappa: float = 1.0, data=None) -
 
 
 CH_BULK_POLY = [0.0, 0.0, 1.0, -2.0, 1.0]   # f(c) = c^2 - 2 c^3 + c^4 = c^2 (1 - c)^2
+
+
+from .fileio import (CsvIO, FieldDB, Float64IO, LoadCSV, LoadFloat64, NewFloat64IO, NewUint8IO,  # noqa: E402
+                     SaveCsv, SaveFloat64, Uint8IO, CreateXDMF, WriteXDMF)

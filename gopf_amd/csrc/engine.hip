@@ -38,6 +38,7 @@
 #include "batch.h"
 #include "common.h"
 #include "fft_core.h"
+#include "io.h"
 #include "krylov.h"
 #include "model.h"
 #include "pointwise.h"
@@ -348,6 +349,9 @@ public:
                          int nscal, const char* const* names, double* grad_scalars) = 0;
     virtual void set_newton_krylov(int maxiter, double step_size, double tol, int stencil, int restart) = 0;
     virtual void newton_stats(int* newton_iters, int* krylov_iters, int* converged, double* residual) const = 0;
+    virtual void save_fields(const char* prefix, int epoch, int format) = 0;
+    virtual void flush_io() = 0;
+    virtual void save_csv(const char* prefix, int epoch) = 0;
 };
 
 template <typename T>
@@ -375,6 +379,9 @@ public:
                  const char* const* names, double* grad_scalars) override;
     void set_newton_krylov(int maxiter, double step_size, double tol, int stencil, int restart) override;
     void newton_stats(int* newton_iters, int* krylov_iters, int* converged, double* residual) const override;
+    void save_fields(const char* prefix, int epoch, int format) override;
+    void flush_io() override;
+    void save_csv(const char* prefix, int epoch) override;
 
 private:
     // transforms
@@ -445,6 +452,9 @@ private:
     // residual there, Arnoldi work vector, GMRES basis (restart + 1 vectors)
     Buf<double> nx_, nfx_, ndx_, nxp_, nfp_, nw_, nv_, npart_, ndev_;
     std::vector<double> nhost_;
+    // field snapshots (Float64IO / Uint8IO)
+    std::unique_ptr<SnapshotWriter> io_;
+    Buf<double> io_part_, io_mm_;
 
     Geometry g_;
     hipStream_t st_ = nullptr;
@@ -612,6 +622,7 @@ Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gp
 
 template <typename T>
 Engine<T>::~Engine() {
+    io_.reset();  // writes what is queued
     drop_graph();
     if (st_) {
         (void)hipStreamSynchronize(st_);
@@ -816,6 +827,74 @@ void Engine<T>::get_field(int idx, double* out) {
         out[2 * j] = (double)h[j] + mean;
         out[2 * j + 1] = 0.0;
     }
+}
+
+// ---------------------------------------------------------------------------
+// field output (pf/fileIO.go)
+// ---------------------------------------------------------------------------
+// Float64IO.SaveFields / Uint8IO.SaveFields: <prefix>_<field>_<epoch>.bin for every
+// field.  The bytes are packed on the solver's stream; the copy to pinned host memory and
+// the file writes overlap the steps that follow.
+template <typename T>
+void Engine<T>::save_fields(const char* prefix, int epoch, int format) {
+    if (format != kIoFloat64 && format != kIoUint8) throw Error(GPF_EINVAL, "unknown field file format");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (!io_) io_.reset(new SnapshotWriter(dev));
+    const long n = g_.n;
+    const size_t per = (format == kIoFloat64) ? (size_t)n * 8 : (size_t)n;
+    if (format == kIoUint8 && !io_mm_.get()) {
+        io_part_.alloc(2 * io_blocks(n));
+        io_mm_.alloc(2);
+    }
+    const int slot = io_->acquire(per * nf_);
+    char* buf = static_cast<char*>(io_->device_buffer(slot));
+    std::vector<std::pair<std::string, std::pair<size_t, size_t>>> files;
+    try {
+        for (int i = 0; i < nf_; ++i) {
+            const T* x;
+            const double* mean = nullptr;
+            if (fp_.ok) {
+                T* tmp = rscratch(0);
+                c2r({S_.get()}, {tmp}, T(1));
+                x = tmp;
+                if (split_mean_) mean = mean_.get();
+            } else {
+                x = F_[i].get();
+            }
+            if (format == kIoFloat64) {
+                HIPCHK(launch_pack_be64<T>(x, mean, reinterpret_cast<uint64_t*>(buf + i * per), n, st_));
+            } else {
+                HIPCHK(launch_minmax<T>(x, mean, n, io_part_.get(), io_mm_.get(), st_));
+                HIPCHK(launch_pack_u8<T>(x, mean, io_mm_.get(), reinterpret_cast<uint8_t*>(buf + i * per), n, st_));
+            }
+            files.push_back({std::string(prefix) + "_" + mcopy_.field_names[i] + "_" + std::to_string(epoch) + ".bin",
+                             {i * per, per}});
+        }
+    } catch (...) {
+        io_->submit(slot, st_, 0, {});  // releases the slot
+        throw;
+    }
+    io_->submit(slot, st_, per * nf_, std::move(files));
+}
+
+template <typename T>
+void Engine<T>::flush_io() {
+    if (io_) io_->flush();
+}
+
+// CsvIO.SaveFields (fileIO.go:103-115): <prefix>_<epoch>.csv through SaveCsv
+template <typename T>
+void Engine<T>::save_csv(const char* prefix, int epoch) {
+    std::vector<std::vector<double>> values(nf_);
+    std::vector<double> buf(2 * g_.n);
+    for (int i = 0; i < nf_; ++i) {
+        get_field(i, buf.data());
+        values[i].resize(g_.n);
+        for (long j = 0; j < g_.n; ++j) values[i][j] = buf[2 * j];
+    }
+    std::vector<int> dims(g_.dims, g_.dims + g_.ndim);
+    write_csv(std::string(prefix) + "_" + std::to_string(epoch) + ".csv", mcopy_.field_names, values, dims);
 }
 
 template <typename T>
@@ -2412,6 +2491,93 @@ int gpf_solver_newton_stats(gpf_solver* s, int* newton_iters, int* krylov_iters,
         if (!s) throw Error(GPF_EINVAL, "bad argument");
         if (s->stepper != GPF_IMPLICIT_EULER) throw Error(GPF_EINVAL, "the solver's stepper is not ImplicitEuler");
         s->eng->newton_stats(newton_iters, krylov_iters, converged, residual);
+    });
+}
+
+int gpf_solver_save_fields(gpf_solver* s, const char* prefix, int epoch, int format) {
+    return guarded([&] {
+        if (!s || !prefix) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        s->eng->save_fields(prefix, epoch, format);
+    });
+}
+
+int gpf_solver_flush_io(gpf_solver* s) {
+    return guarded([&] {
+        if (!s) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        s->eng->flush_io();
+    });
+}
+
+int gpf_solver_save_csv(gpf_solver* s, const char* prefix, int epoch) {
+    return guarded([&] {
+        if (!s || !prefix) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        s->eng->save_csv(prefix, epoch);
+    });
+}
+
+int gpf_save_float64(const char* path, const double* data, long n) {
+    return guarded([&] {
+        if (!path || (n > 0 && !data) || n < 0) throw Error(GPF_EINVAL, "bad argument");
+        save_float64(path, data, n);
+    });
+}
+
+int gpf_load_float64(const char* path, double* out, long cap, long* n) {
+    return guarded([&] {
+        if (!path || !n) throw Error(GPF_EINVAL, "bad argument");
+        const std::vector<double> v = load_float64(path);
+        *n = (long)v.size();
+        if (out) std::memcpy(out, v.data(), std::min((long)v.size(), cap) * sizeof(double));
+    });
+}
+
+static std::vector<std::string> name_list(const char* const* names, int n) {
+    if (n < 0 || (n > 0 && !names)) throw Error(GPF_EINVAL, "bad argument");
+    std::vector<std::string> out;
+    for (int i = 0; i < n; ++i) {
+        if (!names[i]) throw Error(GPF_EINVAL, "bad argument");
+        out.emplace_back(names[i]);
+    }
+    return out;
+}
+
+int gpf_create_xdmf(const char* const* fields, int nfields, const char* prefix, int num, const int* dims, int ndim,
+                    const char* indent, char* buf, long cap, long* len) {
+    return guarded([&] {
+        if (!prefix || !dims || !len) throw Error(GPF_EINVAL, "bad argument");
+        const std::string t = xdmf_text_indent(name_list(fields, nfields), prefix, num,
+                                               std::vector<int>(dims, dims + ndim), indent ? indent : "    ");
+        *len = (long)t.size();
+        if (buf && cap > 0) {
+            const long m = std::min((long)t.size(), cap - 1);
+            std::memcpy(buf, t.data(), m);
+            buf[m] = '\0';
+        }
+    });
+}
+
+int gpf_write_xdmf(const char* path, const char* const* fields, int nfields, const char* prefix, int num,
+                   const int* dims, int ndim) {
+    return guarded([&] {
+        if (!path || !prefix || !dims) throw Error(GPF_EINVAL, "bad argument");
+        const std::string t = xdmf_text(name_list(fields, nfields), prefix, num, std::vector<int>(dims, dims + ndim));
+        FILE* f = std::fopen(path, "wb");
+        if (!f) throw Error(GPF_EINVAL, std::string("open ") + path + ": cannot create file");
+        std::fwrite(t.data(), 1, t.size(), f);
+        std::fclose(f);
+    });
+}
+
+int gpf_save_csv(const char* path, const char* const* names, int nfields, const double* values, long n,
+                 const int* dims, int ndim) {
+    return guarded([&] {
+        if (!path || (nfields > 0 && !values) || n < 0) throw Error(GPF_EINVAL, "bad argument");
+        std::vector<std::vector<double>> v(nfields);
+        for (int i = 0; i < nfields; ++i) v[i].assign(values + (long)i * n, values + (long)(i + 1) * n);
+        write_csv(path, name_list(names, nfields), v, dims ? std::vector<int>(dims, dims + ndim) : std::vector<int>());
     });
 }
 

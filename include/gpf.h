@@ -158,7 +158,37 @@ void* gpf_solver_stream(gpf_solver* s);
  * Vandeven.Eval at |f| 2/pi; table NULL removes the filter */
 int gpf_solver_set_filter(gpf_solver* s, const double* table, long n);
 
-/* The nonlinear solver of an ImplicitEuler stepper: ImplicitEuler.NonlinSolver, a
+/* ---------------- field output (pf/fileIO.go, pf/xdmf.go) ---------------- */
+
+#define GPF_IO_FLOAT64 0  /* Float64IO: real part, big-endian float64 (pf/fileIO.go:45-62, 87-95) */
+#define GPF_IO_UINT8 1    /* Uint8IO: real part as uint8 between min and max (pf/fileIO.go:15-42, util.go:108-117) */
+
+/* Float64IO.SaveFields / Uint8IO.SaveFields (pf/fileIO.go:28-42, 57-62) as a solver
+ * callback: writes <prefix>_<field>_<epoch>.bin for every field.  Asynchronous: the device
+ * packs the file bytes on the solver's stream, a copy on a second stream moves them to
+ * pinned host memory and a host thread writes the files while the solver keeps stepping
+ * (two snapshots in flight at most).  gpf_solver_flush_io waits for every queued file and
+ * reports the first write error (the reference prints it, pf/fileIO.go:89-92). */
+int gpf_solver_save_fields(gpf_solver* s, const char* prefix, int epoch, int format);
+int gpf_solver_flush_io(gpf_solver* s);
+/* CsvIO.SaveFields (pf/fileIO.go:103-115): <prefix>_<epoch>.csv through SaveCsv */
+int gpf_solver_save_csv(gpf_solver* s, const char* prefix, int epoch);
+/* SaveFloat64 / LoadFloat64 (pf/fileIO.go:65-95); gpf_load_float64 stores the value count
+ * in *n and copies at most cap values (out may be NULL to query the count) */
+int gpf_save_float64(const char* path, const double* data, long n);
+int gpf_load_float64(const char* path, double* out, long cap, long* n);
+/* SaveCsv (pf/fileIO.go:147-200); values holds nfields arrays of n doubles back to back */
+int gpf_save_csv(const char* path, const char* const* names, int nfields, const double* values, long n,
+                 const int* dims, int ndim);
+/* WriteXDMF (pf/xdmf.go:163-173): CreateXDMF(fields, prefix, num, dims) encoded with a
+ * four-space indent.  gpf_create_xdmf returns the text (indent NULL = four spaces, "" = no
+ * indentation as a bare xml.Encoder), its length in *len, NUL-terminated in buf up to cap. */
+int gpf_write_xdmf(const char* path, const char* const* fields, int nfields, const char* prefix, int num,
+                   const int* dims, int ndim);
+int gpf_create_xdmf(const char* const* fields, int nfields, const char* prefix, int num, const int* dims, int ndim,
+                    const char* indent, char* buf, long cap, long* len);
+
+/* The nonlinear solver of an ImplicitEuler stepper:ImplicitEuler.NonlinSolver, a
  * nonlin.NewtonKrylov (pf/implicitEuler.go:26-28, 221-229; defaults Maxiter 50,
  * StepSize 1e-3, Tol 1e-7, Stencil 6).  stencil is the number of points of the central
  * difference that approximates the Jacobian-vector product (2, 4, 6 or 8; 1 selects a
