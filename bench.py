@@ -401,6 +401,70 @@ def run_ie(G, W, args):
     return out
 
 
+def run_io(G, W, args):
+    """Float64IO snapshots of the 256^3 CH FP32 sim while it steps: 1000 steps with a snapshot
+    every 200 steps against 1000 plain steps, and one synchronous snapshot (Field() to the host,
+    big-endian encoding, file write) for comparison.  The files go to /tmp (page cache), whose
+    write rate bounds how often snapshots can be taken without holding the stepping back."""
+    import shutil
+    import tempfile
+    N, dt, steps, every = 256, 0.05, 1000, 200
+    dims = [N, N, N]
+    n = N ** 3
+    c0 = W.ch_initial(1, n)
+    s = build_solver(G, W, n, dims, c0, "fp32", dt)
+    s.StepperPropagate(args.warmup)
+    d = tempfile.mkdtemp(prefix="gpf_io_", dir="/tmp")
+    try:
+        w = G.Float64IO(os.path.join(d, "snap"))
+        for k in (98, 99):    # allocates both staging and pinned buffers, starts the writer
+            w.SaveFields(s, k)
+        s.FlushIO()
+        plain = time_steps(s, steps)
+        s.Synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps // every):
+            s.StepperPropagate(every)
+            w.SaveFields(s, k)
+        s.Synchronize()
+        t_dev = time.perf_counter() - t0
+        s.FlushIO()
+        t_all = time.perf_counter() - t0
+        nsnap = steps // every
+        # the synchronous path a host callback takes
+        t1 = time.perf_counter()
+        f = s.Field("conc").real
+        np.asarray(f, dtype=">f8").tofile(os.path.join(d, "sync_conc_0.bin"))
+        t_sync = time.perf_counter() - t1
+        ok = G.LoadFloat64(os.path.join(d, f"snap_conc_{nsnap - 1}.bin")).size == n
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    snap_bytes = n * 8
+    overhead = (t_dev - plain) / nsnap
+    print(f"# io: plain {1e3 * plain / steps:.4f} ms/step, with snapshots {1e3 * t_dev / steps:.4f} ms/step, "
+          f"all files written after {t_all:.2f} s; sync snapshot {t_sync:.3f} s", file=sys.stderr, flush=True)
+    out = {"metric": "stepping overhead per asynchronous Float64IO snapshot, 256^3 CH FP32",
+           "value": 1e3 * overhead, "unit": "ms/snapshot", "higher_is_better": False, "dtype": "f32",
+           "ms_per_step_plain": 1e3 * plain / steps, "ms_per_step_with_snapshots": 1e3 * t_dev / steps,
+           "snapshots": nsnap, "snapshot_bytes": snap_bytes, "files_complete_s": t_all,
+           "file_rate_GBs": nsnap * snap_bytes / t_all / 1e9, "sync_snapshot_s": t_sync, "files_ok": bool(ok),
+           "config": {"workload": "float64io_snapshots_ch3d_256", "grid": dims, "dt": dt, "steps": steps,
+                      "snapshot_every": every}}
+    if not args.no_cpu:
+        x = np.random.default_rng(0).random(n)
+        d2 = tempfile.mkdtemp(prefix="gpf_io_cpu_", dir="/tmp")
+        try:
+            t1 = time.perf_counter()
+            np.asarray(x, dtype=">f8").tofile(os.path.join(d2, "c.bin"))
+            el = time.perf_counter() - t1
+        finally:
+            shutil.rmtree(d2, ignore_errors=True)
+        out["cpu_baseline"] = {"value": 1e3 * el, "unit": "ms/snapshot", "cores": 1, "kind": "port",
+                               "sample": "one 256^3 field encoded big-endian and written from host memory "
+                                         "(SaveFloat64's work, fileIO.go:87-95), numpy"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -411,9 +475,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle timing")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--err-steps", type=int, default=1000, help="steps of the FP32-vs-FP64 accuracy run")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "ie"],
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "ie", "io"],
                     help="c2 (default): 256^3 CH headline; c1 128^2 CH FP64 with energy series; c3 batched "
-                         "sweep; c4 192^3 elastic RK4; c5 512^2 adjoint; ie 128^2 CH ImplicitEuler")
+                         "sweep; c4 192^3 elastic RK4; c5 512^2 adjoint; ie 128^2 CH ImplicitEuler; io snapshots")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the multi-GPU slab decomposition is "
@@ -423,12 +487,14 @@ def main() -> int:
     from gopf_amd import workloads as W
 
     if args.config != "c2":
-        defaults = {"c1": 1000, "c3": 500, "c4": 500, "c5": 2000, "ie": 50}
+        defaults = {"c1": 1000, "c3": 500, "c4": 500, "c5": 2000, "ie": 50, "io": 1000}
         if "--steps" not in sys.argv:
             args.steps = defaults[args.config]
-        out = {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5, "ie": run_ie}[args.config](G, W, args)
-        out.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
-                    "data": "synthetic", "vs_baseline": None})
+        out = {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5, "ie": run_ie,
+               "io": run_io}[args.config](G, W, args)
+        out.setdefault("higher_is_better", True)
+        out.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "data": "synthetic",
+                    "vs_baseline": None})
         print(json.dumps(out))
         return 0
 
