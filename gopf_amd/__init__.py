@@ -76,6 +76,9 @@ def _declare(L) -> None:
         "gpf_model_add_equation": (C.c_int, [_vp, C.c_char_p]),
         "gpf_model_register_square_gradient": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.c_double, C.c_int]),
         "gpf_model_register_homogeneous_modulus": (C.c_int, [_vp, C.c_char_p, C.c_char_p, _dp, _dp]),
+        "gpf_model_register_volume_conserving": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_double,
+                                                          C.c_long]),
+        "gpf_model_register_spectral_viscosity": (C.c_int, [_vp, C.c_char_p, C.c_double, C.c_double, C.c_int]),
         "gpf_model_register_polynomial": (C.c_int, [_vp, C.c_char_p, C.c_int, _dp, _dp, C.c_int]),
         "gpf_model_init": (C.c_int, [_vp]),
         "gpf_model_num_bricks": (C.c_int, [_vp, _ip, _ip]),
@@ -162,6 +165,29 @@ def _dptr(a: np.ndarray):
 # ---------------------------------------------------------------------------
 # registered terms
 # ---------------------------------------------------------------------------
+
+@dataclass
+class VolumeConservingLP:
+    """pf.VolumeConservingLP (volumeConserving.go:8-61): Multiplier * Indicator, with the
+    multiplier tuned after every step so that the sum of Field stays constant."""
+    Field: str
+    Indicator: str
+    Dt: float
+    NumNodes: int
+
+
+def NewVolumeConservingLP(fieldName: str, indicator: str, dt: float, numNodes: int) -> VolumeConservingLP:
+    return VolumeConservingLP(fieldName, indicator, dt, numNodes)
+
+
+@dataclass
+class SpectralViscosity:
+    """pf.SpectralViscosity (spectralViscosity.go:10-58): the implicit multiplier
+    -Eps Q(|f|) |f|^Power."""
+    Eps: float
+    DissipationThreshold: float
+    Power: int
+
 
 @dataclass
 class SquaredGradient:
@@ -258,6 +284,19 @@ class Model:
         self._register(name, term, implicit=True)
 
     def _register(self, name: str, term, implicit: bool) -> None:
+        if isinstance(term, VolumeConservingLP):
+            if implicit:
+                raise GpfError(1, "VolumeConservingLP is an explicit term")
+            _check(lib().gpf_model_register_volume_conserving(self._h, name.encode(), term.Field.encode(),
+                                                              term.Indicator.encode(), float(term.Dt),
+                                                              int(term.NumNodes)))
+            return
+        if isinstance(term, SpectralViscosity):
+            if not implicit:
+                raise GpfError(1, "SpectralViscosity runs as an implicit term")
+            _check(lib().gpf_model_register_spectral_viscosity(self._h, name.encode(), float(term.Eps),
+                                                               float(term.DissipationThreshold), int(term.Power)))
+            return
         if isinstance(term, SquaredGradient):
             _check(lib().gpf_model_register_square_gradient(self._h, name.encode(), term.Field.encode(),
                                                             float(term.Factor), 1 if implicit else 0))

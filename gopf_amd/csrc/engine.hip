@@ -226,6 +226,7 @@ struct TermRef {
 struct FieldEq {
     std::vector<TermRef> terms;
     KPoly denom;
+    SvPoly sv;
 };
 
 // fused single-field plan
@@ -471,6 +472,11 @@ private:
     std::vector<Buf<T>> F_, Dr_;
     std::vector<Buf<cpx<T>>> K_, Tsq_, Tel_, INIT_, FINAL_, KF_;
     std::vector<Buf<T>> lag_;
+    // VolumeConservingLP: indicator brick, field, device state {Multiplier, CurrentIntegral,
+    // IsFirstUpdate} per occurrence, and the partial sums of its field integral
+    std::vector<int> vc_brick_, vc_field_;
+    std::vector<Buf<double>> vc_state_;
+    Buf<double> vc_part_;
     std::vector<DerivedArgs<T>> dargs_;
     bool kvalid_ = false;
     std::vector<Buf<cpx<T>>> ks_;
@@ -541,6 +547,17 @@ Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gp
             }
             if (fe.terms.size() > (size_t)kMaxUTerms) throw Error(GPF_EINVAL, "too many terms in one equation");
             for (const auto& t : m.rhs[i].second) {
+                if (t.kind == kSpectralViscosity) {
+                    const UserTerm& u = m.user_table[kSpectralViscosity][t.user];
+                    if (fe.sv.n >= kMaxSv) throw Error(GPF_EINVAL, "too many SpectralViscosity terms in one equation");
+                    if (u.power < 0) throw Error(GPF_EINVAL, "SpectralViscosity.Power must be non-negative");
+                    const int q = fe.sv.n++;
+                    fe.sv.coef[q] = t.coef.real() * -u.eps;
+                    fe.sv.thr[q] = u.threshold;
+                    fe.sv.pw[q] = u.power;
+                    fe.sv.lap[q] = t.lap;
+                    continue;
+                }
                 if (t.kind != kConcrete || t.brick >= 0)
                     throw Error(GPF_EINVAL, "implicit terms that depend on fields cannot be evaluated on the device "
                                             "(only scalar * LAP^p terms are implicit)");
@@ -565,6 +582,17 @@ Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gp
         for (const auto& s : m.user_table[kSquaredGradient]) {
             if (!m.is_field_name(s.field)) throw Error(GPF_EINVAL, "SquaredGradient field is not a field");
         }
+        for (const auto& v : m.user_table[kVolumeConserving]) {
+            if (!m.is_field_name(v.field)) throw Error(GPF_EINVAL, "VolumeConservingLP field is not a field");
+            if (v.num_nodes < 0 || v.num_nodes > g_.n)
+                throw Error(GPF_EINVAL, "VolumeConservingLP.NumNodes exceeds the number of nodes");
+            vc_brick_.push_back(m.brick_index(v.indicator));
+            vc_field_.push_back(m.brick_index(v.field));
+            vc_state_.emplace_back(3);
+            const double init[3] = {0.0, 0.0, 1.0};  // Multiplier, CurrentIntegral, IsFirstUpdate
+            HIPCHK(hipMemcpy(vc_state_.back().get(), init, sizeof(init), hipMemcpyHostToDevice));
+        }
+        if (!m.user_table[kVolumeConserving].empty()) vc_part_.alloc(1024);
         if (stepper_ == GPF_RK4)
             for (int i = 0; i < nf_; ++i) {
                 INIT_.emplace_back(g_.nk);
@@ -1374,6 +1402,7 @@ void Engine<T>::rhs_update(int eq, cpx<T>* dst, const cpx<T>* cur, double alpha,
     a.beta = beta;
     a.gamma = gamma;
     a.denom = eqs_[eq].denom;
+    a.sv = eqs_[eq].sv;
     a.filter = (filt && has_filter_) ? filter_.get() : nullptr;
     if (extra) {
         a.brick[0] = extra;
@@ -1395,9 +1424,18 @@ void Engine<T>::terms_into(int eq, A& a) {
         const int q = a.nterm++;
         a.coef[q] = t.coef;
         a.pw[q] = t.pw;
-        if (t.kind == kConcrete) a.brick[q] = (t.brick >= 0) ? K_[t.brick].get() : nullptr;
-        else if (t.kind == kSquaredGradient) a.brick[q] = Tsq_[t.user].get();
-        else a.brick[q] = Tel_[t.user].get();
+        a.dmul[q] = nullptr;
+        if (t.kind == kConcrete) {
+            a.brick[q] = (t.brick >= 0) ? K_[t.brick].get() : nullptr;
+        } else if (t.kind == kSquaredGradient) {
+            a.brick[q] = Tsq_[t.user].get();
+        } else if (t.kind == kVolumeConserving) {
+            // Multiplier * indicator brick (volumeConserving.go:19-28)
+            a.brick[q] = K_[vc_brick_[t.user]].get();
+            a.dmul[q] = vc_state_[t.user].get();
+        } else {
+            a.brick[q] = Tel_[t.user].get();
+        }
     }
 }
 
@@ -1482,6 +1520,7 @@ void Engine<T>::general_step_rk4() {
         a.beta = 0.0;
         a.gamma = dt_;
         a.denom = eqs_[i].denom;
+        a.sv = eqs_[i].sv;
         a.filter = has_filter_ ? filter_.get() : nullptr;
         a.g = g_.kg;
         a.nmain = g_.nmain;
@@ -1535,6 +1574,7 @@ void Engine<T>::implicit_residual(const double* x, double* out) {
             a.orig = ORIG_[i].get();
             a.rprev = RP_[i].get();
             a.denom = eqs_[i].denom;
+            a.sv = eqs_[i].sv;
             a.dt = dt_;
             a.g = g_.kg;
             a.nmain = g_.nmain;
@@ -1776,6 +1816,18 @@ void Engine<T>::step() {
 template <typename T>
 void Engine<T>::on_step_finished() {
     if (fp_.ok) return;
+    // VolumeConservingLP (volumeConserving.go:31-50): the field integral over its first
+    // NumNodes nodes (real space after the step); the indicator integral is the zero mode of
+    // the derived brick (still in k-space), or element 0 of a field brick
+    for (size_t v = 0; v < vc_state_.size(); ++v) {
+        const UserTerm& ut = mcopy_.user_table[kVolumeConserving][v];
+        const long nb = 512;
+        HIPCHK(launch_real_sums<T>(F_[vc_field_[v]].get(), ut.num_nodes, 0.0, 0, nullptr, vc_part_.get(),
+                                   vc_part_.get() + nb, nb, st_));
+        const int b = vc_brick_[v];
+        const T* ind = (b < nf_) ? F_[b].get() : reinterpret_cast<const T*>(K_[b].get());
+        HIPCHK(launch_volcons_update<T>(vc_part_.get() + nb, nb, ind, ut.dt, vc_state_[v].get(), st_));
+    }
     for (size_t e = 0; e < lag_.size(); ++e) {
         const UserTerm& ut = mcopy_.user_table[kHomogeneousModulus][e];
         int f = 0;
@@ -2216,6 +2268,32 @@ int gpf_model_register_homogeneous_modulus(gpf_model* m, const char* term, const
         std::memcpy(u.C, C, sizeof(u.C));
         std::memcpy(u.misfit, misfit, sizeof(u.misfit));
         m->m.register_explicit_term(term, u);
+    });
+}
+
+int gpf_model_register_volume_conserving(gpf_model* m, const char* term, const char* field, const char* indicator,
+                                         double dt, long num_nodes) {
+    return guarded([&] {
+        if (!m || !term || !field || !indicator) throw Error(GPF_EINVAL, "bad argument");
+        UserTerm u;
+        u.kind = kVolumeConserving;
+        u.field = field;
+        u.indicator = indicator;
+        u.dt = dt;
+        u.num_nodes = num_nodes;
+        m->m.register_explicit_term(term, u);
+    });
+}
+
+int gpf_model_register_spectral_viscosity(gpf_model* m, const char* term, double eps, double threshold, int power) {
+    return guarded([&] {
+        if (!m || !term) throw Error(GPF_EINVAL, "bad argument");
+        UserTerm u;
+        u.kind = kSpectralViscosity;
+        u.eps = eps;
+        u.threshold = threshold;
+        u.power = power;
+        m->m.register_implicit_term(term, u);
     });
 }
 

@@ -430,6 +430,9 @@ TermDesc Model::user_term(const UserTerm& u, const std::vector<std::string>& pre
         else if (p == "LAP^2") lap += 2;
         else if (p == "LAP") lap += 1;
     }
+    if (u.kind == kVolumeConserving && !is_field_name(u.indicator) &&
+        std::none_of(derived.begin(), derived.end(), [&](const DerivedDesc& d) { return d.name == u.indicator; }))
+        throw ParseError("VolumeConservingLP: Indicator is not a derived field");
     auto& table = user_table[u.kind];
     table.push_back(u);
     TermDesc t;

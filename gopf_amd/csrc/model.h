@@ -20,7 +20,14 @@ struct ParseError : std::runtime_error {
     explicit ParseError(const std::string& m) : std::runtime_error(m) {}
 };
 
-enum TermKind { kConcrete = 0, kSquaredGradient = 1, kHomogeneousModulus = 2 };
+enum TermKind {
+    kConcrete = 0,
+    kSquaredGradient = 1,
+    kHomogeneousModulus = 2,
+    kVolumeConserving = 3,   // VolumeConservingLP (pf/volumeConserving.go)
+    kSpectralViscosity = 4,  // SpectralViscosity (pf/spectralViscosity.go)
+    kNumTermKinds = 5
+};
 
 struct TermDesc {
     int kind = kConcrete;
@@ -51,6 +58,12 @@ struct UserTerm {
     double factor = 1.0;     // SquaredGradient.Factor
     double C[81] = {};       // elastic tensor C_ijkl, index i*27 + j*9 + k*3 + l
     double misfit[9] = {};   // misfit strain, row-major 3x3
+    std::string indicator;   // VolumeConservingLP.Indicator (field = VolumeConservingLP.Field)
+    double dt = 0.0;         // VolumeConservingLP.Dt
+    long num_nodes = 0;      // VolumeConservingLP.NumNodes
+    double eps = 0.0;        // SpectralViscosity.Eps
+    double threshold = 0.0;  // SpectralViscosity.DissipationThreshold
+    int power = 0;           // SpectralViscosity.Power
 };
 
 struct SubStringDelimiter {
@@ -68,7 +81,7 @@ public:
     std::map<std::string, UserTerm> implicit_terms, explicit_terms;
     std::vector<std::string> equations;
     std::vector<std::pair<std::vector<TermDesc>, std::vector<TermDesc>>> rhs;
-    std::vector<UserTerm> user_table[3];
+    std::vector<UserTerm> user_table[kNumTermKinds];
 
     void add_field(const std::string& name, std::vector<std::complex<double>> data);
     void add_scalar(const std::string& name, std::complex<double> v);

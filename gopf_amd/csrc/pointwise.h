@@ -47,6 +47,18 @@ inline int make_dense_derived(DerivedArgs<T>& a) {
     return a.dense = 1;
 }
 
+// implicit SpectralViscosity terms of a denominator (spectralViscosity.go:44-52 with the
+// constructFunc prefixes): sum_t coef_t L^lap_t Q(|f|, thr_t) |f|^pw_t, |f| in cycles per
+// sample; coef_t carries the prefix sign and -Eps
+constexpr int kMaxSv = 4;
+struct SvPoly {
+    int n = 0;
+    double coef[kMaxSv];
+    double thr[kMaxSv];
+    int pw[kMaxSv];
+    int lap[kMaxSv];
+};
+
 // half-spectrum k-space update
 //   dst = filt * (alpha * cur + beta * sum_t coef_t L^pow_t B_t) / (1 - gamma * D(k))
 // B_t == null means the brick is the constant 1.
@@ -58,8 +70,10 @@ struct KUpdateArgs {
     const cpx<T>* brick[kMaxUTerms] = {};
     double coef[kMaxUTerms];
     int pw[kMaxUTerms];
+    const double* dmul[kMaxUTerms] = {};  // optional device-resident factor of a term (null = 1)
     double alpha = 1.0, beta = 0.0, gamma = 0.0;
     KPoly denom;
+    SvPoly sv;
     const double* filter = nullptr;
     KGrid g;
     long nmain = 0, nk = 0;            // main elements; total (main + nyq)
@@ -79,7 +93,9 @@ struct EtdArgs {
     const cpx<T>* brick[kMaxUTerms] = {};
     double coef[kMaxUTerms];
     int pw[kMaxUTerms];
+    const double* dmul[kMaxUTerms] = {};
     KPoly denom;
+    SvPoly sv;
     double dt = 0.0;
     KGrid g;
     long nmain = 0, nk = 0;
@@ -156,6 +172,12 @@ template <typename T> hipError_t launch_kupdate(const KUpdateArgs<T>& a, hipStre
 template <typename T> hipError_t launch_etd_residual(const EtdArgs<T>& a, hipStream_t s);
 template <typename T> hipError_t launch_kgrad(const KGradArgs<T>& a, int ndim, hipStream_t s);
 template <typename T> hipError_t launch_kelastic(const KElasticArgs<T>& a, hipStream_t s);
+// VolumeConservingLP.OnStepFinished (volumeConserving.go:31-50) on device state
+// st = {Multiplier, CurrentIntegral, IsFirstUpdate}: the field integral is the sum of the
+// partials px[0..nb), the indicator integral *ind (the zero mode of a k-space derived brick,
+// or element 0 of a real field)
+template <typename T> hipError_t launch_volcons_update(const double* px, long nb, const T* ind, double dt,
+                                                       double* st, hipStream_t s);
 // out = factor * sum_d w_d^2
 template <typename T> hipError_t launch_sqsum(const T* const* w, int nw, T factor, T* out, long n, hipStream_t s);
 // indicator H(phi) = 3 phi^2 - 2 phi^3  (homoLinElast.go:10-12)

@@ -63,6 +63,38 @@ __device__ __forceinline__ double vandeven(const double* __restrict__ tab, doubl
     return tab[idx] + (x - x0) * dy / dx;
 }
 
+// math.Pow(x, n) for a small non-negative integer n: Go multiplies the (renormalised)
+// mantissas by repeated squaring, which rounds as this loop does
+__device__ __forceinline__ double go_ipow(double x, int n) {
+    double a = 1.0, b = x;
+    for (int i = n; i != 0; i >>= 1) {
+        if (i & 1) a *= b;
+        b *= b;
+    }
+    return a;
+}
+
+// interpolant (spectralViscosity.go:29-39), including its value -1 at f = peak
+__device__ __forceinline__ double sv_interp(double f, double peak) {
+    const double frac = 1.0 / 3.0;
+    if (f < frac * peak) return 0.0;
+    if (f > peak) return 1.0;
+    const double x = 1.5 * (f - frac * peak) / peak;
+    return 2.0 * x * x - 3.0 * x * x * x;
+}
+
+// D(k) = sum of the LAP^p terms and the SpectralViscosity terms (f2 = |f|^2)
+__device__ __forceinline__ double denominator(const KPoly& p, const SvPoly& sv, double L, double f2) {
+    double D = 0.0;
+    for (int t = 0; t < p.n; ++t) D += p.coef[t] * ipow(L, p.pow[t]);
+    if (sv.n) {
+        const double fr = sqrt(f2);
+        for (int t = 0; t < sv.n; ++t)
+            D += sv.coef[t] * ipow(L, sv.lap[t]) * sv_interp(fr, sv.thr[t]) * go_ipow(fr, sv.pw[t]);
+    }
+    return D;
+}
+
 template <typename T>
 __device__ __forceinline__ T powT(T x, double p) {
     const double r = rint(p);
@@ -122,7 +154,8 @@ __global__ void k_kupdate(KUpdateArgs<T> a) {
         const double L = -(4.0 * kPi * kPi) * f2;
         double rx = 0.0, ry = 0.0;
         for (int t = 0; t < a.nterm; ++t) {
-            const double m = a.coef[t] * ipow(L, a.pw[t]);
+            double m = a.coef[t] * ipow(L, a.pw[t]);
+            if (a.dmul[t]) m *= a.dmul[t][0];
             if (a.brick[t]) {
                 const cpx<T> b = a.brick[t][i];
                 rx += m * (double)b.x;
@@ -131,8 +164,7 @@ __global__ void k_kupdate(KUpdateArgs<T> a) {
                 rx += m;
             }
         }
-        double D = 0.0;
-        for (int t = 0; t < a.denom.n; ++t) D += a.denom.coef[t] * ipow(L, a.denom.pow[t]);
+        const double D = denominator(a.denom, a.sv, L, f2);
         double s = 1.0 / (1.0 - a.gamma * D);
         if (a.filter) s *= vandeven(a.filter, sqrt(f2) * 2.0 / kPi);
         double cx = 0.0, cy = 0.0;
@@ -155,10 +187,12 @@ __global__ void k_etd_residual(EtdArgs<T> a) {
         kcoords(a.g, i, a.nmain, kx, ky, kz);
         double f[3];
         ref_freq(a.g, kx, ky, kz, f);
-        const double L = -(4.0 * kPi * kPi) * (f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        const double f2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+        const double L = -(4.0 * kPi * kPi) * f2;
         double rx = 0.0, ry = 0.0;
         for (int t = 0; t < a.nterm; ++t) {
-            const double m = a.coef[t] * ipow(L, a.pw[t]);
+            double m = a.coef[t] * ipow(L, a.pw[t]);
+            if (a.dmul[t]) m *= a.dmul[t][0];
             if (a.brick[t]) {
                 const cpx<T> b = a.brick[t][i];
                 rx += m * (double)b.x;
@@ -167,8 +201,7 @@ __global__ void k_etd_residual(EtdArgs<T> a) {
                 rx += m;
             }
         }
-        double D = 0.0;
-        for (int t = 0; t < a.denom.n; ++t) D += a.denom.coef[t] * ipow(L, a.denom.pow[t]);
+        const double D = denominator(a.denom, a.sv, L, f2);
         const double e = exp(D * a.dt);
         const cpx<T> p = a.rprev[i];
         const double px = (double)p.x, py = (double)p.y;
@@ -185,6 +218,23 @@ __global__ void k_etd_residual(EtdArgs<T> a) {
         const cpx<T> o = a.orig[i];
         const cpx<T> c = a.cur[i];
         a.dst[i] = mk<T>((T)((double)c.x - ((double)o.x * e + ix)), (T)((double)c.y - ((double)o.y * e + iy)));
+    }
+}
+
+template <typename T>
+__global__ void k_volcons_update(const double* __restrict__ px, long nb, const T* __restrict__ ind, double dt,
+                                 double* __restrict__ st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double fi = 0.0;
+    for (long b = 0; b < nb; ++b) fi += px[b];
+    const double ii = (double)ind[0];
+    if (st[2] != 0.0) {
+        st[1] = fi;
+        st[2] = 0.0;
+    } else {
+        const double d = fi - st[1];
+        st[1] = fi;
+        st[0] = st[0] - d / (dt * ii);
     }
 }
 
@@ -541,6 +591,11 @@ hipError_t launch_etd_residual(const EtdArgs<T>& a, hipStream_t s) {
     return hipGetLastError();
 }
 template <typename T>
+hipError_t launch_volcons_update(const double* px, long nb, const T* ind, double dt, double* st, hipStream_t s) {
+    hipLaunchKernelGGL(k_volcons_update<T>, dim3(1), dim3(64), 0, s, px, nb, ind, dt, st);
+    return hipGetLastError();
+}
+template <typename T>
 hipError_t launch_kgrad(const KGradArgs<T>& a, int ndim, hipStream_t s) {
     hipLaunchKernelGGL(k_kgrad<T>, dim3(grid_for(a.nk)), dim3(256), 0, s, a, ndim);
     return hipGetLastError();
@@ -654,6 +709,7 @@ hipError_t launch_reduce2(const double* a, long na, double ca, const double* b, 
     template hipError_t launch_derived<T>(const DerivedArgs<T>&, hipStream_t);                                \
     template hipError_t launch_kupdate<T>(const KUpdateArgs<T>&, hipStream_t);                                \
     template hipError_t launch_etd_residual<T>(const EtdArgs<T>&, hipStream_t);                               \
+    template hipError_t launch_volcons_update<T>(const double*, long, const T*, double, double*, hipStream_t); \
     template hipError_t launch_kgrad<T>(const KGradArgs<T>&, int, hipStream_t);                               \
     template hipError_t launch_kelastic<T>(const KElasticArgs<T>&, hipStream_t);                              \
     template hipError_t launch_sqsum<T>(const T* const*, int, T, T*, long, hipStream_t);                      \

@@ -76,6 +76,16 @@ int gpf_model_register_square_gradient(gpf_model* m, const char* term, const cha
  * C is C_ijkl (81 doubles, index i*27+j*9+k*3+l), misfit is 3x3 row-major */
 int gpf_model_register_homogeneous_modulus(gpf_model* m, const char* term, const char* field,
                                            const double* C, const double* misfit);
+/* pf.NewVolumeConservingLP + RegisterExplicitTerm (pf/volumeConserving.go:8-61): the term
+ * Multiplier * indicator, whose Lagrange multiplier is updated after every step
+ * (OnStepFinished) from the change of sum_i field[i], i < num_nodes; indicator names a
+ * derived field (a registered function) or a field */
+int gpf_model_register_volume_conserving(gpf_model* m, const char* term, const char* field, const char* indicator,
+                                         double dt, long num_nodes);
+/* pf.SpectralViscosity + RegisterImplicitTerm (pf/spectralViscosity.go:10-58): the
+ * implicit multiplier -eps Q(|f|) |f|^power with Tadmor's interpolant Q between
+ * threshold / 3 and threshold */
+int gpf_model_register_spectral_viscosity(gpf_model* m, const char* term, double eps, double threshold, int power);
 /* Model.RegisterFunction (pf/model.go:400-412) for a polynomial of the fields' real parts:
  * value = sum_t coefs[t] * prod_f re(field_f)^powers[t * nfields + f]; the device cannot run
  * Go closures, so the function is given as data. */
@@ -86,7 +96,8 @@ int gpf_model_init(gpf_model* m);
 
 /* introspection of the parsed model (Model.AllFieldNames, Model.RHS) */
 typedef struct {
-    int kind;          /* 0 ConcreteTerm, 1 SquaredGradient, 2 HomogeneousModulusLinElast */
+    int kind;          /* 0 ConcreteTerm, 1 SquaredGradient, 2 HomogeneousModulusLinElast,
+                          3 VolumeConservingLP, 4 SpectralViscosity */
     double coef_re;    /* concrete: sign * prod Pow(scalar, p); user terms: +-1 */
     double coef_im;
     int brick;         /* brick index (fields, then derived fields) or -1 */

@@ -85,6 +85,13 @@ def _declare(L) -> None:
     L.or_current_step.argtypes = [C.c_void_p]
     L.or_current_step.restype = C.c_long
     L.or_eval_user_term.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+    L.or_add_volcons.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_long]
+    L.or_add_svisc.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int]
+    L.or_volcons_multiplier.argtypes = [C.c_void_p, C.c_int]
+    L.or_volcons_multiplier.restype = C.c_double
+    L.or_on_step_finished.argtypes = [C.c_void_p]
+    L.or_sv_interpolant.argtypes = [C.c_double, C.c_double]
+    L.or_sv_interpolant.restype = C.c_double
     L.or_ch_energy.argtypes = [C.c_int, _ip, _dp, C.c_int, _dp, C.c_double]
     L.or_ch_energy.restype = C.c_double
     L.or_euler_adjoint.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp]
@@ -236,6 +243,11 @@ class OracleSolver:
             len(sq), I(sq_brick), Dd(sq_factor),
             len(el), I(el_field), Dd(el_C), Dd(el_mis),
             self.EULER if stepper == "euler" else self.RK4, float(dt), int(filter_order))
+        for v in model.user_tables.get(P.KIND_VOLCONS, []):
+            lib().or_add_volcons(self.h, model.brick_index(v["field"]), model.brick_index(v["indicator"]),
+                                 float(v["dt"]), int(v["num_nodes"]))
+        for v in model.user_tables.get(P.KIND_SVISC, []):
+            lib().or_add_svisc(self.h, float(v["eps"]), float(v["threshold"]), int(v["power"]))
         for name, data in fields.items():
             self.set_field(name, data)
 
@@ -292,6 +304,12 @@ class OracleSolver:
         out = np.zeros(self.n, dtype=np.complex128)
         lib().or_eval_user_term(self.h, kind, user, out.ctypes.data_as(_dp))
         return out
+
+    def on_step_finished(self) -> None:
+        lib().or_on_step_finished(self.h)
+
+    def volcons_multiplier(self, v: int = 0) -> float:
+        return float(lib().or_volcons_multiplier(self.h, v))
 
     # -- stepping --------------------------------------------------------------
     def step(self) -> None:

@@ -248,7 +248,8 @@ def get_known_prefixes(s: str) -> List[str]:
 # Model (pf/model.go) restatement producing a flat term description
 # ---------------------------------------------------------------------------
 
-KIND_CONCRETE, KIND_SQGRAD, KIND_ELAST = 0, 1, 2
+KIND_CONCRETE, KIND_SQGRAD, KIND_ELAST, KIND_VOLCONS, KIND_SVISC = 0, 1, 2, 3, 4
+_USER_KINDS = (KIND_SQGRAD, KIND_ELAST, KIND_VOLCONS, KIND_SVISC)
 
 
 @dataclass
@@ -287,7 +288,7 @@ class DerivedDesc:
 
 @dataclass
 class UserTerm:
-    kind: int                       # KIND_SQGRAD or KIND_ELAST
+    kind: int                       # KIND_SQGRAD, KIND_ELAST, KIND_VOLCONS or KIND_SVISC
     params: dict
 
 
@@ -302,7 +303,7 @@ class Model:
         self.explicit_terms: Dict[str, UserTerm] = {}
         self.equations: List[str] = []
         self.rhs: List[Tuple[List[TermDesc], List[TermDesc]]] = []
-        self.user_tables: Dict[int, List[dict]] = {KIND_SQGRAD: [], KIND_ELAST: []}
+        self.user_tables: Dict[int, List[dict]] = {k: [] for k in _USER_KINDS}
 
     # -- names -------------------------------------------------------------
     def field_names(self) -> List[str]:
@@ -435,6 +436,8 @@ class Model:
                 lap += 2
             elif p == "LAP":
                 lap += 1
+        if ut.kind == KIND_VOLCONS and not self.is_field_name(ut.params["indicator"]):
+            raise ParseError("VolumeConservingLP: Indicator is not a derived field")
         table = self.user_tables[ut.kind]
         table.append(ut.params)
         return TermDesc(ut.kind, coef=complex(sign, 0.0), lap=lap, user=len(table) - 1)
@@ -464,7 +467,7 @@ class Model:
     def init(self) -> None:
         """Model.Init (model.go:244-260)."""
         self.rhs = []
-        self.user_tables = {KIND_SQGRAD: [], KIND_ELAST: []}
+        self.user_tables = {k: [] for k in _USER_KINDS}
         for eq in self.equations:
             self.rhs.append(self.build(eq))
 

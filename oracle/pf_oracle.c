@@ -462,7 +462,7 @@ static int solve3x2(double A[3][3], double B[3][2], double X[3][2])
 /* Model engine                                                        */
 /* ------------------------------------------------------------------ */
 
-enum { TERM_CONCRETE = 0, TERM_SQGRAD = 1, TERM_ELAST = 2 };
+enum { TERM_CONCRETE = 0, TERM_SQGRAD = 1, TERM_ELAST = 2, TERM_VOLCONS = 3, TERM_SVISC = 4 };
 
 typedef struct {
     int kind;
@@ -486,6 +486,21 @@ typedef struct {
     zc *inject_disp;  /* optional injected displacement (3 x n, component-major), test only */
 } or_elast;
 
+/* VolumeConservingLP (volumeConserving.go:8-17) */
+typedef struct {
+    int field_brick, ind_brick;
+    double dt;
+    long num_nodes;
+    double multiplier, current_integral;
+    int is_first_update;
+} or_volcons;
+
+/* SpectralViscosity (spectralViscosity.go:22-26) */
+typedef struct {
+    double eps, threshold;
+    int power;
+} or_svisc;
+
 typedef struct {
     int ndim;
     int dims[3];
@@ -507,6 +522,9 @@ typedef struct {
     int nsq, nel;
     or_sqgrad *sq;
     or_elast *el;
+    int nvc, nsv;
+    or_volcons *vc;
+    or_svisc *sv;
     /* stepper */
     int stepper; /* 0 euler, 1 rk4 */
     double dt;
@@ -606,6 +624,7 @@ void or_model_destroy(void *h)
     free(m->sq);
     for (int e = 0; e < m->nel; ++e) { free(m->el[e].lag); free(m->el[e].inject_disp); }
     free(m->el);
+    free(m->vc); free(m->sv);
     fftnd_destroy(m->ft);
     free(m->tmp); free(m->tmp2); free(m->work);
     free(m);
@@ -787,6 +806,48 @@ static void eval_elast(or_model *m, const or_elast *e, zc *field)
     if (own_disp) free(disp);
 }
 
+/* NewVolumeConservingLP + RegisterExplicitTerm, in the order of the model's user table */
+void or_add_volcons(void *h, int field_brick, int ind_brick, double dt, long num_nodes)
+{
+    or_model *m = (or_model *)h;
+    m->vc = (or_volcons *)realloc(m->vc, sizeof(or_volcons) * (size_t)(m->nvc + 1));
+    or_volcons *v = &m->vc[m->nvc++];
+    v->field_brick = field_brick; v->ind_brick = ind_brick; v->dt = dt; v->num_nodes = num_nodes;
+    v->multiplier = 0.0; v->current_integral = 0.0; v->is_first_update = 1;
+}
+
+/* SpectralViscosity + RegisterImplicitTerm, in the order of the model's user table */
+void or_add_svisc(void *h, double eps, double threshold, int power)
+{
+    or_model *m = (or_model *)h;
+    m->sv = (or_svisc *)realloc(m->sv, sizeof(or_svisc) * (size_t)(m->nsv + 1));
+    or_svisc *v = &m->sv[m->nsv++];
+    v->eps = eps; v->threshold = threshold; v->power = power;
+}
+
+double or_volcons_multiplier(void *h, int v) { return ((or_model *)h)->vc[v].multiplier; }
+
+/* interpolant (spectralViscosity.go:29-39) */
+double or_sv_interpolant(double f, double peak)
+{
+    double frac = 1.0 / 3.0;
+    if (f < frac * peak) return 0.0;
+    else if (f > peak) return 1.0;
+    double x = 1.5 * (f - frac * peak) / peak;
+    return 2.0 * x * x - 3.0 * x * x * x;
+}
+
+/* math.Pow(x, n), n a non-negative integer: repeated squaring */
+static double go_pow_int(double x, int n)
+{
+    double a = 1.0, b = x;
+    for (int i = n; i != 0; i >>= 1) {
+        if (i & 1) a *= b;
+        b *= b;
+    }
+    return a;
+}
+
 /* evaluate one term (closure) into out */
 static void eval_term(or_model *m, const or_term *t, zc *out)
 {
@@ -806,8 +867,26 @@ static void eval_term(or_model *m, const or_term *t, zc *out)
         }
         return;
     }
-    if (t->kind == TERM_SQGRAD) eval_sqgrad(m, &m->sq[t->user], out);
-    else eval_elast(m, &m->el[t->user], out);
+    if (t->kind == TERM_SQGRAD) {
+        eval_sqgrad(m, &m->sq[t->user], out);
+    } else if (t->kind == TERM_VOLCONS) {
+        /* VolumeConservingLP.Construct (volumeConserving.go:19-28) */
+        const or_volcons *v = &m->vc[t->user];
+        const zc *ind = m->bricks[v->ind_brick];
+        for (long i = 0; i < n; ++i) out[i] = go_cmul(ind[i], v->multiplier + 0.0 * I);
+    } else if (t->kind == TERM_SVISC) {
+        /* SpectralViscosity.Construct (spectralViscosity.go:44-52) */
+        const or_svisc *v = &m->sv[t->user];
+        for (long i = 0; i < n; ++i) {
+            ref_freq(m->ndim, m->dims, i, f);
+            double d = 0.0;
+            for (int a = 0; a < m->ndim; ++a) d += f[a] * f[a];
+            double fr = sqrt(d);
+            out[i] = -v->eps * or_sv_interpolant(fr, v->threshold) * go_pow_int(fr, v->power);
+        }
+    } else {
+        eval_elast(m, &m->el[t->user], out);
+    }
     /* constructFunc prefixes: sign flips and LAP powers */
     for (long i = 0; i < n; ++i) {
         zc v = out[i];
@@ -977,17 +1056,39 @@ double or_get_time(void *h)
     return (double)m->current_step * m->dt;
 }
 
-/* Solver.Propagate (solver.go:70-85): Step, then OnStepFinished of every term.
- * Only HomogeneousModulusLinElast has state (homoLinElast.go:130-134). */
+/* OnStepFinished of every term with state: HomogeneousModulusLinElast's lagged field
+ * (homoLinElast.go:130-134) and VolumeConservingLP's multiplier (volumeConserving.go:31-50;
+ * the field brick is real space after a step, a derived indicator brick still k-space) */
+void or_on_step_finished(void *h)
+{
+    or_model *m = (or_model *)h;
+    for (int e = 0; e < m->nel; ++e) {
+        zc *f = m->bricks[m->el[e].field];
+        for (long i = 0; i < m->n; ++i) m->el[e].lag[i] = creal(f[i]);
+    }
+    for (int v = 0; v < m->nvc; ++v) {
+        or_volcons *c = &m->vc[v];
+        double field_integral = 0.0;
+        for (long i = 0; i < c->num_nodes; ++i) field_integral += creal(m->bricks[c->field_brick][i]);
+        double indicator_integral = creal(m->bricks[c->ind_brick][0]);
+        if (c->is_first_update) {
+            c->current_integral = field_integral;
+            c->is_first_update = 0;
+        } else {
+            double delta = field_integral - c->current_integral;
+            c->current_integral = field_integral;
+            c->multiplier = c->multiplier - delta / (c->dt * indicator_integral);
+        }
+    }
+}
+
+/* Solver.Propagate (solver.go:70-85): Step, then OnStepFinished of every term. */
 void or_solver_propagate(void *h, int nsteps)
 {
     or_model *m = (or_model *)h;
     for (int s = 0; s < nsteps; ++s) {
         or_step(m);
-        for (int e = 0; e < m->nel; ++e) {
-            zc *f = m->bricks[m->el[e].field];
-            for (long i = 0; i < m->n; ++i) m->el[e].lag[i] = creal(f[i]);
-        }
+        or_on_step_finished(m);
     }
 }
 
@@ -995,8 +1096,12 @@ void or_solver_propagate(void *h, int nsteps)
 void or_eval_user_term(void *h, int kind, int user, double *out)
 {
     or_model *m = (or_model *)h;
-    if (kind == TERM_SQGRAD) eval_sqgrad(m, &m->sq[user], (zc *)out);
-    else eval_elast(m, &m->el[user], (zc *)out);
+    or_term t = {0};
+    t.kind = kind;
+    t.coef_re = 1.0;
+    t.user = user;
+    if (kind == TERM_ELAST) eval_elast(m, &m->el[user], (zc *)out);
+    else eval_term(m, &t, (zc *)out);
 }
 
 /* ------------------------------------------------------------------ */
