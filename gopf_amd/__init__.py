@@ -79,6 +79,7 @@ def _declare(L) -> None:
         "gpf_model_register_volume_conserving": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_double,
                                                           C.c_long]),
         "gpf_model_register_spectral_viscosity": (C.c_int, [_vp, C.c_char_p, C.c_double, C.c_double, C.c_int]),
+        "gpf_model_register_tensorial_hessian": (C.c_int, [_vp, C.c_char_p, C.c_char_p, _dp, C.c_int]),
         "gpf_model_register_polynomial": (C.c_int, [_vp, C.c_char_p, C.c_int, _dp, _dp, C.c_int]),
         "gpf_model_init": (C.c_int, [_vp]),
         "gpf_model_num_bricks": (C.c_int, [_vp, _ip, _ip]),
@@ -190,6 +191,14 @@ class SpectralViscosity:
 
 
 @dataclass
+class TensorialHessian:
+    """pf.TensorialHessian (tensorialHessian.go:19-66): the implicit multiplier of
+    sum_ij K_ij d^2/dx_i dx_j; K row-major with 4 (2D) or 9 (3D) entries."""
+    Field: str
+    K: Sequence[float]
+
+
+@dataclass
 class SquaredGradient:
     """pf.SquaredGradient (squareGradientTerm.go:15-37): Factor * |grad Field|^2."""
     Field: str
@@ -290,6 +299,13 @@ class Model:
             _check(lib().gpf_model_register_volume_conserving(self._h, name.encode(), term.Field.encode(),
                                                               term.Indicator.encode(), float(term.Dt),
                                                               int(term.NumNodes)))
+            return
+        if isinstance(term, TensorialHessian):
+            if not implicit:
+                raise GpfError(1, "TensorialHessian runs as an implicit term")
+            K = np.ascontiguousarray(np.asarray(term.K, dtype=np.float64).reshape(-1))
+            _check(lib().gpf_model_register_tensorial_hessian(self._h, name.encode(), str(term.Field or "").encode(),
+                                                              _dptr(K), K.size))
             return
         if isinstance(term, SpectralViscosity):
             if not implicit:

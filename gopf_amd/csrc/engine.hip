@@ -227,6 +227,7 @@ struct FieldEq {
     std::vector<TermRef> terms;
     KPoly denom;
     SvPoly sv;
+    HessPoly hs;
 };
 
 // fused single-field plan
@@ -556,6 +557,19 @@ Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gp
                     fe.sv.thr[q] = u.threshold;
                     fe.sv.pw[q] = u.power;
                     fe.sv.lap[q] = t.lap;
+                    continue;
+                }
+                if (t.kind == kTensorialHessian) {
+                    const UserTerm& u = m.user_table[kTensorialHessian][t.user];
+                    if (fe.hs.n >= kMaxHess) throw Error(GPF_EINVAL, "too many TensorialHessian terms in one equation");
+                    const int d = (u.hess_len == 9) ? 3 : 2;
+                    if (u.hess_len != d * d || d < g_.ndim)
+                        throw Error(GPF_EINVAL, "TensorialHessian.K must hold ndim x ndim coefficients");
+                    const int q = fe.hs.n++;
+                    fe.hs.coef[q] = t.coef.real();
+                    fe.hs.lap[q] = t.lap;
+                    fe.hs.kd[q] = d;
+                    for (int k = 0; k < 9; ++k) fe.hs.K[q][k] = u.hess[k];
                     continue;
                 }
                 if (t.kind != kConcrete || t.brick >= 0)
@@ -1403,6 +1417,7 @@ void Engine<T>::rhs_update(int eq, cpx<T>* dst, const cpx<T>* cur, double alpha,
     a.gamma = gamma;
     a.denom = eqs_[eq].denom;
     a.sv = eqs_[eq].sv;
+    a.hs = eqs_[eq].hs;
     a.filter = (filt && has_filter_) ? filter_.get() : nullptr;
     if (extra) {
         a.brick[0] = extra;
@@ -1521,6 +1536,7 @@ void Engine<T>::general_step_rk4() {
         a.gamma = dt_;
         a.denom = eqs_[i].denom;
         a.sv = eqs_[i].sv;
+        a.hs = eqs_[i].hs;
         a.filter = has_filter_ ? filter_.get() : nullptr;
         a.g = g_.kg;
         a.nmain = g_.nmain;
@@ -1575,6 +1591,7 @@ void Engine<T>::implicit_residual(const double* x, double* out) {
             a.rprev = RP_[i].get();
             a.denom = eqs_[i].denom;
             a.sv = eqs_[i].sv;
+            a.hs = eqs_[i].hs;
             a.dt = dt_;
             a.g = g_.kg;
             a.nmain = g_.nmain;
@@ -2293,6 +2310,19 @@ int gpf_model_register_spectral_viscosity(gpf_model* m, const char* term, double
         u.eps = eps;
         u.threshold = threshold;
         u.power = power;
+        m->m.register_implicit_term(term, u);
+    });
+}
+
+int gpf_model_register_tensorial_hessian(gpf_model* m, const char* term, const char* field, const double* K,
+                                         int nk) {
+    return guarded([&] {
+        if (!m || !term || !K || (nk != 4 && nk != 9)) throw Error(GPF_EINVAL, "bad argument");
+        UserTerm u;
+        u.kind = kTensorialHessian;
+        u.field = field ? field : "";
+        u.hess_len = nk;
+        for (int k = 0; k < nk; ++k) u.hess[k] = K[k];
         m->m.register_implicit_term(term, u);
     });
 }

@@ -26,7 +26,8 @@ enum TermKind {
     kHomogeneousModulus = 2,
     kVolumeConserving = 3,   // VolumeConservingLP (pf/volumeConserving.go)
     kSpectralViscosity = 4,  // SpectralViscosity (pf/spectralViscosity.go)
-    kNumTermKinds = 5
+    kTensorialHessian = 5,   // TensorialHessian (pf/tensorialHessian.go)
+    kNumTermKinds = 6
 };
 
 struct TermDesc {
@@ -64,6 +65,8 @@ struct UserTerm {
     double eps = 0.0;        // SpectralViscosity.Eps
     double threshold = 0.0;  // SpectralViscosity.DissipationThreshold
     int power = 0;           // SpectralViscosity.Power
+    double hess[9] = {};     // TensorialHessian.K, row-major d x d
+    int hess_len = 0;        // len(K): 4 (d = 2) or 9 (d = 3)
 };
 
 struct SubStringDelimiter {

@@ -59,6 +59,17 @@ struct SvPoly {
     int lap[kMaxSv];
 };
 
+// implicit TensorialHessian terms (tensorialHessian.go:37-60 with the constructFunc
+// prefixes): sum_t coef_t L^lap_t [sum_j -4 pi^2 f_j^2 K_jj + sum_{j<k} -8 pi^2 f_j f_k K_jk]
+constexpr int kMaxHess = 2;
+struct HessPoly {
+    int n = 0;
+    double coef[kMaxHess];
+    int lap[kMaxHess];
+    double K[kMaxHess][9];  // row-major, leading dimension kd
+    int kd[kMaxHess];
+};
+
 // half-spectrum k-space update
 //   dst = filt * (alpha * cur + beta * sum_t coef_t L^pow_t B_t) / (1 - gamma * D(k))
 // B_t == null means the brick is the constant 1.
@@ -74,6 +85,7 @@ struct KUpdateArgs {
     double alpha = 1.0, beta = 0.0, gamma = 0.0;
     KPoly denom;
     SvPoly sv;
+    HessPoly hs;
     const double* filter = nullptr;
     KGrid g;
     long nmain = 0, nk = 0;            // main elements; total (main + nyq)
@@ -96,6 +108,7 @@ struct EtdArgs {
     const double* dmul[kMaxUTerms] = {};
     KPoly denom;
     SvPoly sv;
+    HessPoly hs;
     double dt = 0.0;
     KGrid g;
     long nmain = 0, nk = 0;

@@ -83,8 +83,10 @@ __device__ __forceinline__ double sv_interp(double f, double peak) {
     return 2.0 * x * x - 3.0 * x * x * x;
 }
 
-// D(k) = sum of the LAP^p terms and the SpectralViscosity terms (f2 = |f|^2)
-__device__ __forceinline__ double denominator(const KPoly& p, const SvPoly& sv, double L, double f2) {
+// D(k) = sum of the LAP^p, SpectralViscosity and TensorialHessian terms (f2 = |f|^2; f has
+// ndim components in the reference's Freq order)
+__device__ __forceinline__ double denominator(const KPoly& p, const SvPoly& sv, const HessPoly& hs, int ndim,
+                                              const double f[3], double L, double f2) {
     double D = 0.0;
     for (int t = 0; t < p.n; ++t) D += p.coef[t] * ipow(L, p.pow[t]);
     if (sv.n) {
@@ -92,7 +94,36 @@ __device__ __forceinline__ double denominator(const KPoly& p, const SvPoly& sv, 
         for (int t = 0; t < sv.n; ++t)
             D += sv.coef[t] * ipow(L, sv.lap[t]) * sv_interp(fr, sv.thr[t]) * go_ipow(fr, sv.pw[t]);
     }
+    for (int t = 0; t < hs.n; ++t) {
+        const int d = hs.kd[t];
+        double h = 0.0;
+        for (int j = 0; j < ndim; ++j) h += -4.0 * kPi * kPi * f[j] * f[j] * hs.K[t][j * d + j];
+        for (int j = 0; j < ndim; ++j)
+            for (int k = j + 1; k < ndim; ++k) h += -8.0 * kPi * kPi * f[j] * f[k] * hs.K[t][j * d + k];
+        D += hs.coef[t] * ipow(L, hs.lap[t]) * h;
+    }
     return D;
+}
+
+// The reference's Freq gives the Nyquist index +0.5 for k and for its conjugate -k, so an
+// off-diagonal f_j f_k with exactly one Nyquist component changes sign between the two and
+// D(k) != D(-k) there: the reference's update is not Hermitian and its fields pick up an
+// imaginary part.  The device keeps real fields, i.e. the Hermitian part of the reference's
+// update, which is the mean of the update evaluated with D(k) and with D(-k).  conj_freq
+// gives the Freq vector of -k; asym_row tells whether D can differ at k.
+__device__ __forceinline__ bool asym_row(const HessPoly& hs, int ndim, const double f[3]) {
+    if (hs.n == 0) return false;
+    bool any = false, all = true;
+    for (int j = 0; j < ndim; ++j) {
+        const bool nyq = (f[j] == 0.5);
+        any |= nyq;
+        all &= nyq;
+    }
+    return any && !all;
+}
+
+__device__ __forceinline__ void conj_freq(int ndim, const double f[3], double fc[3]) {
+    for (int j = 0; j < 3; ++j) fc[j] = (j < ndim && f[j] != 0.5) ? -f[j] : f[j];
 }
 
 template <typename T>
@@ -164,8 +195,14 @@ __global__ void k_kupdate(KUpdateArgs<T> a) {
                 rx += m;
             }
         }
-        const double D = denominator(a.denom, a.sv, L, f2);
+        const double D = denominator(a.denom, a.sv, a.hs, a.g.ndim, f, L, f2);
         double s = 1.0 / (1.0 - a.gamma * D);
+        if (asym_row(a.hs, a.g.ndim, f)) {
+            double fc[3];
+            conj_freq(a.g.ndim, f, fc);
+            const double Dc = denominator(a.denom, a.sv, a.hs, a.g.ndim, fc, L, f2);
+            s = 0.5 * (s + 1.0 / (1.0 - a.gamma * Dc));
+        }
         if (a.filter) s *= vandeven(a.filter, sqrt(f2) * 2.0 / kPi);
         double cx = 0.0, cy = 0.0;
         if (a.cur) {
@@ -174,6 +211,21 @@ __global__ void k_kupdate(KUpdateArgs<T> a) {
             cy = (double)c.y;
         }
         a.dst[i] = mk<T>((T)(s * (a.alpha * cx + a.beta * rx)), (T)(s * (a.alpha * cy + a.beta * ry)));
+    }
+}
+
+// e^{D dt} and nonlinearIntegral (implicitEuler.go:151-162) for real D
+__device__ __forceinline__ void etd_update(double D, double dt, double rx, double ry, double px, double py, double& e,
+                                           double& ix, double& iy) {
+    e = exp(D * dt);
+    if (fabs(D) < 1e-5) {
+        ix = 0.5 * dt * (rx + px * e);
+        iy = 0.5 * dt * (ry + py * e);
+    } else {
+        const double c1 = (e - 1.0) / D;
+        const double c2 = (e - D * dt - 1.0) / (D * D);
+        ix = px * c1 + (rx - px) / dt * c2;
+        iy = py * c1 + (ry - py) / dt * c2;
     }
 }
 
@@ -201,19 +253,17 @@ __global__ void k_etd_residual(EtdArgs<T> a) {
                 rx += m;
             }
         }
-        const double D = denominator(a.denom, a.sv, L, f2);
-        const double e = exp(D * a.dt);
         const cpx<T> p = a.rprev[i];
         const double px = (double)p.x, py = (double)p.y;
-        double ix, iy;
-        if (fabs(D) < 1e-5) {
-            ix = 0.5 * a.dt * (rx + px * e);
-            iy = 0.5 * a.dt * (ry + py * e);
-        } else {
-            const double c1 = (e - 1.0) / D;
-            const double c2 = (e - D * a.dt - 1.0) / (D * D);
-            ix = px * c1 + (rx - px) / a.dt * c2;
-            iy = py * c1 + (ry - py) / a.dt * c2;
+        double e, ix, iy;
+        etd_update(denominator(a.denom, a.sv, a.hs, a.g.ndim, f, L, f2), a.dt, rx, ry, px, py, e, ix, iy);
+        if (asym_row(a.hs, a.g.ndim, f)) {
+            double fc[3], ec, icx, icy;
+            conj_freq(a.g.ndim, f, fc);
+            etd_update(denominator(a.denom, a.sv, a.hs, a.g.ndim, fc, L, f2), a.dt, rx, ry, px, py, ec, icx, icy);
+            e = 0.5 * (e + ec);
+            ix = 0.5 * (ix + icx);
+            iy = 0.5 * (iy + icy);
         }
         const cpx<T> o = a.orig[i];
         const cpx<T> c = a.cur[i];

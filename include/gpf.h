@@ -86,6 +86,11 @@ int gpf_model_register_volume_conserving(gpf_model* m, const char* term, const c
  * implicit multiplier -eps Q(|f|) |f|^power with Tadmor's interpolant Q between
  * threshold / 3 and threshold */
 int gpf_model_register_spectral_viscosity(gpf_model* m, const char* term, double eps, double threshold, int power);
+/* TensorialHessian + RegisterImplicitTerm (pf/tensorialHessian.go:19-66): the implicit
+ * multiplier sum_ij K_ij (i 2 pi f_i)(i 2 pi f_j) of d^2/dx_i dx_j; K is row-major with
+ * nk = 4 (2D) or 9 (3D) entries; field is kept for the reference's struct and unused */
+int gpf_model_register_tensorial_hessian(gpf_model* m, const char* term, const char* field, const double* K,
+                                         int nk);
 /* Model.RegisterFunction (pf/model.go:400-412) for a polynomial of the fields' real parts:
  * value = sum_t coefs[t] * prod_f re(field_f)^powers[t * nfields + f]; the device cannot run
  * Go closures, so the function is given as data. */
@@ -97,7 +102,7 @@ int gpf_model_init(gpf_model* m);
 /* introspection of the parsed model (Model.AllFieldNames, Model.RHS) */
 typedef struct {
     int kind;          /* 0 ConcreteTerm, 1 SquaredGradient, 2 HomogeneousModulusLinElast,
-                          3 VolumeConservingLP, 4 SpectralViscosity */
+                          3 VolumeConservingLP, 4 SpectralViscosity, 5 TensorialHessian */
     double coef_re;    /* concrete: sign * prod Pow(scalar, p); user terms: +-1 */
     double coef_im;
     int brick;         /* brick index (fields, then derived fields) or -1 */

@@ -87,6 +87,7 @@ def _declare(L) -> None:
     L.or_eval_user_term.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
     L.or_add_volcons.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_long]
     L.or_add_svisc.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int]
+    L.or_add_hessian.argtypes = [C.c_void_p, _dp, C.c_int]
     L.or_volcons_multiplier.argtypes = [C.c_void_p, C.c_int]
     L.or_volcons_multiplier.restype = C.c_double
     L.or_on_step_finished.argtypes = [C.c_void_p]
@@ -248,6 +249,11 @@ class OracleSolver:
                                  float(v["dt"]), int(v["num_nodes"]))
         for v in model.user_tables.get(P.KIND_SVISC, []):
             lib().or_add_svisc(self.h, float(v["eps"]), float(v["threshold"]), int(v["power"]))
+        for v in model.user_tables.get(P.KIND_HESS, []):
+            K = np.zeros(9)
+            k = np.asarray(v["K"], dtype=np.float64).reshape(-1)
+            K[:k.size] = k
+            lib().or_add_hessian(self.h, K.ctypes.data_as(_dp), int(k.size))
         for name, data in fields.items():
             self.set_field(name, data)
 

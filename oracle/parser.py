@@ -248,8 +248,8 @@ def get_known_prefixes(s: str) -> List[str]:
 # Model (pf/model.go) restatement producing a flat term description
 # ---------------------------------------------------------------------------
 
-KIND_CONCRETE, KIND_SQGRAD, KIND_ELAST, KIND_VOLCONS, KIND_SVISC = 0, 1, 2, 3, 4
-_USER_KINDS = (KIND_SQGRAD, KIND_ELAST, KIND_VOLCONS, KIND_SVISC)
+KIND_CONCRETE, KIND_SQGRAD, KIND_ELAST, KIND_VOLCONS, KIND_SVISC, KIND_HESS = 0, 1, 2, 3, 4, 5
+_USER_KINDS = (KIND_SQGRAD, KIND_ELAST, KIND_VOLCONS, KIND_SVISC, KIND_HESS)
 
 
 @dataclass

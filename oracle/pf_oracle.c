@@ -462,7 +462,7 @@ static int solve3x2(double A[3][3], double B[3][2], double X[3][2])
 /* Model engine                                                        */
 /* ------------------------------------------------------------------ */
 
-enum { TERM_CONCRETE = 0, TERM_SQGRAD = 1, TERM_ELAST = 2, TERM_VOLCONS = 3, TERM_SVISC = 4 };
+enum { TERM_CONCRETE = 0, TERM_SQGRAD = 1, TERM_ELAST = 2, TERM_VOLCONS = 3, TERM_SVISC = 4, TERM_HESS = 5 };
 
 typedef struct {
     int kind;
@@ -501,6 +501,12 @@ typedef struct {
     int power;
 } or_svisc;
 
+/* TensorialHessian (tensorialHessian.go:33-36): K row-major, len 4 or 9 */
+typedef struct {
+    double K[9];
+    int len;
+} or_hess;
+
 typedef struct {
     int ndim;
     int dims[3];
@@ -522,9 +528,10 @@ typedef struct {
     int nsq, nel;
     or_sqgrad *sq;
     or_elast *el;
-    int nvc, nsv;
+    int nvc, nsv, nhs;
     or_volcons *vc;
     or_svisc *sv;
+    or_hess *hs;
     /* stepper */
     int stepper; /* 0 euler, 1 rk4 */
     double dt;
@@ -624,7 +631,7 @@ void or_model_destroy(void *h)
     free(m->sq);
     for (int e = 0; e < m->nel; ++e) { free(m->el[e].lag); free(m->el[e].inject_disp); }
     free(m->el);
-    free(m->vc); free(m->sv);
+    free(m->vc); free(m->sv); free(m->hs);
     fftnd_destroy(m->ft);
     free(m->tmp); free(m->tmp2); free(m->work);
     free(m);
@@ -827,6 +834,16 @@ void or_add_svisc(void *h, double eps, double threshold, int power)
 
 double or_volcons_multiplier(void *h, int v) { return ((or_model *)h)->vc[v].multiplier; }
 
+/* TensorialHessian + RegisterImplicitTerm, in the order of the model's user table */
+void or_add_hessian(void *h, const double *K, int len)
+{
+    or_model *m = (or_model *)h;
+    m->hs = (or_hess *)realloc(m->hs, sizeof(or_hess) * (size_t)(m->nhs + 1));
+    or_hess *v = &m->hs[m->nhs++];
+    memcpy(v->K, K, sizeof(double) * 9);
+    v->len = len;
+}
+
 /* interpolant (spectralViscosity.go:29-39) */
 double or_sv_interpolant(double f, double peak)
 {
@@ -883,6 +900,18 @@ static void eval_term(or_model *m, const or_term *t, zc *out)
             for (int a = 0; a < m->ndim; ++a) d += f[a] * f[a];
             double fr = sqrt(d);
             out[i] = -v->eps * or_sv_interpolant(fr, v->threshold) * go_pow_int(fr, v->power);
+        }
+    } else if (t->kind == TERM_HESS) {
+        /* TensorialHessian.Construct (tensorialHessian.go:37-60); GetCoeff's d is 3 only for len 9 */
+        const or_hess *v = &m->hs[t->user];
+        int d = (v->len == 9) ? 3 : 2;
+        for (long i = 0; i < n; ++i) {
+            ref_freq(m->ndim, m->dims, i, f);
+            zc acc = 0.0;
+            for (int j = 0; j < m->ndim; ++j) acc += -4.0 * OR_PI * OR_PI * f[j] * f[j] * v->K[j * d + j];
+            for (int j = 0; j < m->ndim; ++j)
+                for (int k = j + 1; k < m->ndim; ++k) acc += -8.0 * OR_PI * OR_PI * f[j] * f[k] * v->K[j * d + k];
+            out[i] = acc;
         }
     } else {
         eval_elast(m, &m->el[t->user], out);

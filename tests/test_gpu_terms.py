@@ -150,3 +150,84 @@ def test_volume_conserving_stepper_propagate_skips_the_update():
     a.StepperPropagate(10)
     b.StepperPropagate(10)
     assert np.array_equal(a.Field("phase").real, b.Field("phase").real)
+
+
+HESS_CASES = {
+    "2d_diag": ([32, 32], [1.0, 0.0, 0.0, 0.5]),
+    "3d_diag": ([16, 16, 16], [1.0, 0.0, 0.0, 0.0, 0.7, 0.0, 0.0, 0.0, 0.4]),
+    "2d_offdiag": ([32, 32], [1.0, 0.3, 0.3, 0.5]),
+    "3d_offdiag": ([16, 16, 16], [1.0, 0.2, 0.0, 0.2, 0.7, 0.1, 0.0, 0.1, 0.4]),
+}
+
+
+def hess_models(dims, K):
+    n = int(np.prod(dims))
+    c0 = 0.3 * (np.random.default_rng(9).random(n) - 0.5)
+    eq = "dconc/dt = HESSIAN + LAP conc^3 - LAP^2 conc"
+    pm, om = G.Model(), P.Model()
+    pm.AddField("conc", c0)
+    om.add_field("conc", n)
+    pm.RegisterImplicitTerm("HESSIAN", G.TensorialHessian("conc", K))
+    om.register_implicit_term("HESSIAN", P.KIND_HESS, K=K)
+    pm.AddEquation(eq)
+    om.add_equation(eq)
+    pm.Init()
+    om.init()
+    return pm, om, c0
+
+
+@pytest.mark.parametrize("case", list(HESS_CASES))
+@pytest.mark.parametrize("stepper", ["euler", "rk4"])
+def test_tensorial_hessian_parity(case, stepper):
+    """Anisotropic diffusion through an implicit TensorialHessian term.
+
+    With a diagonal K the multiplier is Hermitian and the device matches the oracle to 1e-10.
+    An off-diagonal K_jk multiplies by f_j f_k, which the reference's Freq (+0.5 for the
+    Nyquist index of k and of -k) makes non-Hermitian on Nyquist rows: the reference's fields
+    pick up imaginary parts (about 1e-6 of the field here) that feed back into the real part
+    (about 1e-8 after 10 steps).  The device keeps real fields, so it is compared with the
+    oracle projected onto real fields after every step (1e-10 Euler, 1e-9 RK4, whose stages
+    keep the within-step coupling) and with the plain oracle at 1e-7."""
+    dims, K = HESS_CASES[case]
+    pm, om, c0 = hess_models(dims, K)
+    dt, steps = 0.02, 10
+    s = G.Solver(pm, dims, dt, stepper=stepper)
+    plain = OracleSolver(om, dims, {"conc": c0.astype(complex)}, dt, stepper=stepper)
+    real = OracleSolver(om, dims, {"conc": c0.astype(complex)}, dt, stepper=stepper)
+    s.Propagate(steps)
+    plain.solver_propagate(steps)
+    for _ in range(steps):
+        real.solver_propagate(1)
+        real.set_field("conc", real.field("conc").real.astype(complex))
+    got = s.Field("conc").real
+    if "diag" in case and "off" not in case:
+        assert maxrel(got, plain.field("conc").real) <= 1e-10
+    else:
+        assert maxrel(got, real.field("conc").real) <= (1e-10 if stepper == "euler" else 1e-9)
+        assert maxrel(got, plain.field("conc").real) <= 1e-7
+
+
+@pytest.mark.parametrize("case", ["2d_offdiag", "3d_offdiag"])
+def test_tensorial_hessian_implicit_euler(case):
+    """ImplicitEuler's unknowns are real vectors (implicitEuler.go:48-66), so its residual
+    sees the Hermitian part of the multiplier: the device matches the oracle to 1e-9."""
+    dims, K = HESS_CASES[case]
+    pm, om, c0 = hess_models(dims, K)
+    dt = 0.02
+    s = G.Solver(pm, dims, dt, stepper="implicitEuler")
+    s.SetNewtonKrylov(Tol=1e-12)
+    o = ImplicitEulerOracle(om, dims, {"conc": c0.astype(complex)}, dt, f_tol=1e-12)
+    for _ in range(3):
+        s.Propagate(1)
+        o.step()
+    assert maxrel(s.Field("conc").real, o.field("conc").real) <= 1e-9
+
+
+def test_tensorial_hessian_checks_k():
+    m = G.Model()
+    m.AddField("conc", np.zeros(16 ** 3))
+    m.RegisterImplicitTerm("HESSIAN", G.TensorialHessian("conc", [1.0, 0.0, 0.0, 1.0]))
+    m.AddEquation("dconc/dt = HESSIAN")
+    m.Init()
+    with pytest.raises(G.GpfError):
+        G.Solver(m, [16, 16, 16], 0.1)   # a 2 x 2 K on a 3D grid

@@ -111,3 +111,50 @@ def test_volcons_indicator_must_exist():
     m.add_equation("dphase/dt = CONS")
     with pytest.raises(P.ParseError):
         m.init()
+
+
+def _hess_data(N):
+    """GetData (tensorialHessian_test.go:11-41): a separable bump and its second derivatives / N^2."""
+    i = np.arange(N * N)
+    y = (i % N) / N
+    x = (i // N) / N
+    px = 16.0 * (x * x - 2.0 * x ** 3 + x ** 4)
+    py = 16.0 * (y * y - 2.0 * y ** 3 + y ** 4)
+    d = {"data": px * py}
+    d["dx2"] = 16.0 * (2.0 - 12.0 * x + 12 * x * x) * py / (N * N)
+    d["dy2"] = px * 16.0 * (2.0 - 12.0 * y + 12.0 * y * y) / (N * N)
+    d["dxdy"] = 16.0 * (2.0 * x - 6.0 * x * x + 4.0 * x ** 3) * 16.0 * (2.0 * y - 6.0 * y * y + 4.0 * y ** 3) / (N * N)
+    return d
+
+
+@pytest.mark.parametrize("K,res,tol", [([1.0, 0.0, 0.0, 0.0], "dx2", 1e-3), ([0.0, 0.0, 0.0, 1.0], "dy2", 1e-3),
+                                       ([0.0, 0.5, 0.5, 0.0], "dxdy", 1e-6)])
+def test_tensorial_hessian(K, res, tol):
+    """TestTensorialHessian (tensorialHessian_test.go:43-103): the multiplier times FFT(H),
+    inverse-transformed, is the closed-form second derivative."""
+    N = 64
+    d = _hess_data(N)
+    m = P.Model()
+    m.add_field("myfield", N * N)
+    m.register_implicit_term("HESSIAN", P.KIND_HESS, K=K)
+    m.add_equation("dmyfield/dt = HESSIAN")
+    m.init()
+    o = E.OracleSolver(m, [N, N], {"myfield": np.zeros(N * N, dtype=complex)}, 0.1)
+    mult = o.eval_user_term(P.KIND_HESS, 0)
+    Hk = np.fft.fft2(d["data"].reshape(N, N)).reshape(-1)
+    out = np.fft.ifft2((mult * Hk).reshape(N, N)).reshape(-1)
+    assert np.abs(out.real - d[res]).max() <= tol and np.abs(out.imag).max() <= tol
+
+
+def test_hessian_with_model():
+    """TestHessianWithModel (tensorialHessian_test.go:105-150): HESSIAN is the one implicit term."""
+    N = 16
+    m = P.Model()
+    m.add_field("conc1", N * N)
+    m.add_field("conc2", N * N)
+    m.register_implicit_term("HESSIAN", P.KIND_HESS, K=[1.0, 2.0, 2.0, 2.0])
+    m.add_equation("dconc1/dt = HESSIAN")
+    m.add_equation("dconc2/dt = -conc2")
+    m.init()
+    terms, denum = m.rhs[0]
+    assert len(denum) == 1 and len(terms) == 0
