@@ -80,6 +80,8 @@ def _declare(L) -> None:
                                                           C.c_long]),
         "gpf_model_register_spectral_viscosity": (C.c_int, [_vp, C.c_char_p, C.c_double, C.c_double, C.c_int]),
         "gpf_model_register_tensorial_hessian": (C.c_int, [_vp, C.c_char_p, C.c_char_p, _dp, C.c_int]),
+        "gpf_model_register_white_noise": (C.c_int, [_vp, C.c_char_p, C.c_double, C.c_ulonglong]),
+        "gpf_model_register_conservative_noise": (C.c_int, [_vp, C.c_char_p, C.c_double, C.c_int, C.c_ulonglong]),
         "gpf_model_register_polynomial": (C.c_int, [_vp, C.c_char_p, C.c_int, _dp, _dp, C.c_int]),
         "gpf_model_init": (C.c_int, [_vp]),
         "gpf_model_num_bricks": (C.c_int, [_vp, _ip, _ip]),
@@ -191,6 +193,27 @@ class SpectralViscosity:
 
 
 @dataclass
+class WhiteNoise:
+    """pf.WhiteNoise (noise.go:10-22): Gaussian noise with <Y Y'> = 2 Strength delta delta.
+    Register it with Model.RegisterFunction(name, noise); Seed keys the device generator."""
+    Strength: float
+    Seed: int = 0
+
+
+@dataclass
+class ConservativeNoise:
+    """pf.ConservativeNoise (noise.go:24-100): the discrete divergence of Dim Gaussian current
+    fields.  Register it with Model.RegisterExplicitTerm(name, noise)."""
+    Strength: float
+    Dim: int
+    Seed: int = 0
+
+
+def NewConservativeNoise(strength: float, dim: int, seed: int = 0) -> ConservativeNoise:
+    return ConservativeNoise(strength, dim, seed)
+
+
+@dataclass
 class TensorialHessian:
     """pf.TensorialHessian (tensorialHessian.go:19-66): the implicit multiplier of
     sum_ij K_ij d^2/dx_i dx_j; K row-major with 4 (2D) or 9 (3D) entries."""
@@ -292,7 +315,22 @@ class Model:
     def RegisterImplicitTerm(self, name: str, term) -> None:
         self._register(name, term, implicit=True)
 
+    def RegisterFunction(self, name: str, fn) -> None:
+        """Model.RegisterFunction (model.go:400-412) for the device-evaluable functions: a
+        WhiteNoise (its Generate), or use RegisterPolynomial for polynomials of the fields."""
+        if isinstance(fn, WhiteNoise):
+            _check(lib().gpf_model_register_white_noise(self._h, name.encode(), float(fn.Strength),
+                                                        int(fn.Seed) & 0xFFFFFFFFFFFFFFFF))
+            return
+        raise TypeError("RegisterFunction takes a WhiteNoise here; register polynomials with RegisterPolynomial")
+
     def _register(self, name: str, term, implicit: bool) -> None:
+        if isinstance(term, ConservativeNoise):
+            if implicit:
+                raise GpfError(1, "ConservativeNoise is an explicit term")
+            _check(lib().gpf_model_register_conservative_noise(self._h, name.encode(), float(term.Strength),
+                                                               int(term.Dim), int(term.Seed) & 0xFFFFFFFFFFFFFFFF))
+            return
         if isinstance(term, VolumeConservingLP):
             if implicit:
                 raise GpfError(1, "VolumeConservingLP is an explicit term")

@@ -275,6 +275,7 @@ static FusedPlan compile_fused(const Model& m, int stepper,
             continue;
         }
         const DerivedDesc& d = m.derived[t.brick - 1];
+        if (d.kind == 2) return fail("noise source");
         if (src_lap >= 0 && src_lap != pw) return fail("explicit sources with different LAP powers");
         src_lap = pw;
         for (const auto& dt : d.terms) {
@@ -479,6 +480,8 @@ private:
     std::vector<Buf<double>> vc_state_;
     Buf<double> vc_part_;
     std::vector<DerivedArgs<T>> dargs_;
+    std::vector<unsigned> dcount_;  // syncs so far of each derived field (the noise counter)
+    std::vector<std::vector<int>> cn_bricks_;  // ConservativeNoise: current-field bricks per occurrence
     bool kvalid_ = false;
     std::vector<Buf<cpx<T>>> ks_;
     std::vector<Buf<T>> rs_;
@@ -607,6 +610,12 @@ Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gp
             HIPCHK(hipMemcpy(vc_state_.back().get(), init, sizeof(init), hipMemcpyHostToDevice));
         }
         if (!m.user_table[kVolumeConserving].empty()) vc_part_.alloc(1024);
+        for (const auto& c : m.user_table[kConservativeNoise]) {
+            std::vector<int> b;
+            for (int k = 0; k < c.cn_dim; ++k) b.push_back(m.brick_index(c.cn_names[k]));
+            if (c.cn_dim > g_.ndim) throw Error(GPF_EINVAL, "ConservativeNoise.Dim exceeds the grid's dimension");
+            cn_bricks_.push_back(b);
+        }
         if (stepper_ == GPF_RK4)
             for (int i = 0; i < nf_; ++i) {
                 INIT_.emplace_back(g_.nk);
@@ -646,7 +655,15 @@ Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gp
             }
             for (int f = 0; f < nf_; ++f) a.f[f] = F_[f].get();
             make_dense_derived(a);
+            if (dd.kind == 2) {
+                a.noise = 1;
+                a.dense = 0;
+                a.noise_std = dd.noise_std;
+                a.seed = dd.noise_seed;
+                a.stream = (unsigned)d;
+            }
             dargs_.push_back(a);
+            dcount_.push_back(0);
         }
         pass_names_ = {"general_step"};
     }
@@ -1333,6 +1350,7 @@ void Engine<T>::sync_derived() {
     std::vector<const T*> in;
     std::vector<cpx<T>*> out;
     for (int d = 0; d < nd_; ++d) {
+        dargs_[d].counter = dcount_[d]++;
         HIPCHK(launch_derived<T>(dargs_[d], st_));
         in.push_back(Dr_[d].get());
         out.push_back(K_[nf_ + d].get());
@@ -1436,10 +1454,26 @@ template <class A>
 void Engine<T>::terms_into(int eq, A& a) {
     a.nterm = 0;
     for (const auto& t : eqs_[eq].terms) {
+        if (t.kind == kConservativeNoise) {
+            // sum_comp i 2 sin(pi f_comp) FFT(current_comp) (noise.go:60-78)
+            for (size_t c = 0; c < cn_bricks_[t.user].size(); ++c) {
+                if (a.nterm >= kMaxUTerms) throw Error(GPF_EINVAL, "too many terms in one equation");
+                const int q = a.nterm++;
+                a.coef[q] = t.coef;
+                a.pw[q] = t.pw;
+                a.dmul[q] = nullptr;
+                a.tmode[q] = 1;
+                a.tcomp[q] = (int)c;
+                a.brick[q] = K_[cn_bricks_[t.user][c]].get();
+            }
+            continue;
+        }
+        if (a.nterm >= kMaxUTerms) throw Error(GPF_EINVAL, "too many terms in one equation");
         const int q = a.nterm++;
         a.coef[q] = t.coef;
         a.pw[q] = t.pw;
         a.dmul[q] = nullptr;
+        a.tmode[q] = 0;
         if (t.kind == kConcrete) {
             a.brick[q] = (t.brick >= 0) ? K_[t.brick].get() : nullptr;
         } else if (t.kind == kSquaredGradient) {
@@ -2324,6 +2358,21 @@ int gpf_model_register_tensorial_hessian(gpf_model* m, const char* term, const c
         u.hess_len = nk;
         for (int k = 0; k < nk; ++k) u.hess[k] = K[k];
         m->m.register_implicit_term(term, u);
+    });
+}
+
+int gpf_model_register_white_noise(gpf_model* m, const char* name, double strength, unsigned long long seed) {
+    return guarded([&] {
+        if (!m || !name || !(strength >= 0.0)) throw Error(GPF_EINVAL, "bad argument");
+        m->m.register_white_noise(name, strength, seed);
+    });
+}
+
+int gpf_model_register_conservative_noise(gpf_model* m, const char* term, double strength, int dim,
+                                          unsigned long long seed) {
+    return guarded([&] {
+        if (!m || !term || !(strength >= 0.0)) throw Error(GPF_EINVAL, "bad argument");
+        m->m.register_conservative_noise(term, strength, dim, seed);
     });
 }
 

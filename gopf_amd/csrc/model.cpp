@@ -337,6 +337,39 @@ void Model::register_polynomial(const std::string& name,
     derived.push_back(d);
 }
 
+void Model::register_white_noise(const std::string& name, double strength, unsigned long long seed) {
+    panic_on_prefix_in_name(name);
+    if (field_data.empty()) throw ParseError("runtime error: index out of range [0] with length 0");
+    DerivedDesc d;
+    d.name = name;
+    d.kind = 2;
+    d.noise_std = std::sqrt(2.0 * strength);
+    d.noise_seed = seed;
+    derived.push_back(d);
+}
+
+void Model::register_conservative_noise(const std::string& term, double strength, int dim, unsigned long long seed) {
+    panic_on_prefix_in_name(term);
+    if (dim < 1 || dim > 3) throw ParseError("ConservativeNoise: Dim must be 1, 2 or 3");
+    if (field_data.empty()) throw ParseError("runtime error: index out of range [0] with length 0");
+    UserTerm u;
+    u.kind = kConservativeNoise;
+    u.cn_dim = dim;
+    // the reference prefixes the names with a random uint32 (noise.go:35-46); the seed's low
+    // 32 bits play that part here
+    const unsigned prefix = (unsigned)(seed & 0xffffffffu);
+    for (int c = 0; c < dim; ++c) {
+        DerivedDesc d;
+        d.name = std::to_string(prefix) + "_current_" + std::to_string(c);
+        d.kind = 2;
+        d.noise_std = std::sqrt(2.0 * strength);
+        d.noise_seed = seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(c + 1);
+        u.cn_names[c] = d.name;
+        derived.push_back(d);
+    }
+    explicit_terms[term] = u;
+}
+
 void Model::add_equation(const std::string& eq0) {
     const std::string eq = replace_all(eq0, " ", "");
     equations.push_back(eq);

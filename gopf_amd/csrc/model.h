@@ -27,7 +27,8 @@ enum TermKind {
     kVolumeConserving = 3,   // VolumeConservingLP (pf/volumeConserving.go)
     kSpectralViscosity = 4,  // SpectralViscosity (pf/spectralViscosity.go)
     kTensorialHessian = 5,   // TensorialHessian (pf/tensorialHessian.go)
-    kNumTermKinds = 6
+    kConservativeNoise = 6,  // ConservativeNoise (pf/noise.go)
+    kNumTermKinds = 7
 };
 
 struct TermDesc {
@@ -48,7 +49,10 @@ double coef_derivative(const TermDesc& t, const std::map<std::string, std::compl
 
 struct DerivedDesc {
     std::string name;
-    int kind = 0;  // 0: product of cmplx.Pow factors (util.go:38-65); 1: polynomial of real parts
+    int kind = 0;  // 0: product of cmplx.Pow factors (util.go:38-65); 1: polynomial of real parts;
+                   // 2: Gaussian noise of standard deviation noise_std, redrawn at every sync
+    double noise_std = 0.0;
+    unsigned long long noise_seed = 0;
     // terms: coef * prod field^power
     std::vector<std::pair<double, std::vector<std::pair<int, double>>>> terms;
 };
@@ -67,6 +71,8 @@ struct UserTerm {
     int power = 0;           // SpectralViscosity.Power
     double hess[9] = {};     // TensorialHessian.K, row-major d x d
     int hess_len = 0;        // len(K): 4 (d = 2) or 9 (d = 3)
+    int cn_dim = 0;          // ConservativeNoise.Dim; the current fields are derived fields
+    std::string cn_names[3];
 };
 
 struct SubStringDelimiter {
@@ -92,6 +98,11 @@ public:
     void register_implicit_term(const std::string& name, const UserTerm& t);
     void register_polynomial(const std::string& name,
                              const std::vector<std::pair<double, std::vector<double>>>& terms);
+    // RegisterFunction(name, WhiteNoise{strength}.Generate) (noise.go:14-22)
+    void register_white_noise(const std::string& name, double strength, unsigned long long seed);
+    // NewConservativeNoise + RegisterExplicitTerm(term, &noise, noise.RequiredDerivedFields(N))
+    // (noise.go:24-100): dim derived current fields "<prefix>_current_<comp>"
+    void register_conservative_noise(const std::string& term, double strength, int dim, unsigned long long seed);
     void add_equation(const std::string& eq);
     void init();
 

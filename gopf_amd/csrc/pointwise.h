@@ -25,6 +25,47 @@ struct DerivedArgs {
     // out = sum_ij dc[i][j] f0^i f1^j (f1 = 0 when nf == 1)
     int dense = 0;
     double dc[4][4] = {};
+    // Gaussian noise (WhiteNoise.Generate, noise.go:19-22, and ConservativeNoise's current
+    // fields, noise.go:86-100): out[i] = noise_std * z, z standard normal from a counter-based
+    // generator keyed by seed on (i, counter, stream); counter advances at every sync
+    int noise = 0;
+    double noise_std = 0.0;
+    unsigned long long seed = 0;
+    unsigned stream = 0, counter = 0;
+};
+
+// Philox4x32-10 (Salmon et al., SC'11) and the normal draw of one node, shared by the
+// device kernels and the host tests
+struct NoiseDraw {
+    __host__ __device__ static inline unsigned mulhi(unsigned a, unsigned b) {
+        return (unsigned)(((unsigned long long)a * b) >> 32);
+    }
+    __host__ __device__ static inline void philox(unsigned c[4], unsigned k0, unsigned k1) {
+        for (int r = 0; r < 10; ++r) {
+            if (r > 0) {
+                k0 += 0x9E3779B9u;
+                k1 += 0xBB67AE85u;
+            }
+            const unsigned hi0 = mulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
+            const unsigned hi1 = mulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+            const unsigned n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+            c[0] = n0;
+            c[1] = lo1;
+            c[2] = n2;
+            c[3] = lo0;
+        }
+    }
+    // two 53-bit uniforms in (0, 1] from the four words, then Box-Muller
+    __host__ __device__ static inline double normal(long i, unsigned counter, unsigned stream,
+                                                    unsigned long long seed) {
+        unsigned c[4] = {(unsigned)((unsigned long long)i & 0xffffffffu), (unsigned)((unsigned long long)i >> 32),
+                         counter, stream};
+        philox(c, (unsigned)(seed & 0xffffffffu), (unsigned)(seed >> 32));
+        const double s53 = 1.0 / 9007199254740992.0;
+        const double u1 = ((double)((((unsigned long long)(c[0] >> 5)) << 26) | (c[1] >> 6)) + 0.5) * s53;
+        const double u2 = ((double)((((unsigned long long)(c[2] >> 5)) << 26) | (c[3] >> 6)) + 0.5) * s53;
+        return sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925 * u2);
+    }
 };
 
 // fill DerivedArgs::dc when the term list has the dense form; returns a.dense
@@ -82,6 +123,10 @@ struct KUpdateArgs {
     double coef[kMaxUTerms];
     int pw[kMaxUTerms];
     const double* dmul[kMaxUTerms] = {};  // optional device-resident factor of a term (null = 1)
+    // tmode 1: the multiplier is i 2 sin(pi f_comp), 0 on Nyquist rows (ConservativeNoise's
+    // discrete divergence, noise.go:60-78), comp = tcomp
+    int tmode[kMaxUTerms] = {};
+    int tcomp[kMaxUTerms] = {};
     double alpha = 1.0, beta = 0.0, gamma = 0.0;
     KPoly denom;
     SvPoly sv;
@@ -106,6 +151,8 @@ struct EtdArgs {
     double coef[kMaxUTerms];
     int pw[kMaxUTerms];
     const double* dmul[kMaxUTerms] = {};
+    int tmode[kMaxUTerms] = {};
+    int tcomp[kMaxUTerms] = {};
     KPoly denom;
     SvPoly sv;
     HessPoly hs;

@@ -145,6 +145,11 @@ __device__ __forceinline__ T powT(T x, double p) {
 
 template <typename T>
 __global__ void k_derived(DerivedArgs<T> a) {
+    if (a.noise) {
+        for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.n; i += (long)gridDim.x * blockDim.x)
+            a.out[i] = (T)(a.noise_std * NoiseDraw::normal(i, a.counter, a.stream, a.seed));
+        return;
+    }
     if (a.dense) {
         // Horner in f0 of polynomials in f1
         for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.n; i += (long)gridDim.x * blockDim.x) {
@@ -187,6 +192,16 @@ __global__ void k_kupdate(KUpdateArgs<T> a) {
         for (int t = 0; t < a.nterm; ++t) {
             double m = a.coef[t] * ipow(L, a.pw[t]);
             if (a.dmul[t]) m *= a.dmul[t][0];
+            if (a.tmode[t] == 1) {
+                const double fc = f[a.tcomp[t]];
+                if (fabs(fabs(fc) - 0.5) > 1e-6) {
+                    const double w = m * 2.0 * sin(kPi * fc);
+                    const cpx<T> b = a.brick[t][i];
+                    rx -= w * (double)b.y;
+                    ry += w * (double)b.x;
+                }
+                continue;
+            }
             if (a.brick[t]) {
                 const cpx<T> b = a.brick[t][i];
                 rx += m * (double)b.x;
@@ -245,6 +260,16 @@ __global__ void k_etd_residual(EtdArgs<T> a) {
         for (int t = 0; t < a.nterm; ++t) {
             double m = a.coef[t] * ipow(L, a.pw[t]);
             if (a.dmul[t]) m *= a.dmul[t][0];
+            if (a.tmode[t] == 1) {
+                const double fc = f[a.tcomp[t]];
+                if (fabs(fabs(fc) - 0.5) > 1e-6) {
+                    const double w = m * 2.0 * sin(kPi * fc);
+                    const cpx<T> b = a.brick[t][i];
+                    rx -= w * (double)b.y;
+                    ry += w * (double)b.x;
+                }
+                continue;
+            }
             if (a.brick[t]) {
                 const cpx<T> b = a.brick[t][i];
                 rx += m * (double)b.x;

@@ -91,6 +91,17 @@ int gpf_model_register_spectral_viscosity(gpf_model* m, const char* term, double
  * nk = 4 (2D) or 9 (3D) entries; field is kept for the reference's struct and unused */
 int gpf_model_register_tensorial_hessian(gpf_model* m, const char* term, const char* field, const double* K,
                                          int nk);
+/* Model.RegisterFunction(name, WhiteNoise{strength}.Generate) (pf/noise.go:10-22): a derived
+ * field of Gaussian noise with variance 2 strength, redrawn at every SyncDerivedFields.  The
+ * reference draws from the global math/rand source; the device draws from Philox4x32-10
+ * keyed by seed on (node, draw count, derived-field index), so runs are reproducible. */
+int gpf_model_register_white_noise(gpf_model* m, const char* name, double strength, unsigned long long seed);
+/* NewConservativeNoise(strength, dim) + RegisterExplicitTerm(term, &noise,
+ * noise.RequiredDerivedFields(N)) (pf/noise.go:24-100): dim derived Gaussian current fields
+ * and the term sum_c i 2 sin(pi f_c) FFT(current_c) (0 on Nyquist rows), which conserves the
+ * field's integral */
+int gpf_model_register_conservative_noise(gpf_model* m, const char* term, double strength, int dim,
+                                          unsigned long long seed);
 /* Model.RegisterFunction (pf/model.go:400-412) for a polynomial of the fields' real parts:
  * value = sum_t coefs[t] * prod_f re(field_f)^powers[t * nfields + f]; the device cannot run
  * Go closures, so the function is given as data. */
@@ -102,7 +113,8 @@ int gpf_model_init(gpf_model* m);
 /* introspection of the parsed model (Model.AllFieldNames, Model.RHS) */
 typedef struct {
     int kind;          /* 0 ConcreteTerm, 1 SquaredGradient, 2 HomogeneousModulusLinElast,
-                          3 VolumeConservingLP, 4 SpectralViscosity, 5 TensorialHessian */
+                          3 VolumeConservingLP, 4 SpectralViscosity, 5 TensorialHessian,
+                          6 ConservativeNoise */
     double coef_re;    /* concrete: sign * prod Pow(scalar, p); user terms: +-1 */
     double coef_im;
     int brick;         /* brick index (fields, then derived fields) or -1 */

@@ -88,6 +88,11 @@ def _declare(L) -> None:
     L.or_add_volcons.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_long]
     L.or_add_svisc.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int]
     L.or_add_hessian.argtypes = [C.c_void_p, _dp, C.c_int]
+    L.or_set_noise.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_ulonglong]
+    L.or_add_cnoise.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+    L.or_philox.argtypes = [C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(C.c_uint)]
+    L.or_noise_normal.argtypes = [C.c_long, C.c_uint, C.c_uint, C.c_ulonglong]
+    L.or_noise_normal.restype = C.c_double
     L.or_volcons_multiplier.argtypes = [C.c_void_p, C.c_int]
     L.or_volcons_multiplier.restype = C.c_double
     L.or_on_step_finished.argtypes = [C.c_void_p]
@@ -249,6 +254,13 @@ class OracleSolver:
                                  float(v["dt"]), int(v["num_nodes"]))
         for v in model.user_tables.get(P.KIND_SVISC, []):
             lib().or_add_svisc(self.h, float(v["eps"]), float(v["threshold"]), int(v["power"]))
+        for d, dd in enumerate(model.derived):
+            if dd.kind == 2:
+                lib().or_set_noise(self.h, d, float(dd.noise_std), int(dd.noise_seed))
+        for v in model.user_tables.get(P.KIND_CNOISE, []):
+            b = [model.brick_index(nm) for nm in v["names"]] + [-1, -1, -1]
+            arr = (C.c_int * 3)(*b[:3])
+            lib().or_add_cnoise(self.h, len(v["names"]), arr)
         for v in model.user_tables.get(P.KIND_HESS, []):
             K = np.zeros(9)
             k = np.asarray(v["K"], dtype=np.float64).reshape(-1)

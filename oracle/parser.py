@@ -22,6 +22,8 @@ coefficients carry the same rounding as the reference closures.
 """
 from __future__ import annotations
 
+import numpy as np
+
 import cmath
 import math
 import re
@@ -248,8 +250,8 @@ def get_known_prefixes(s: str) -> List[str]:
 # Model (pf/model.go) restatement producing a flat term description
 # ---------------------------------------------------------------------------
 
-KIND_CONCRETE, KIND_SQGRAD, KIND_ELAST, KIND_VOLCONS, KIND_SVISC, KIND_HESS = 0, 1, 2, 3, 4, 5
-_USER_KINDS = (KIND_SQGRAD, KIND_ELAST, KIND_VOLCONS, KIND_SVISC, KIND_HESS)
+KIND_CONCRETE, KIND_SQGRAD, KIND_ELAST, KIND_VOLCONS, KIND_SVISC, KIND_HESS, KIND_CNOISE = 0, 1, 2, 3, 4, 5, 6
+_USER_KINDS = (KIND_SQGRAD, KIND_ELAST, KIND_VOLCONS, KIND_SVISC, KIND_HESS, KIND_CNOISE)
 
 
 @dataclass
@@ -282,8 +284,10 @@ class TermDesc:
 @dataclass
 class DerivedDesc:
     name: str
-    kind: int                       # 0 product of cmplx.Pow factors, 1 real polynomial
+    kind: int                       # 0 product of cmplx.Pow factors, 1 real polynomial, 2 Gaussian noise
     terms: List[Tuple[float, List[Tuple[int, float]]]] = dc_field(default_factory=list)
+    noise_std: float = 0.0
+    noise_seed: int = 0
 
 
 @dataclass
@@ -354,6 +358,24 @@ class Model:
         for coef, powers in terms:
             dterms.append((float(coef), [(idx[f], float(p)) for f, p in powers.items()]))
         self.derived.append(DerivedDesc(name, 1, dterms))
+
+    def register_white_noise(self, name: str, strength: float, seed: int) -> None:
+        """RegisterFunction(name, WhiteNoise{strength}.Generate) (noise.go:14-22); the draws come
+        from a counter-based generator keyed by seed."""
+        self._panic_on_prefix_in_name(name)
+        self.derived.append(DerivedDesc(name, 2, [], float(np.sqrt(2.0 * strength)), int(seed)))
+
+    def register_conservative_noise(self, term: str, strength: float, dim: int, seed: int) -> None:
+        """NewConservativeNoise + RegisterExplicitTerm with RequiredDerivedFields (noise.go:24-100)."""
+        self._panic_on_prefix_in_name(term)
+        prefix = seed & 0xFFFFFFFF
+        names = []
+        for c in range(dim):
+            name = f"{prefix}_current_{c}"
+            s = (seed + 0x9E3779B97F4A7C15 * (c + 1)) & 0xFFFFFFFFFFFFFFFF
+            self.derived.append(DerivedDesc(name, 2, [], float(np.sqrt(2.0 * strength)), s))
+            names.append(name)
+        self.explicit_terms[term] = UserTerm(KIND_CNOISE, {"names": names})
 
     def add_equation(self, eq: str) -> None:
         eq = eq.replace(" ", "")

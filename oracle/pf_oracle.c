@@ -462,7 +462,8 @@ static int solve3x2(double A[3][3], double B[3][2], double X[3][2])
 /* Model engine                                                        */
 /* ------------------------------------------------------------------ */
 
-enum { TERM_CONCRETE = 0, TERM_SQGRAD = 1, TERM_ELAST = 2, TERM_VOLCONS = 3, TERM_SVISC = 4, TERM_HESS = 5 };
+enum { TERM_CONCRETE = 0, TERM_SQGRAD = 1, TERM_ELAST = 2, TERM_VOLCONS = 3, TERM_SVISC = 4, TERM_HESS = 5,
+       TERM_CNOISE = 6 };
 
 typedef struct {
     int kind;
@@ -501,6 +502,12 @@ typedef struct {
     int power;
 } or_svisc;
 
+/* ConservativeNoise (noise.go:24-29): the bricks of its Dim current fields */
+typedef struct {
+    int dim;
+    int bricks[3];
+} or_cnoise;
+
 /* TensorialHessian (tensorialHessian.go:33-36): K row-major, len 4 or 9 */
 typedef struct {
     double K[9];
@@ -528,10 +535,15 @@ typedef struct {
     int nsq, nel;
     or_sqgrad *sq;
     or_elast *el;
-    int nvc, nsv, nhs;
+    int nvc, nsv, nhs, ncn;
     or_volcons *vc;
     or_svisc *sv;
     or_hess *hs;
+    or_cnoise *cn;
+    /* Gaussian-noise derived fields (kind 2): standard deviation, generator key, draws so far */
+    double *dnoise_std;
+    unsigned long long *dnoise_seed;
+    unsigned *dcount;
     /* stepper */
     int stepper; /* 0 euler, 1 rk4 */
     double dt;
@@ -574,6 +586,9 @@ void *or_model_create(int ndim, const int *dims, int nfields, int nderived,
     m->bricks = (zc **)calloc((size_t)nb, sizeof(zc *));
     for (int b = 0; b < nb; ++b) m->bricks[b] = (zc *)calloc((size_t)m->n, sizeof(zc));
     m->dkind = (int *)dupmem(dkind, sizeof(int) * nderived);
+    m->dnoise_std = (double *)calloc((size_t)(nderived ? nderived : 1), sizeof(double));
+    m->dnoise_seed = (unsigned long long *)calloc((size_t)(nderived ? nderived : 1), sizeof(unsigned long long));
+    m->dcount = (unsigned *)calloc((size_t)(nderived ? nderived : 1), sizeof(unsigned));
     m->dterm_start = (int *)dupmem(dterm_start, sizeof(int) * (nderived + 1));
     int nt = dterm_start[nderived];
     m->dterm_coef = (double *)dupmem(dterm_coef, sizeof(double) * nt);
@@ -631,7 +646,8 @@ void or_model_destroy(void *h)
     free(m->sq);
     for (int e = 0; e < m->nel; ++e) { free(m->el[e].lag); free(m->el[e].inject_disp); }
     free(m->el);
-    free(m->vc); free(m->sv); free(m->hs);
+    free(m->vc); free(m->sv); free(m->hs); free(m->cn);
+    free(m->dnoise_std); free(m->dnoise_seed); free(m->dcount);
     fftnd_destroy(m->ft);
     free(m->tmp); free(m->tmp2); free(m->work);
     free(m);
@@ -666,11 +682,64 @@ long or_current_step(void *h) { return ((or_model *)h)->current_step; }
 void or_set_current_step(void *h, long s) { ((or_model *)h)->current_step = s; }
 
 /* Model.SyncDerivedFields (model.go:237-241) */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+ * SC'11): ten rounds of two 32x32->64 multiplies with the Weyl key schedule */
+void or_philox(const unsigned *ctr, const unsigned *key, unsigned *out)
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0, hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* the normal draw of node i: Philox on (i, counter, stream) keyed by seed, two 53-bit
+ * uniforms in (0, 1], Box-Muller */
+double or_noise_normal(long i, unsigned counter, unsigned stream, unsigned long long seed)
+{
+    unsigned c[4] = {(unsigned)((unsigned long long)i & 0xffffffffu), (unsigned)((unsigned long long)i >> 32),
+                     counter, stream};
+    unsigned k[2] = {(unsigned)(seed & 0xffffffffu), (unsigned)(seed >> 32)}, o[4];
+    or_philox(c, k, o);
+    double s53 = 1.0 / 9007199254740992.0;
+    double u1 = ((double)((((unsigned long long)(o[0] >> 5)) << 26) | (o[1] >> 6)) + 0.5) * s53;
+    double u2 = ((double)((((unsigned long long)(o[2] >> 5)) << 26) | (o[3] >> 6)) + 0.5) * s53;
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925 * u2);
+}
+
+void or_set_noise(void *h, int d, double std, unsigned long long seed)
+{
+    or_model *m = (or_model *)h;
+    m->dnoise_std[d] = std;
+    m->dnoise_seed[d] = seed;
+}
+
+/* ConservativeNoise + RegisterExplicitTerm, in the order of the model's user table */
+void or_add_cnoise(void *h, int dim, const int *bricks)
+{
+    or_model *m = (or_model *)h;
+    m->cn = (or_cnoise *)realloc(m->cn, sizeof(or_cnoise) * (size_t)(m->ncn + 1));
+    or_cnoise *v = &m->cn[m->ncn++];
+    v->dim = dim;
+    for (int c = 0; c < 3; ++c) v->bricks[c] = (c < dim) ? bricks[c] : -1;
+}
+
 static void sync_derived(or_model *m)
 {
     for (int d = 0; d < m->nderived; ++d) {
         zc *out = m->bricks[m->nfields + d];
         int t0 = m->dterm_start[d], t1 = m->dterm_start[d + 1];
+        unsigned draw = m->dcount[d]++;
+        if (m->dkind[d] == 2) {
+            /* WhiteNoise.Generate / the current fields of ConservativeNoise (noise.go:19-22, 90-97) */
+            for (long i = 0; i < m->n; ++i)
+                out[i] = m->dnoise_std[d] * or_noise_normal(i, draw, (unsigned)d, m->dnoise_seed[d]);
+            continue;
+        }
         if (m->dkind[d] == 0) {
             /* DerivedFieldCalcFromDesc: data = 1; data *= cmplx.Pow(field, p) in order */
             int f0 = m->dfac_start[t0], f1 = m->dfac_start[t0 + 1];
@@ -900,6 +969,21 @@ static void eval_term(or_model *m, const or_term *t, zc *out)
             for (int a = 0; a < m->ndim; ++a) d += f[a] * f[a];
             double fr = sqrt(d);
             out[i] = -v->eps * or_sv_interpolant(fr, v->threshold) * go_pow_int(fr, v->power);
+        }
+    } else if (t->kind == TERM_CNOISE) {
+        /* ConservativeNoise.Construct (noise.go:60-78) */
+        const or_cnoise *v = &m->cn[t->user];
+        for (long i = 0; i < n; ++i) {
+            ref_freq(m->ndim, m->dims, i, f);
+            zc acc = 0.0;
+            for (int c = 0; c < v->dim; ++c) {
+                double fc = f[c];
+                if (fabs(fabs(fc) - 0.5) > 1e-6) {
+                    double omega_half = OR_PI * fc;
+                    acc += go_cmul(0.0 + 2.0 * sin(omega_half) * I, m->bricks[v->bricks[c]][i]);
+                }
+            }
+            out[i] = acc;
         }
     } else if (t->kind == TERM_HESS) {
         /* TensorialHessian.Construct (tensorialHessian.go:37-60); GetCoeff's d is 3 only for len 9 */
