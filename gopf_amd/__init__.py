@@ -24,7 +24,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgopf_amd.so")
 
 GPF_FP64, GPF_FP32 = 0, 1
-GPF_EULER, GPF_RK4, GPF_IMPLICIT_EULER = 0, 1, 2
+GPF_EULER, GPF_RK4, GPF_IMPLICIT_EULER, GPF_SDD = 0, 1, 2, 3
 GPF_IO_FLOAT64, GPF_IO_UINT8 = 0, 1
 KIND_CONCRETE, KIND_SQGRAD, KIND_ELAST = 0, 1, 2
 
@@ -108,6 +108,12 @@ def _declare(L) -> None:
         "gpf_solver_set_filter": (C.c_int, [_vp, _dp, C.c_long]),
         "gpf_solver_set_newton_krylov": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int]),
         "gpf_solver_newton_stats": (C.c_int, [_vp, _ip, _ip, _ip, _dp]),
+        "gpf_solver_sdd_init": (C.c_int, [_vp, _dp, _dp, C.c_long]),
+        "gpf_solver_sdd_set_orientation": (C.c_int, [_vp, _dp, C.c_long]),
+        "gpf_solver_sdd_set_params": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
+        "gpf_solver_sdd_monitor": (C.c_int, [_vp, _dp]),
+        "gpf_solver_sdd_orientation": (C.c_int, [_vp, _dp, C.c_long]),
+        "gpf_solver_sdd_dimer_length": (C.c_int, [_vp, C.c_double, _dp]),
         "gpf_solver_save_fields": (C.c_int, [_vp, C.c_char_p, C.c_int, C.c_int]),
         "gpf_solver_flush_io": (C.c_int, [_vp]),
         "gpf_solver_save_csv": (C.c_int, [_vp, C.c_char_p, C.c_int]),
@@ -425,7 +431,7 @@ class Solver:
         o = _Options()
         lib().gpf_options_default(C.byref(o))
         o.precision = GPF_FP32 if precision in ("fp32", "float32") else GPF_FP64
-        steppers = {"euler": GPF_EULER, "rk4": GPF_RK4, "implicitEuler": GPF_IMPLICIT_EULER}
+        steppers = {"euler": GPF_EULER, "rk4": GPF_RK4, "implicitEuler": GPF_IMPLICIT_EULER, "sdd": GPF_SDD}
         if stepper not in steppers:
             raise GpfError(1, "Unknown stepper scheme")
         o.stepper = steppers[stepper]
@@ -558,6 +564,44 @@ class Solver:
         r = C.c_double()
         _check(lib().gpf_solver_newton_stats(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(r)))
         return {"newton_iters": a.value, "krylov_iters": b.value, "converged": bool(c.value), "residual": r.value}
+
+    # -- pf.SDD (sdd.go) ---------------------------------------------------------
+    def _flat(self, fields) -> np.ndarray:
+        arrs = [np.asarray(f).real.astype(np.float64).reshape(-1) for f in fields]
+        return np.ascontiguousarray(np.concatenate(arrs))
+
+    def SDDInit(self, init, final) -> None:
+        """SDD.Init (sdd.go:382-402): init and final are lists with one array per field."""
+        a, b = self._flat(init), self._flat(final)
+        _check(lib().gpf_solver_sdd_init(self._h, _dptr(a), _dptr(b), a.size))
+
+    def SDDSetInitialOrientation(self, orient) -> None:
+        """SDD.SetInitialOrientation (sdd.go:404-420)."""
+        v = np.ascontiguousarray(np.asarray(orient, dtype=np.float64).reshape(-1))
+        _check(lib().gpf_solver_sdd_set_orientation(self._h, _dptr(v), v.size))
+
+    def SDDSetParams(self, Alpha: float = 0.5, Orientation: float = 1.0, DimerLength: float = 1.0,
+                     InitDimerLength: float = -1.0, MinDimerLength: float = 0.0) -> None:
+        """SDD.Alpha, TimeConstants{Orientation, DimerLength}, InitDimerLength (negative keeps
+        the value from SDDInit), MinDimerLength (sdd.go:84-130)."""
+        _check(lib().gpf_solver_sdd_set_params(self._h, float(Alpha), float(Orientation), float(DimerLength),
+                                               float(InitDimerLength), float(MinDimerLength)))
+
+    def SDDMonitor(self) -> Dict[str, float]:
+        out = np.zeros(6)
+        _check(lib().gpf_solver_sdd_monitor(self._h, _dptr(out)))
+        keys = ["MaxForce", "ForcePowerSpectrum", "MaxTorque", "FieldNorm", "FieldNormChange", "TorqueDotOrientation"]
+        return dict(zip(keys, out.tolist()))
+
+    def SDDOrientation(self) -> np.ndarray:
+        out = np.zeros(self.n * self.model.NumFields())
+        _check(lib().gpf_solver_sdd_orientation(self._h, _dptr(out), out.size))
+        return out
+
+    def SDDDimerLength(self, t: float) -> float:
+        v = C.c_double()
+        _check(lib().gpf_solver_sdd_dimer_length(self._h, float(t), C.byref(v)))
+        return v.value
 
     def Adjoint(self, nsteps: int, target, scalars: Sequence[str] = (), checkpoint_every: int = 100
                 ) -> Tuple[float, np.ndarray, Dict[str, float]]:

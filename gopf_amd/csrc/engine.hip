@@ -353,6 +353,12 @@ public:
     virtual void set_newton_krylov(int maxiter, double step_size, double tol, int stencil, int restart) = 0;
     virtual void newton_stats(int* newton_iters, int* krylov_iters, int* converged, double* residual) const = 0;
     virtual void save_fields(const char* prefix, int epoch, int format) = 0;
+    virtual void sdd_init(const double* init, const double* final, long n) = 0;
+    virtual void sdd_set_orientation(const double* v, long n) = 0;
+    virtual void sdd_set_params(double alpha, double tau_o, double tau_l, double init_len, double min_len) = 0;
+    virtual void sdd_monitor(double* out) = 0;
+    virtual void sdd_orientation(double* out, long n) = 0;
+    virtual double sdd_dimer_length(double t) const = 0;
     virtual void flush_io() = 0;
     virtual void save_csv(const char* prefix, int epoch) = 0;
 };
@@ -383,6 +389,12 @@ public:
     void set_newton_krylov(int maxiter, double step_size, double tol, int stencil, int restart) override;
     void newton_stats(int* newton_iters, int* krylov_iters, int* converged, double* residual) const override;
     void save_fields(const char* prefix, int epoch, int format) override;
+    void sdd_init(const double* init, const double* final, long n) override;
+    void sdd_set_orientation(const double* v, long n) override;
+    void sdd_set_params(double alpha, double tau_o, double tau_l, double init_len, double min_len) override;
+    void sdd_monitor(double* out) override;
+    void sdd_orientation(double* out, long n) override;
+    double sdd_dimer_length(double t) const override;
     void flush_io() override;
     void save_csv(const char* prefix, int epoch) override;
 
@@ -455,6 +467,15 @@ private:
     // residual there, Arnoldi work vector, GMRES basis (restart + 1 vectors)
     Buf<double> nx_, nfx_, ndx_, nxp_, nfp_, nw_, nv_, npart_, ndev_;
     std::vector<double> nhost_;
+    // SDD (sdd.go:84-301): orientation (nf n, unit norm), its spectra, the right-hand sides at
+    // the dimer's ends, the real torque, partial sums, device scalars and monitors
+    void sdd_step();
+    double* npart_sdd() { return sdd_part_.get() + 3 * kSddBlocks; }  // krylov_blocks(n) <= 1024 partials
+    bool sdd_ready_ = false;
+    double sdd_alpha_ = 0.5, sdd_tau_o_ = 1.0, sdd_tau_l_ = 1.0, sdd_len0_ = 0.0, sdd_min_len_ = 0.0;
+    long sdd_step_ = 0;
+    Buf<double> sdd_v_, sdd_tau_, sdd_part_, sdd_sc_, sdd_mon_;
+    std::vector<Buf<cpx<T>>> sdd_ov_, sdd_rs_, sdd_re_;
     // field snapshots (Float64IO / Uint8IO)
     std::unique_ptr<SnapshotWriter> io_;
     Buf<double> io_part_, io_mm_;
@@ -622,6 +643,21 @@ Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gp
                 FINAL_.emplace_back(g_.nk);
                 KF_.emplace_back(g_.nk);
             }
+        if (stepper_ == GPF_SDD) {
+            if (sizeof(T) != sizeof(double)) throw Error(GPF_EINVAL, "SDD runs in FP64 (GPF_FP64)");
+            if (has_filter_) throw Error(GPF_EINVAL, "SDD: Does not support modal filters");
+            for (int i = 0; i < nf_; ++i) {
+                sdd_ov_.emplace_back(g_.nk);
+                sdd_rs_.emplace_back(g_.nk);
+                sdd_re_.emplace_back(g_.nk);
+            }
+            sdd_v_.alloc((size_t)nf_ * g_.n);
+            sdd_tau_.alloc((size_t)nf_ * g_.n);
+            sdd_part_.alloc(3 * kSddBlocks + 1024);
+            sdd_sc_.alloc(16);
+            sdd_mon_.alloc(8);
+            HIPCHK(hipMemsetAsync(sdd_mon_.get(), 0, 8 * sizeof(double), st_));
+        }
         if (stepper_ == GPF_IMPLICIT_EULER) {
             if (sizeof(T) != sizeof(double)) throw Error(GPF_EINVAL, "ImplicitEuler runs in FP64 (GPF_FP64)");
             for (int i = 0; i < nf_; ++i) {
@@ -1199,6 +1235,7 @@ void Engine<T>::autotune() {
 // (pf/vandeven.go:31-40); a null table removes the filter
 template <typename T>
 void Engine<T>::set_filter(const double* table) {
+    if (table && stepper_ == GPF_SDD) throw Error(GPF_EINVAL, "SDD: Does not support modal filters");
     HIPCHK(hipStreamSynchronize(st_));
     drop_graph();
     if (!table) {
@@ -1849,6 +1886,189 @@ void Engine<T>::newton_stats(int* newton_iters, int* krylov_iters, int* converge
     if (residual) *residual = nk_res_;
 }
 
+// ---------------------------------------------------------------------------
+// SDD: shrinking-dimer dynamics (sdd.go:84-460)
+// ---------------------------------------------------------------------------
+// Step (sdd.go:159-301) on the device: the right-hand sides at the dimer's ends x -+ l v / 2,
+// the Householder-reflected force on the centre with the Sherman-Morrison solve of the
+// implicit part, the torque on the orientation and its explicit update.  Every full-spectrum
+// sum of the reference becomes a Hermitian-weighted half-spectrum sum; the scalars stay in
+// device memory, so a step needs no host synchronisation.  As in the reference, every
+// field's weighted force is built from the first field's right-hand sides (sdd.go:212-216).
+template <typename T>
+double Engine<T>::sdd_dimer_length(double t) const {
+    const double l = sdd_len0_ * std::exp(-t / sdd_tau_l_);
+    return (l < sdd_min_len_) ? sdd_min_len_ : l;
+}
+
+template <typename T>
+void Engine<T>::sdd_init(const double* init, const double* final, long n) {
+    if (stepper_ != GPF_SDD) throw Error(GPF_EINVAL, "the solver's stepper is not SDD");
+    if (n != (long)nf_ * g_.n) throw Error(GPF_EINVAL, "buffer length does not match nfields * nodes");
+    std::vector<double> v(n);
+    for (long i = 0; i < n; ++i) v[i] = final[i] - init[i];
+    sdd_set_orientation(v.data(), n);
+}
+
+template <typename T>
+void Engine<T>::sdd_set_orientation(const double* v, long n) {
+    if (stepper_ != GPF_SDD) throw Error(GPF_EINVAL, "the solver's stepper is not SDD");
+    if (n != (long)nf_ * g_.n) throw Error(GPF_EINVAL, "Inconsistent length of the passed orientaiton vector");
+    double len = 0.0;
+    for (long i = 0; i < n; ++i) len += v[i] * v[i];
+    len = std::sqrt(len);
+    std::vector<double> u(n);
+    for (long i = 0; i < n; ++i) u[i] = v[i] / len;
+    HIPCHK(hipMemcpy(sdd_v_.get(), u.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    sdd_len0_ = len;
+    sdd_ready_ = true;
+}
+
+template <typename T>
+void Engine<T>::sdd_set_params(double alpha, double tau_o, double tau_l, double init_len, double min_len) {
+    if (stepper_ != GPF_SDD) throw Error(GPF_EINVAL, "the solver's stepper is not SDD");
+    sdd_alpha_ = alpha;
+    sdd_tau_o_ = tau_o;
+    sdd_tau_l_ = tau_l;
+    if (init_len >= 0.0) sdd_len0_ = init_len;
+    sdd_min_len_ = min_len;
+}
+
+template <typename T>
+void Engine<T>::sdd_monitor(double* out) {
+    if (stepper_ != GPF_SDD) throw Error(GPF_EINVAL, "the solver's stepper is not SDD");
+    HIPCHK(hipMemcpyAsync(out, sdd_mon_.get(), 6 * sizeof(double), hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+}
+
+template <typename T>
+void Engine<T>::sdd_orientation(double* out, long n) {
+    if (stepper_ != GPF_SDD) throw Error(GPF_EINVAL, "the solver's stepper is not SDD");
+    if (n != (long)nf_ * g_.n) throw Error(GPF_EINVAL, "buffer length does not match nfields * nodes");
+    HIPCHK(hipMemcpyAsync(out, sdd_v_.get(), n * sizeof(double), hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+}
+
+template <typename T>
+void Engine<T>::sdd_step() {
+    if constexpr (std::is_same<T, double>::value) {
+        if (!sdd_ready_) throw Error(GPF_ESTATE, "SDD: The method have to be initialized first. See SDD.Init");
+        if (dt_ < 1e-16) throw Error(GPF_EINVAL, "Timestep not set in SDD.");
+        const long n = g_.n;
+        const long M = (long)nf_ * n;
+        const double inv_n = 1.0 / (double)n;
+        const double len = sdd_dimer_length((double)sdd_step_ * dt_);
+        double* part = sdd_part_.get();
+        double* sc = sdd_sc_.get();
+        double* mon = sdd_mon_.get();
+        // FieldNorm (sdd.go:371-380) before the step
+        const long nb = krylov_blocks(n);
+        for (int i = 0; i < nf_; ++i) {
+            HIPCHK(launch_multidot(F_[i].get(), n, 1, F_[i].get(), n, npart_sdd(), st_));
+            HIPCHK(launch_reduce_rows(npart_sdd(), 1, nb, sc + 8 + i, st_));
+        }
+        HIPCHK(launch_sdd_scalars(6, sc + 8, sc, mon, (double)nf_, 0.0, st_));
+        eval_elastic();  // the lagged elastic field is constant within a step
+        // orientation spectra
+        {
+            std::vector<const T*> in;
+            std::vector<cpx<T>*> out;
+            for (int i = 0; i < nf_; ++i) {
+                in.push_back(sdd_v_.get() + i * n);
+                out.push_back(sdd_ov_[i].get());
+            }
+            r2c(in, out, T(1));
+        }
+        auto shift = [&](double scale) {
+            for (int i = 0; i < nf_; ++i)
+                HIPCHK(launch_axpby(1.0, F_[i].get(), scale, sdd_v_.get() + i * n, F_[i].get(), n, st_));
+        };
+        auto transform = [&]() {  // SDD.fft: SyncDerivedFields, then FFT of the fields
+            sync_derived();
+            std::vector<const T*> in;
+            std::vector<cpx<T>*> out;
+            for (int i = 0; i < nf_; ++i) {
+                in.push_back(F_[i].get());
+                out.push_back(K_[i].get());
+            }
+            r2c(in, out, T(1));
+        };
+        auto rhs = [&](std::vector<Buf<cpx<T>>>& dst) {
+            for (int i = 0; i < nf_; ++i) {
+                eval_user_terms_for(i);
+                rhs_update(i, dst[i].get(), nullptr, 0.0, 1.0, 0.0, false, nullptr);
+            }
+        };
+        shift(-0.5 * len);
+        transform();
+        rhs(sdd_rs_);
+        shift(len);
+        transform();
+        rhs(sdd_re_);
+        shift(-0.5 * len);
+        transform();
+        auto args = [&](int i) {
+            SddArgs a;
+            a.g = g_.kg;
+            a.nmain = g_.nmain;
+            a.nk = g_.nk;
+            a.inv_n = inv_n;
+            a.denom = eqs_[i].denom;
+            a.sv = eqs_[i].sv;
+            a.hs = eqs_[i].hs;
+            a.dt = dt_;
+            a.alpha = sdd_alpha_;
+            a.len = len;
+            a.rs0 = sdd_rs_[0].get();
+            a.re0 = sdd_re_[0].get();
+            a.ov = sdd_ov_[i].get();
+            a.k = K_[i].get();
+            a.torque = sdd_rs_[i].get();
+            a.re = sdd_re_[i].get();
+            a.sc = sc;
+            a.part = part;
+            return a;
+        };
+        // the centre's update, field by field (sdd.go:205-245)
+        for (int i = 0; i < nf_; ++i) {
+            const SddArgs a = args(i);
+            HIPCHK(launch_sdd_dot(a, st_));
+            HIPCHK(launch_sdd_scalars(0, part, sc, mon, inv_n, 0.0, st_));
+            HIPCHK(launch_sdd_sm(a, st_));
+            HIPCHK(launch_sdd_scalars(1, part, sc, mon, 0.0, 0.0, st_));
+            HIPCHK(launch_sdd_update(a, st_));
+            HIPCHK(launch_sdd_scalars(i == 0 ? 5 : 2, part, sc, mon, 0.0, 0.0, st_));
+        }
+        // torque (sdd.go:252-266): rs - re - D ov l, projected off the orientation, to real space
+        for (int i = 0; i < nf_; ++i) HIPCHK(launch_sdd_torque(args(i), i == 0, st_));
+        HIPCHK(launch_sdd_scalars(3, part, sc, mon, inv_n, 0.0, st_));
+        std::vector<const cpx<T>*> tin;
+        std::vector<T*> tout;
+        for (int i = 0; i < nf_; ++i) {
+            HIPCHK(launch_sdd_project(args(i), st_));
+            tin.push_back(sdd_rs_[i].get());
+            tout.push_back(sdd_tau_.get() + i * n);
+        }
+        c2r(tin, tout, T(1) / (T)n);
+        // orientation (sdd.go:277-291)
+        HIPCHK(launch_sdd_orient(sdd_v_.get(), sdd_tau_.get(), M, dt_ / (len * sdd_tau_o_), part, st_));
+        HIPCHK(launch_sdd_scalars(4, part, sc, mon, 0.0, (double)M, st_));
+        HIPCHK(launch_normalize(sdd_v_.get(), sc + 4, sdd_v_.get(), M, st_));
+        // the fields back to real space (sdd.go:293)
+        std::vector<const cpx<T>*> in;
+        std::vector<T*> out;
+        for (int i = 0; i < nf_; ++i) {
+            in.push_back(K_[i].get());
+            out.push_back(F_[i].get());
+        }
+        c2r(in, out, T(1) / (T)n);
+        kvalid_ = true;
+        ++sdd_step_;
+    } else {
+        throw Error(GPF_EINVAL, "SDD runs in FP64 (GPF_FP64)");
+    }
+}
+
 template <typename T>
 void Engine<T>::step() {
     if (fp_.ok) {
@@ -1858,6 +2078,7 @@ void Engine<T>::step() {
     tbegin(0);
     if (stepper_ == GPF_EULER) general_step_euler();
     else if (stepper_ == GPF_RK4) general_step_rk4();
+    else if (stepper_ == GPF_SDD) sdd_step();
     else implicit_step();
     tend(0);
 }
@@ -2458,7 +2679,8 @@ int gpf_solver_create(gpf_model* m, const int* dims, int ndim, double dt, const 
         gpf_options opt;
         gpf_options_default(&opt);
         if (o) opt = *o;
-        if (opt.stepper != GPF_EULER && opt.stepper != GPF_RK4 && opt.stepper != GPF_IMPLICIT_EULER)
+        if (opt.stepper != GPF_EULER && opt.stepper != GPF_RK4 && opt.stepper != GPF_IMPLICIT_EULER &&
+            opt.stepper != GPF_SDD)
             throw Error(GPF_EINVAL, "Unknown stepper scheme");
         set_device(opt.device);
         auto s = std::make_unique<gpf_solver>();
@@ -2648,6 +2870,53 @@ int gpf_solver_newton_stats(gpf_solver* s, int* newton_iters, int* krylov_iters,
         if (!s) throw Error(GPF_EINVAL, "bad argument");
         if (s->stepper != GPF_IMPLICIT_EULER) throw Error(GPF_EINVAL, "the solver's stepper is not ImplicitEuler");
         s->eng->newton_stats(newton_iters, krylov_iters, converged, residual);
+    });
+}
+
+int gpf_solver_sdd_init(gpf_solver* s, const double* init, const double* final, long n) {
+    return guarded([&] {
+        if (!s || !init || !final) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        s->eng->sdd_init(init, final, n);
+    });
+}
+
+int gpf_solver_sdd_set_orientation(gpf_solver* s, const double* orient, long n) {
+    return guarded([&] {
+        if (!s || !orient) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        s->eng->sdd_set_orientation(orient, n);
+    });
+}
+
+int gpf_solver_sdd_set_params(gpf_solver* s, double alpha, double tau_orientation, double tau_length,
+                              double init_length, double min_length) {
+    return guarded([&] {
+        if (!s) throw Error(GPF_EINVAL, "bad argument");
+        s->eng->sdd_set_params(alpha, tau_orientation, tau_length, init_length, min_length);
+    });
+}
+
+int gpf_solver_sdd_monitor(gpf_solver* s, double* out) {
+    return guarded([&] {
+        if (!s || !out) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        s->eng->sdd_monitor(out);
+    });
+}
+
+int gpf_solver_sdd_orientation(gpf_solver* s, double* out, long n) {
+    return guarded([&] {
+        if (!s || !out) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        s->eng->sdd_orientation(out, n);
+    });
+}
+
+int gpf_solver_sdd_dimer_length(gpf_solver* s, double t, double* length) {
+    return guarded([&] {
+        if (!s || !length) throw Error(GPF_EINVAL, "bad argument");
+        *length = s->eng->sdd_dimer_length(t);
     });
 }
 

@@ -229,6 +229,46 @@ hipError_t launch_accumulate(const double* partial, int nscal, long nblocks, dou
 
 template <typename T> hipError_t launch_derived(const DerivedArgs<T>& a, hipStream_t s);
 template <typename T> hipError_t launch_kupdate(const KUpdateArgs<T>& a, hipStream_t s);
+
+// SDD.Step (sdd.go:159-301), one field's spectral part on the half spectrum (FP64).  Sums over
+// the full spectrum of Hermitian products are Hermitian-weighted half sums.  Device scalars
+// sc: [0] <work, ov> / N, [1] the Sherman-Morrison denominator, [2] its projection, [3] the
+// torque projection; every kernel writes one partial per block to part (kSddBlocks blocks).
+constexpr int kSddBlocks = 256;
+struct SddArgs {
+    KGrid g;
+    long nmain = 0, nk = 0;
+    double inv_n = 1.0;
+    KPoly denom;          // D(k) of the field's equation
+    SvPoly sv;
+    HessPoly hs;
+    double dt = 0.0, alpha = 0.5, len = 0.0;
+    const cpx<double>* rs0 = nullptr;  // field 0's GetRHS at the dimer's start and end
+    const cpx<double>* re0 = nullptr;
+    const cpx<double>* ov = nullptr;   // FFT of this field's orientation block
+    cpx<double>* k = nullptr;          // this field's spectrum, updated in place
+    cpx<double>* torque = nullptr;     // k_sdd_torque: rs_i - re_i in, torque out
+    const cpx<double>* re = nullptr;   //   re_i
+    const double* sc = nullptr;
+    double* part = nullptr;
+};
+// part[b] = sum w Re(work conj(ov)), work = alpha rs0 + (1 - alpha) re0
+hipError_t launch_sdd_dot(const SddArgs& a, hipStream_t s);
+// part[b] = sum w (2 dt D / N) |ov|^2 / (1 - dt D), part[kSddBlocks + b] = sum w (2 dt D / N) Re(conj(ov) d) / (1 - dt D)
+// with d = k + dt (work - 2 ov sc[0])
+hipError_t launch_sdd_sm(const SddArgs& a, hipStream_t s);
+// k = (d - ov sc[2] / sc[1]) / (1 - dt D); part[b] = sum w |k_new - k_old|^2 / dt^2 / N,
+// part[kSddBlocks + b] = max |k_new - k_old| / dt
+hipError_t launch_sdd_update(const SddArgs& a, hipStream_t s);
+// torque = torque - re - D ov len (torque holds rs on input); part[b] += sum w Re(torque conj(ov))
+hipError_t launch_sdd_torque(const SddArgs& a, bool first, hipStream_t s);
+// torque -= ov * sc[3]
+hipError_t launch_sdd_project(const SddArgs& a, hipStream_t s);
+// v -= c tau; part[b] = sum v_new^2, part[kSddBlocks + b] = max |tau|, part[2 kSddBlocks + b] = sum tau v_old
+hipError_t launch_sdd_orient(double* v, const double* tau, long n, double c, double* part, hipStream_t s);
+// single-thread bookkeeping of the SDD scalars and monitors (mode selects the step)
+hipError_t launch_sdd_scalars(int mode, const double* part, double* sc, double* mon, double a0, double a1,
+                              hipStream_t s);
 template <typename T> hipError_t launch_etd_residual(const EtdArgs<T>& a, hipStream_t s);
 template <typename T> hipError_t launch_kgrad(const KGradArgs<T>& a, int ndim, hipStream_t s);
 template <typename T> hipError_t launch_kelastic(const KElasticArgs<T>& a, hipStream_t s);

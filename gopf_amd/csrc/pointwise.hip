@@ -648,7 +648,213 @@ inline unsigned grid_for(long n, int thr = 256) {
     return (unsigned)g;
 }
 
+// ---------------------------------------------------------------------------
+// SDD (sdd.go:159-301)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sdd_weight(const KGrid& g, long i, long nmain) {
+    int kx, ky, kz;
+    kcoords(g, i, nmain, kx, ky, kz);
+    return (kx == 0 || 2 * kx == g.nx) ? 1.0 : 2.0;
+}
+
+__device__ __forceinline__ double sdd_D(const SddArgs& a, long i) {
+    int kx, ky, kz;
+    kcoords(a.g, i, a.nmain, kx, ky, kz);
+    double f[3];
+    ref_freq(a.g, kx, ky, kz, f);
+    const double f2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+    return denominator(a.denom, a.sv, a.hs, a.g.ndim, f, -(4.0 * kPi * kPi) * f2, f2);
+}
+
+__device__ __forceinline__ double block_reduce(double v, bool is_max) {
+    __shared__ double red[256];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = is_max ? fmax(red[threadIdx.x], red[threadIdx.x + s])
+                                                            : red[threadIdx.x] + red[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(256) k_sdd_dot(SddArgs a) {
+    double acc = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
+        const cpx<double> r0 = a.rs0[i], r1 = a.re0[i], o = a.ov[i];
+        const double wx = a.alpha * r0.x + (1.0 - a.alpha) * r1.x, wy = a.alpha * r0.y + (1.0 - a.alpha) * r1.y;
+        acc += sdd_weight(a.g, i, a.nmain) * (wx * o.x + wy * o.y);
+    }
+    const double t = block_reduce(acc, false);
+    if (threadIdx.x == 0) a.part[blockIdx.x] = t;
+}
+
+// d = k + dt (work - 2 ov dot): the Euler update after the Householder reflection (sdd.go:218-225)
+__device__ __forceinline__ cpx<double> sdd_d(const SddArgs& a, long i, double dot) {
+    const cpx<double> r0 = a.rs0[i], r1 = a.re0[i], o = a.ov[i], k = a.k[i];
+    const double wx = a.alpha * r0.x + (1.0 - a.alpha) * r1.x - 2.0 * o.x * dot;
+    const double wy = a.alpha * r0.y + (1.0 - a.alpha) * r1.y - 2.0 * o.y * dot;
+    return mk<double>(k.x + a.dt * wx, k.y + a.dt * wy);
+}
+
+__global__ void __launch_bounds__(256) k_sdd_sm(SddArgs a) {
+    const double dot = a.sc[0];
+    double accd = 0.0, accs = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
+        const double D = sdd_D(a, i);
+        const double inv = 1.0 / (1.0 - a.dt * D);
+        const cpx<double> o = a.ov[i], d = sdd_d(a, i, dot);
+        const double w = sdd_weight(a.g, i, a.nmain) * 2.0 * D * a.dt * a.inv_n * inv;
+        accd += w * (o.x * o.x + o.y * o.y);
+        accs += w * (o.x * d.x + o.y * d.y);
+    }
+    const double td = block_reduce(accd, false);
+    const double ts = block_reduce(accs, false);
+    if (threadIdx.x == 0) {
+        a.part[blockIdx.x] = td;
+        a.part[kSddBlocks + blockIdx.x] = ts;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sdd_update(SddArgs a) {
+    const double dot = a.sc[0], den = a.sc[1], proj = a.sc[2];
+    double accp = 0.0, accm = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
+        const double D = sdd_D(a, i);
+        const double inv = 1.0 / (1.0 - a.dt * D);
+        const cpx<double> o = a.ov[i], d = sdd_d(a, i, dot), k0 = a.k[i];
+        const double nx = inv * d.x - inv * o.x * proj / den, ny = inv * d.y - inv * o.y * proj / den;
+        const double dx = (nx - k0.x) / a.dt, dy = (ny - k0.y) / a.dt;
+        const double q = dx * dx + dy * dy;
+        accp += sdd_weight(a.g, i, a.nmain) * q * a.inv_n;
+        accm = fmax(accm, sqrt(q));
+        a.k[i] = mk<double>(nx, ny);
+    }
+    const double tp = block_reduce(accp, false);
+    const double tm = block_reduce(accm, true);
+    if (threadIdx.x == 0) {
+        a.part[blockIdx.x] = tp;
+        a.part[kSddBlocks + blockIdx.x] = tm;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sdd_torque(SddArgs a, int first) {
+    double acc = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
+        const double D = sdd_D(a, i);
+        const cpx<double> r = a.torque[i], e = a.re[i], o = a.ov[i];
+        const double tx = r.x - e.x - D * o.x * a.len, ty = r.y - e.y - D * o.y * a.len;
+        a.torque[i] = mk<double>(tx, ty);
+        acc += sdd_weight(a.g, i, a.nmain) * (tx * o.x + ty * o.y);
+    }
+    const double t = block_reduce(acc, false);
+    if (threadIdx.x == 0) a.part[blockIdx.x] = first ? t : a.part[blockIdx.x] + t;
+}
+
+__global__ void __launch_bounds__(256) k_sdd_project(SddArgs a) {
+    const double p = a.sc[3];
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
+        const cpx<double> t = a.torque[i], o = a.ov[i];
+        a.torque[i] = mk<double>(t.x - o.x * p, t.y - o.y * p);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sdd_orient(double* v, const double* tau, long n, double c, double* part) {
+    double s2 = 0.0, mx = 0.0, sd = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double t = tau[i], v0 = v[i];
+        sd += t * v0;
+        mx = fmax(mx, fabs(t));
+        const double v1 = v0 - c * t;
+        v[i] = v1;
+        s2 += v1 * v1;
+    }
+    const double a = block_reduce(s2, false);
+    const double b = block_reduce(mx, true);
+    const double d = block_reduce(sd, false);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a;
+        part[kSddBlocks + blockIdx.x] = b;
+        part[2 * kSddBlocks + blockIdx.x] = d;
+    }
+}
+
+// mode 0: sc[0] = sum part / N (a0 = 1/N)
+// mode 1: sc[1] = 1 + sum part, sc[2] = sum part[kSddBlocks..]
+// mode 2: mon[1] += sum part, mon[0] = max(mon[0], max part[kSddBlocks..])   (mode 5 first resets them)
+// mode 3: sc[3] = sum part / N (a0 = 1/N)
+// mode 4: mon[2] = max part[kSddBlocks..], mon[5] = sum part[2 kSddBlocks..], sc[4] = sum part (|v|^2);
+//         mon[1] = sqrt(mon[1] / a1) (a1 = nf N)
+// mode 6: field norm: fn = sum of the a0 values part[0..a0); mon[4] = mon[3] - fn; mon[3] = fn
+__global__ void k_sdd_scalars(int mode, const double* part, double* sc, double* mon, double a0, double a1) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s0 = 0.0, s1 = 0.0, m1 = 0.0, s2 = 0.0;
+    const int nparts = (mode == 6) ? (int)a0 : kSddBlocks;
+    for (int b = 0; b < nparts; ++b) {
+        s0 += part[b];
+        if (mode == 1) s1 += part[kSddBlocks + b];
+        if (mode == 2 || mode == 5 || mode == 4) m1 = fmax(m1, part[kSddBlocks + b]);
+        if (mode == 4) s2 += part[2 * kSddBlocks + b];
+    }
+    if (mode == 0) sc[0] = s0 * a0;
+    if (mode == 1) {
+        sc[1] = 1.0 + s0;
+        sc[2] = s1;
+    }
+    if (mode == 5) {
+        mon[1] = 0.0;
+        mon[0] = 0.0;
+    }
+    if (mode == 2 || mode == 5) {
+        mon[1] += s0;
+        mon[0] = fmax(mon[0], m1);
+    }
+    if (mode == 3) sc[3] = s0 * a0;
+    if (mode == 4) {
+        mon[2] = m1;
+        mon[5] = s2;
+        sc[4] = s0;
+        mon[1] = sqrt(mon[1] / a1);
+    }
+    if (mode == 6) {
+        mon[4] = mon[3] - s0;
+        mon[3] = s0;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_sdd_dot(const SddArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_sdd_dot, dim3(kSddBlocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_sdd_sm(const SddArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_sdd_sm, dim3(kSddBlocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_sdd_update(const SddArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_sdd_update, dim3(kSddBlocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_sdd_torque(const SddArgs& a, bool first, hipStream_t s) {
+    hipLaunchKernelGGL(k_sdd_torque, dim3(kSddBlocks), dim3(256), 0, s, a, first ? 1 : 0);
+    return hipGetLastError();
+}
+hipError_t launch_sdd_project(const SddArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_sdd_project, dim3(kSddBlocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_sdd_orient(double* v, const double* tau, long n, double c, double* part, hipStream_t s) {
+    hipLaunchKernelGGL(k_sdd_orient, dim3(kSddBlocks), dim3(256), 0, s, v, tau, n, c, part);
+    return hipGetLastError();
+}
+hipError_t launch_sdd_scalars(int mode, const double* part, double* sc, double* mon, double a0, double a1,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_sdd_scalars, dim3(1), dim3(64), 0, s, mode, part, sc, mon, a0, a1);
+    return hipGetLastError();
+}
 
 template <typename T>
 hipError_t launch_derived(const DerivedArgs<T>& a, hipStream_t s) {

@@ -48,6 +48,7 @@ extern "C" {
 #define GPF_EULER 0      /* pf.Euler */
 #define GPF_RK4 1        /* pf.RK4 */
 #define GPF_IMPLICIT_EULER 2  /* pf.ImplicitEuler (FP64 only) */
+#define GPF_SDD 3        /* pf.SDD, shrinking-dimer dynamics (FP64 only) */
 
 typedef struct gpf_model gpf_model;
 typedef struct gpf_solver gpf_solver;
@@ -185,6 +186,28 @@ void* gpf_solver_stream(gpf_solver* s);
  * 1000-entry table of a pf.Vandeven (pf/vandeven.go:8-40; n must be 1000), evaluated as
  * Vandeven.Eval at |f| 2/pi; table NULL removes the filter */
 int gpf_solver_set_filter(gpf_solver* s, const double* table, long n);
+
+/* ---------------- pf.SDD (pf/sdd.go) ---------------- */
+
+/* SDD.Init (pf/sdd.go:382-402): orientation = final - init, normalised, and the initial dimer
+ * length = |final - init|; init and final hold every field's real values back to back
+ * (n = nfields * nodes) */
+int gpf_solver_sdd_init(gpf_solver* s, const double* init, const double* final, long n);
+/* SDD.SetInitialOrientation (pf/sdd.go:404-420) */
+int gpf_solver_sdd_set_orientation(gpf_solver* s, const double* orient, long n);
+/* SDD's parameters: Alpha, TimeConstants.Orientation, TimeConstants.DimerLength,
+ * InitDimerLength (a negative value keeps the current one) and MinDimerLength
+ * (pf/sdd.go:84-130; NewSDD's defaults 0.5, 1, 1, -, 0) */
+int gpf_solver_sdd_set_params(gpf_solver* s, double alpha, double tau_orientation, double tau_length,
+                              double init_length, double min_length);
+/* SDDMonitor (pf/sdd.go:25-52): out[0] MaxForce, [1] ForcePowerSpectrum, [2] MaxTorque,
+ * [3] FieldNorm, [4] FieldNormChange, [5] the torque's projection on the orientation
+ * (the reference's sanity check, sdd.go:268-275) */
+int gpf_solver_sdd_monitor(gpf_solver* s, double* out);
+/* the current orientation vector (SaveOrientation's data, pf/sdd.go:422-435) */
+int gpf_solver_sdd_orientation(gpf_solver* s, double* out, long n);
+/* SDD.DimerLength(t) (pf/sdd.go:356-363) */
+int gpf_solver_sdd_dimer_length(gpf_solver* s, double t, double* length);
 
 /* ---------------- field output (pf/fileIO.go, pf/xdmf.go) ---------------- */
 
