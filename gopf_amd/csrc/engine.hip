@@ -1545,90 +1545,50 @@ void Engine<T>::general_step_euler() {
 
 template <typename T>
 void Engine<T>::general_step_rk4() {
+    // RK4.Step (rk4.go:29-127) with each stage's GetRHS, PrepareNextCorrection and
+    // correction fused into one k-space pass per field: the stage's right-hand side is
+    // evaluated from the bricks inside the pass instead of being stored.  The corrected
+    // spectra go to KF_ (free, as the right-hand side is never stored) and swap with K_
+    // after all fields, so every field of a stage reads the same bricks.
     ensure_k_fields();
-    const size_t kb = g_.nk * sizeof(cpx<T>);
-    // initial = field (rk4.go:35-38); final starts as the field too and is formed by the
-    // first stage's update reading the field directly
-    for (int i = 0; i < nf_; ++i)
-        HIPCHK(hipMemcpyAsync(INIT_[i].get(), K_[i].get(), kb, hipMemcpyDeviceToDevice, st_));
     sync_derived();
     eval_elastic();  // the lagged field is constant within a step (homoLinElast.go:130-134)
-    for (int i = 0; i < nf_; ++i) {
-        eval_user_terms_for(i);
-        rhs_update(i, KF_[i].get(), nullptr, 0.0, 1.0, 0.0, false, nullptr);
-    }
     const double fac_prep[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
     const double fac_corr[3] = {0.5, 0.5, 1.0};
     for (int stage = 0; stage < 4; ++stage) {
         for (int i = 0; i < nf_; ++i) {
-            // final += fac dt kFactor; field = initial  (rk4.go:77-84).  The field is not copied
-            // back: the correction below reads the initial field directly
-            KUpdateArgs<T> a;
-            a.dst = FINAL_[i].get();
-            a.cur = stage == 0 ? K_[i].get() : FINAL_[i].get();
-            a.brick[0] = KF_[i].get();
-            a.coef[0] = 1.0;
-            a.pw[0] = 0;
-            a.nterm = 1;
-            a.alpha = 1.0;
-            a.beta = fac_prep[stage] * dt_;
-            a.gamma = 0.0;
+            eval_user_terms_for(i);
+            Rk4StageArgs<T> a;
+            terms_into(i, a);
+            a.k = K_[i].get();
+            a.kn = KF_[i].get();
+            a.init = INIT_[i].get();
+            a.final = FINAL_[i].get();
+            a.stage = stage;
+            a.dt = dt_;
+            a.fp = fac_prep[stage];
+            a.fc = stage < 3 ? fac_corr[stage] : 0.0;
+            a.denom = eqs_[i].denom;
+            a.sv = eqs_[i].sv;
+            a.hs = eqs_[i].hs;
+            a.filter = (stage == 3 && has_filter_) ? filter_.get() : nullptr;
             a.g = g_.kg;
             a.nmain = g_.nmain;
             a.nk = g_.nk;
-            HIPCHK(launch_kupdate<T>(a, st_));
+            HIPCHK(launch_rk4_stage<T>(a, st_));
         }
-        if (stage == 3) break;
-        // correction(factor) (rk4.go:101-127)
-        const double f = fac_corr[stage];
         std::vector<const cpx<T>*> in;
         std::vector<T*> out;
         for (int i = 0; i < nf_; ++i) {
-            rhs_update(i, K_[i].get(), INIT_[i].get(), 1.0, f * dt_, f * dt_, false, KF_[i].get());
+            K_[i].swap(KF_[i]);
             in.push_back(K_[i].get());
             out.push_back(F_[i].get());
         }
         c2r(in, out, T(1) / (T)g_.n);
-        sync_derived();
-        for (int i = 0; i < nf_; ++i) {
-            eval_user_terms_for(i);
-            rhs_update(i, KF_[i].get(), nullptr, 0.0, 1.0, 0.0, false, nullptr);
-        }
+        if (stage < 3) sync_derived();
     }
-    std::vector<const cpx<T>*> in;
-    std::vector<T*> out;
-    for (int i = 0; i < nf_; ++i) {
-        // final / (1 - dt D), filter (rk4.go:57-67)
-        KUpdateArgs<T> a;
-        a.dst = K_[i].get();
-        a.cur = FINAL_[i].get();
-        a.alpha = 1.0;
-        a.beta = 0.0;
-        a.gamma = dt_;
-        a.denom = eqs_[i].denom;
-        a.sv = eqs_[i].sv;
-        a.hs = eqs_[i].hs;
-        a.filter = has_filter_ ? filter_.get() : nullptr;
-        a.g = g_.kg;
-        a.nmain = g_.nmain;
-        a.nk = g_.nk;
-        HIPCHK(launch_kupdate<T>(a, st_));
-        in.push_back(K_[i].get());
-        out.push_back(F_[i].get());
-    }
-    c2r(in, out, T(1) / (T)g_.n);
 }
 
-// ---------------------------------------------------------------------------
-// ImplicitEuler (implicitEuler.go:68-229)
-// ---------------------------------------------------------------------------
-// Step: orig^ = FFT(c_n) and R^prev = GetRHS at c_n; an explicit Euler step is the initial
-// guess; then x = c_{n+1} solves F(x) = 0 with
-//   F(x) = Re IFFT(x^ - (orig^ e^{D dt} + I(D, R^(x), R^prev))) / N,
-// by Newton's method: each update dx solves J dx = -F(x) with restarted GMRES, J v taken
-// as a central difference of F along v (the NewtonKrylov StepSize and Stencil).  The
-// Krylov basis lives in HBM; per Arnoldi step the host reads back only the Gram-Schmidt
-// coefficients and the new norm (FP64, fixed summation order) for the Givens updates.
 template <typename T>
 double Engine<T>::dev_scalar(const double* p) {
     double v = 0.0;

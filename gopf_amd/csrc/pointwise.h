@@ -161,6 +161,37 @@ struct EtdArgs {
     long nmain = 0, nk = 0;
 };
 
+// one RK4 stage of RK4.Step (rk4.go:29-127) per field, with the stage's right-hand side
+// R = sum_t coef_t L^pow_t B_t evaluated in the kernel instead of stored:
+//   stage 0:    init = k; final = k + fp dt R;    kn = (k + fc dt R) / (1 - fc dt D)
+//   stage 1, 2: final += fp dt R;                 kn = (init + fc dt R) / (1 - fc dt D)
+//   stage 3:    kn = filt (final + fp dt R) / (1 - dt D)
+// (fp, fc: the stage's PrepareNextCorrection and correction factors); kn is a separate
+// array so that later fields of the stage still read this field's brick unchanged
+template <typename T>
+struct Rk4StageArgs {
+    const cpx<T>* k = nullptr;
+    cpx<T>* kn = nullptr;
+    cpx<T>* init = nullptr;
+    cpx<T>* final = nullptr;
+    int stage = 0;
+    double dt = 0.0, fp = 0.0, fc = 0.0;
+    int nterm = 0;
+    const cpx<T>* brick[kMaxUTerms] = {};
+    double coef[kMaxUTerms];
+    int pw[kMaxUTerms];
+    const double* dmul[kMaxUTerms] = {};
+    int tmode[kMaxUTerms] = {};
+    int tcomp[kMaxUTerms] = {};
+    KPoly denom;
+    SvPoly sv;
+    HessPoly hs;
+    const double* filter = nullptr;
+    KGrid g;
+    long nmain = 0, nk = 0;
+};
+template <typename T> hipError_t launch_rk4_stage(const Rk4StageArgs<T>& a, hipStream_t s);
+
 // spectral gradient components: out_d = i 2 pi f_d' src   (f_d' = 0 at f_d = +0.5)
 template <typename T>
 struct KGradArgs {

@@ -229,6 +229,72 @@ __global__ void k_kupdate(KUpdateArgs<T> a) {
     }
 }
 
+// RK4.Step's stage algebra (rk4.go:35-127) fused with GetRHS; see Rk4StageArgs
+template <typename T>
+__global__ void k_rk4_stage(Rk4StageArgs<T> a) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
+        int kx, ky, kz;
+        kcoords(a.g, i, a.nmain, kx, ky, kz);
+        double f[3];
+        ref_freq(a.g, kx, ky, kz, f);
+        const double f2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+        const double L = -(4.0 * kPi * kPi) * f2;
+        double rx = 0.0, ry = 0.0;
+        for (int t = 0; t < a.nterm; ++t) {
+            double m = a.coef[t] * ipow(L, a.pw[t]);
+            if (a.dmul[t]) m *= a.dmul[t][0];
+            if (a.tmode[t] == 1) {
+                const double fc = f[a.tcomp[t]];
+                if (fabs(fabs(fc) - 0.5) > 1e-6) {
+                    const double w = m * 2.0 * sin(kPi * fc);
+                    const cpx<T> b = a.brick[t][i];
+                    rx -= w * (double)b.y;
+                    ry += w * (double)b.x;
+                }
+                continue;
+            }
+            if (a.brick[t]) {
+                const cpx<T> b = a.brick[t][i];
+                rx += m * (double)b.x;
+                ry += m * (double)b.y;
+            } else {
+                rx += m;
+            }
+        }
+        // KF is stored in T between the stage kernels of RK4.Step; round it the same way
+        rx = (double)(T)rx;
+        ry = (double)(T)ry;
+        const double D = denominator(a.denom, a.sv, a.hs, a.g.ndim, f, L, f2);
+        const double gam = (a.stage == 3) ? a.dt : a.fc * a.dt;
+        double s = 1.0 / (1.0 - gam * D);
+        if (asym_row(a.hs, a.g.ndim, f)) {
+            double fcj[3];
+            conj_freq(a.g.ndim, f, fcj);
+            const double Dc = denominator(a.denom, a.sv, a.hs, a.g.ndim, fcj, L, f2);
+            s = 0.5 * (s + 1.0 / (1.0 - gam * Dc));
+        }
+        const double bp = a.fp * a.dt;
+        if (a.stage == 0) {
+            const cpx<T> k = a.k[i];
+            a.init[i] = k;
+            a.final[i] = mk<T>((T)((double)k.x + bp * rx), (T)((double)k.y + bp * ry));
+            const double bc = a.fc * a.dt;
+            a.kn[i] = mk<T>((T)(s * ((double)k.x + bc * rx)), (T)(s * ((double)k.y + bc * ry)));
+        } else if (a.stage < 3) {
+            const cpx<T> fin = a.final[i], ini = a.init[i];
+            a.final[i] = mk<T>((T)((double)fin.x + bp * rx), (T)((double)fin.y + bp * ry));
+            const double bc = a.fc * a.dt;
+            a.kn[i] = mk<T>((T)(s * ((double)ini.x + bc * rx)), (T)(s * ((double)ini.y + bc * ry)));
+        } else {
+            if (a.filter) s *= vandeven(a.filter, sqrt(f2) * 2.0 / kPi);
+            const cpx<T> fin = a.final[i];
+            // final is stored in T before the division (rk4.go:57-67)
+            const T fx = (T)((double)fin.x + bp * rx), fy = (T)((double)fin.y + bp * ry);
+            a.kn[i] = mk<T>((T)(s * (double)fx), (T)(s * (double)fy));
+        }
+    }
+}
+
 // e^{D dt} and nonlinearIntegral (implicitEuler.go:151-162) for real D
 __device__ __forceinline__ void etd_update(double D, double dt, double rx, double ry, double px, double py, double& e,
                                            double& ix, double& iy) {
@@ -867,6 +933,11 @@ hipError_t launch_kupdate(const KUpdateArgs<T>& a, hipStream_t s) {
     return hipGetLastError();
 }
 template <typename T>
+hipError_t launch_rk4_stage(const Rk4StageArgs<T>& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_rk4_stage<T>, dim3(grid_for(a.nk)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+template <typename T>
 hipError_t launch_etd_residual(const EtdArgs<T>& a, hipStream_t s) {
     hipLaunchKernelGGL(k_etd_residual<T>, dim3(grid_for(a.nk)), dim3(256), 0, s, a);
     return hipGetLastError();
@@ -990,6 +1061,7 @@ hipError_t launch_reduce2(const double* a, long na, double ca, const double* b, 
     template hipError_t launch_derived<T>(const DerivedArgs<T>&, hipStream_t);                                \
     template hipError_t launch_kupdate<T>(const KUpdateArgs<T>&, hipStream_t);                                \
     template hipError_t launch_etd_residual<T>(const EtdArgs<T>&, hipStream_t);                               \
+    template hipError_t launch_rk4_stage<T>(const Rk4StageArgs<T>&, hipStream_t);                             \
     template hipError_t launch_volcons_update<T>(const double*, long, const T*, double, double*, hipStream_t); \
     template hipError_t launch_kgrad<T>(const KGradArgs<T>&, int, hipStream_t);                               \
     template hipError_t launch_kelastic<T>(const KElasticArgs<T>&, hipStream_t);                              \
