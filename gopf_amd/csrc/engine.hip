@@ -402,7 +402,10 @@ private:
     // transforms
     const cpx<T>* tw(int n);
     void pencils(const std::vector<cpx<T>*>& dst, const std::vector<const cpx<T>*>& src, int axis, int dir);
-    void r2c(const std::vector<const T*>& in, const std::vector<cpx<T>*>& out, T in_scale);
+    // prog (optional): per input, a dense derived-field program evaluated while loading
+    // instead of reading the input array
+    void r2c(const std::vector<const T*>& in, const std::vector<cpx<T>*>& out, T in_scale,
+             const std::vector<const DerivedArgs<T>*>* prog = nullptr);
     void c2r(const std::vector<const cpx<T>*>& in, const std::vector<T*>& out, T scale,
              const double* add_mean = nullptr);
     cpx<T>* kscratch(int i);
@@ -821,7 +824,8 @@ void Engine<T>::pencils(const std::vector<cpx<T>*>& dst, const std::vector<const
 }
 
 template <typename T>
-void Engine<T>::r2c(const std::vector<const T*>& in, const std::vector<cpx<T>*>& out, T in_scale) {
+void Engine<T>::r2c(const std::vector<const T*>& in, const std::vector<cpx<T>*>& out, T in_scale,
+                    const std::vector<const DerivedArgs<T>*>* prog) {
     for (size_t q0 = 0; q0 < in.size(); q0 += kMaxArr) {
         RowR2CArgs<T> a;
         a.tw = tw(g_.nx);
@@ -830,6 +834,14 @@ void Engine<T>::r2c(const std::vector<const T*>& in, const std::vector<cpx<T>*>&
             a.in[q] = in[q0 + q];
             a.main[q] = out[q0 + q];
             a.nyq[q] = out[q0 + q] + g_.nmain;
+            const DerivedArgs<T>* d = prog ? (*prog)[q0 + q] : nullptr;
+            if (d) {
+                a.poly[q] = 1;
+                a.pf0[q] = d->f[0];
+                a.pf1[q] = (d->nf > 1) ? d->f[1] : nullptr;
+                for (int i = 0; i < 4; ++i)
+                    for (int j = 0; j < 4; ++j) a.dc[q][i][j] = d->dc[i][j];
+            }
         }
         a.rows = g_.rows;
         a.nx = g_.nx;
@@ -1386,13 +1398,18 @@ void Engine<T>::sync_derived() {
     if (nd_ == 0) return;
     std::vector<const T*> in;
     std::vector<cpx<T>*> out;
+    // dense programs (polynomials of at most two fields) run inside the row transform;
+    // others (non-integer powers, noise) are evaluated into Dr_ first
+    std::vector<const DerivedArgs<T>*> prog;
     for (int d = 0; d < nd_; ++d) {
         dargs_[d].counter = dcount_[d]++;
-        HIPCHK(launch_derived<T>(dargs_[d], st_));
-        in.push_back(Dr_[d].get());
+        const bool inline_prog = dargs_[d].dense && !dargs_[d].noise;
+        if (!inline_prog) HIPCHK(launch_derived<T>(dargs_[d], st_));
+        in.push_back(inline_prog ? nullptr : Dr_[d].get());
+        prog.push_back(inline_prog ? &dargs_[d] : nullptr);
         out.push_back(K_[nf_ + d].get());
     }
-    r2c(in, out, T(1));
+    r2c(in, out, T(1), &prog);
 }
 
 // SquaredGradient terms of equation eq, evaluated on the current k-space bricks

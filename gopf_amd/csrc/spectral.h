@@ -117,6 +117,13 @@ struct RowR2CArgs {
     int nx = 0;
     int TW = 16;
     T in_scale = T(1);
+    // optional per array: the real row is a dense polynomial of one or two fields evaluated
+    // while loading (a derived field, sum_ij dc[i][j] f0^i f1^j in T, Horner in f0 of
+    // polynomials in f1), so the derived field never goes through HBM
+    int poly[kMaxArr] = {};
+    const T* pf0[kMaxArr] = {};
+    const T* pf1[kMaxArr] = {};
+    double dc[kMaxArr][4][4] = {};
 };
 
 // x-row C2R of half spectra into real rows (scaled)

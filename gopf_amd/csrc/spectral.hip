@@ -711,6 +711,20 @@ __device__ __forceinline__ void load_herm_pair_lds(cpx<T>* v, const cpx<T>* m, c
     __syncthreads();
 }
 
+// the dense derived-field polynomial of array q at element off (the k_derived formula)
+template <typename T>
+__device__ __forceinline__ T row_poly(const RowR2CArgs<T>& a, int q, long off) {
+    const T x0 = a.pf0[q][off];
+    const T x1 = a.pf1[q] ? a.pf1[q][off] : T(0);
+    T acc = T(0);
+#pragma unroll
+    for (int p = 3; p >= 0; --p) {
+        const T inner = (((T)a.dc[q][p][3] * x1 + (T)a.dc[q][p][2]) * x1 + (T)a.dc[q][p][1]) * x1 + (T)a.dc[q][p][0];
+        acc = acc * x0 + inner;
+    }
+    return acc;
+}
+
 template <typename T, int N>
 __global__ void __launch_bounds__(kMaxBlock) k_rows_r2c(RowR2CArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -733,14 +747,17 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_r2c(RowR2CArgs<T> a) {
             const long off = row * n + lm.t + TT * e;
             T x = T(0), y = T(0);
             if (valid) {
-                if (ia) x = ia[off] * a.in_scale;
-                if (ib) y = ib[off] * a.in_scale;
+                if (a.poly[q]) x = row_poly(a, q, off) * a.in_scale;
+                else if (ia) x = ia[off] * a.in_scale;
+                if (q + 1 < a.narr && a.poly[q + 1]) y = row_poly(a, q + 1, off) * a.in_scale;
+                else if (ib) y = ib[off] * a.in_scale;
             }
             v[e] = mk<T>(x, y);
         }
         line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
+        const bool two = q + 1 < a.narr;
         split_store<T, E>(v, lds + lm.off, n, lm.t, TT, row, valid, a.main[q], a.nyq[q],
-                          ib ? a.main[q + 1] : nullptr, ib ? a.nyq[q + 1] : nullptr);
+                          two ? a.main[q + 1] : nullptr, two ? a.nyq[q + 1] : nullptr);
     }
 }
 
