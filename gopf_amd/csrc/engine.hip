@@ -163,10 +163,25 @@ struct Geometry {
         nmain = rows * nxh;
         nnyq = rows;
         nk = nmain + nnyq;
+        if (nk >= (1l << 32)) throw Error(GPF_EINVAL, "grid too large (the half spectrum must hold fewer than 2^32 values)");
         kg.nx = nx;
         kg.ny = ny;
         kg.nz = nz;
         kg.ndim = nd;
+        kg.dxh = FastDiv::make((unsigned)nxh);
+        kg.dy = FastDiv::make((unsigned)ny);
+    }
+    // KGrid::ftab on the host: float64(k) / float64(n), less 1 above 0.5 (fftWrap.go:57-74)
+    std::vector<double> freq_table() const {
+        auto wrap = [](int k, int m) {
+            const double f = (double)k / (double)m;
+            return (f > 0.5) ? f - 1.0 : f;
+        };
+        std::vector<double> t;
+        for (int k = 0; k < ny; ++k) t.push_back(wrap(k, ny));
+        for (int k = 0; k <= nxh; ++k) t.push_back(wrap(k, nx));
+        for (int k = 0; k < nz; ++k) t.push_back(ndim == 3 ? wrap(k, nz) : 0.0);
+        return t;
     }
 };
 
@@ -445,6 +460,7 @@ private:
     void sync_derived();
     void eval_user_terms_for(int eq);
     void eval_elastic();
+    KElasticArgs<T> elastic_args(size_t e) const;
     void rhs_update(int eq, cpx<T>* dst, const cpx<T>* cur, double alpha, double beta, double gamma, bool filt,
                     const cpx<T>* extra);
     template <class A>
@@ -492,11 +508,13 @@ private:
     FusedPlan fp_;
     std::vector<FieldEq> eqs_;
     Buf<double> filter_;
+    Buf<double> ftab_;  // KGrid::ftab
     bool has_filter_ = false;
     std::vector<std::pair<int, Buf<cpx<T>>>> tw_;
     // general-path storage
     std::vector<Buf<T>> F_, Dr_;
     std::vector<Buf<cpx<T>>> K_, Tsq_, Tel_, INIT_, FINAL_, KF_;
+    std::vector<Buf<double>> Ktab_;  // K(k) of each elastic term, nk values
     std::vector<Buf<T>> lag_;
     // VolumeConservingLP: indicator brick, field, device state {Multiplier, CurrentIntegral,
     // IsFirstUpdate} per occurrence, and the partial sums of its field integral
@@ -536,6 +554,12 @@ Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gp
     : dt_(dt), stepper_(o.stepper), mcopy_(m), autotune_(o.autotune != 0) {
     g_.init(dims, ndim);
     HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    {
+        const std::vector<double> ft = g_.freq_table();
+        ftab_.alloc(ft.size());
+        HIPCHK(hipMemcpy(ftab_.get(), ft.data(), ft.size() * sizeof(double), hipMemcpyHostToDevice));
+        g_.kg.ftab = ftab_.get();
+    }
     nf_ = (int)m.field_names.size();
     nd_ = (int)m.derived.size();
     if (nf_ == 0) throw Error(GPF_EINVAL, "Model: No fields added");
@@ -614,6 +638,8 @@ Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gp
         for (size_t s = 0; s < m.user_table[kSquaredGradient].size(); ++s) Tsq_.emplace_back(g_.nk);
         for (size_t e = 0; e < m.user_table[kHomogeneousModulus].size(); ++e) {
             Tel_.emplace_back(g_.nk);
+            Ktab_.emplace_back(g_.nk);
+            HIPCHK(launch_kelastic_table<T>(elastic_args(e), Ktab_.back().get(), st_));
             lag_.emplace_back(g_.n);
             HIPCHK(hipMemsetAsync(lag_.back().get(), 0, g_.n * sizeof(T), st_));
             bool found = false;
@@ -1441,6 +1467,26 @@ void Engine<T>::eval_user_terms_for(int eq) {
     }
 }
 
+// the scalar kernel K(k) of elastic term e: material C and sigma* = C : misfit
+template <typename T>
+KElasticArgs<T> Engine<T>::elastic_args(size_t e) const {
+    const UserTerm& ut = mcopy_.user_table[kHomogeneousModulus][e];
+    KElasticArgs<T> a;
+    std::memcpy(a.ep.C, ut.C, sizeof(a.ep.C));
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k)
+                for (int l = 0; l < 3; ++l) s = s + ut.C[i * 27 + j * 9 + k * 3 + l] * ut.misfit[k * 3 + l];
+            a.ep.sig[i * 3 + j] = s;
+        }
+    a.ep.dim = g_.ndim;
+    a.g = g_.kg;
+    a.nmain = g_.nmain;
+    a.nk = g_.nk;
+    return a;
+}
+
 // HomogeneousModulusLinElast: FFT(H'(phi) (IFFT(K H^) / N - 2 e* H(phi))), once per step
 template <typename T>
 void Engine<T>::eval_elastic() {
@@ -1450,21 +1496,10 @@ void Engine<T>::eval_elastic() {
         HIPCHK(launch_indicator<T>(lag_[e].get(), H, g_.n, st_));
         cpx<T>* Hk = kscratch(kMaxArr + 1);
         r2c({H}, {Hk}, T(1));
-        KElasticArgs<T> a;
+        KElasticArgs<T> a = elastic_args(e);
         a.src = Hk;
         a.dst = Hk;
-        std::memcpy(a.ep.C, ut.C, sizeof(a.ep.C));
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) {
-                double s = 0.0;
-                for (int k = 0; k < 3; ++k)
-                    for (int l = 0; l < 3; ++l) s = s + ut.C[i * 27 + j * 9 + k * 3 + l] * ut.misfit[k * 3 + l];
-                a.ep.sig[i * 3 + j] = s;
-            }
-        a.ep.dim = g_.ndim;
-        a.g = g_.kg;
-        a.nmain = g_.nmain;
-        a.nk = g_.nk;
+        a.ktab = Ktab_[e].get();
         HIPCHK(launch_kelastic<T>(a, st_));
         T* S = rscratch(2);
         c2r({Hk}, {S}, T(1) / (T)g_.n);

@@ -19,20 +19,25 @@ namespace gpf {
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
+constexpr int kStageThreads = 256;  // threads per block of the k-space stage kernel
+constexpr int kStageEpt = 4;        // its elements per thread
 
+// (kx, ky, kz) of half-spectrum index idx (main part first, then the x-Nyquist plane);
+// idx < nk < 2^32, so the divisions are 32-bit multiply-high ones
 __device__ __forceinline__ void kcoords(const KGrid& g, long idx, long nmain, int& kx, int& ky, int& kz) {
-    const int nxh = g.nx / 2;
+    const unsigned nxh = (unsigned)(g.nx / 2);
+    unsigned r;
     if (idx < nmain) {
-        kx = (int)(idx % nxh);
-        const long r = idx / nxh;
-        ky = (int)(r % g.ny);
-        kz = (int)(r / g.ny);
+        const unsigned u = (unsigned)idx;
+        r = g.dxh.div(u);
+        kx = (int)(u - r * nxh);
     } else {
-        const long j = idx - nmain;
-        kx = nxh;
-        ky = (int)(j % g.ny);
-        kz = (int)(j / g.ny);
+        r = (unsigned)(idx - nmain);
+        kx = (int)nxh;
     }
+    const unsigned z = g.dy.div(r);
+    ky = (int)(r - z * (unsigned)g.ny);
+    kz = (int)z;
 }
 
 __device__ __forceinline__ double wrapf(int k, int n) {
@@ -40,8 +45,15 @@ __device__ __forceinline__ double wrapf(int k, int n) {
     return (f > 0.5) ? f - 1.0 : f;
 }
 
-// reference Freq components (fftWrap.go:57-74): f[0] row (y), f[1] col (x), f[2] depth (z)
+// reference Freq components (fftWrap.go:57-74): f[0] row (y), f[1] col (x), f[2] depth (z),
+// read from the grid's table (the same values wrapf computes)
 __device__ __forceinline__ void ref_freq(const KGrid& g, int kx, int ky, int kz, double f[3]) {
+    if (g.ftab) {
+        f[0] = g.ftab[ky];
+        f[1] = g.ftab[g.ny + kx];
+        f[2] = g.ftab[g.ny + g.nx / 2 + 1 + kz];
+        return;
+    }
     f[0] = wrapf(ky, g.ny);
     f[1] = wrapf(kx, g.nx);
     f[2] = (g.ndim == 3) ? wrapf(kz, g.nz) : 0.0;
@@ -94,12 +106,19 @@ __device__ __forceinline__ double denominator(const KPoly& p, const SvPoly& sv, 
         for (int t = 0; t < sv.n; ++t)
             D += sv.coef[t] * ipow(L, sv.lap[t]) * sv_interp(fr, sv.thr[t]) * go_ipow(fr, sv.pw[t]);
     }
+    // the loops over axes have fixed trip counts (ndim enters as a guard), so that f stays in
+    // registers
     for (int t = 0; t < hs.n; ++t) {
         const int d = hs.kd[t];
         double h = 0.0;
-        for (int j = 0; j < ndim; ++j) h += -4.0 * kPi * kPi * f[j] * f[j] * hs.K[t][j * d + j];
-        for (int j = 0; j < ndim; ++j)
-            for (int k = j + 1; k < ndim; ++k) h += -8.0 * kPi * kPi * f[j] * f[k] * hs.K[t][j * d + k];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            if (j < ndim) h += -4.0 * kPi * kPi * f[j] * f[j] * hs.K[t][j * d + j];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = j + 1; k < 3; ++k)
+                if (k < ndim) h += -8.0 * kPi * kPi * f[j] * f[k] * hs.K[t][j * d + k];
         D += hs.coef[t] * ipow(L, hs.lap[t]) * h;
     }
     return D;
@@ -114,7 +133,9 @@ __device__ __forceinline__ double denominator(const KPoly& p, const SvPoly& sv, 
 __device__ __forceinline__ bool asym_row(const HessPoly& hs, int ndim, const double f[3]) {
     if (hs.n == 0) return false;
     bool any = false, all = true;
-    for (int j = 0; j < ndim; ++j) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        if (j >= ndim) continue;
         const bool nyq = (f[j] == 0.5);
         any |= nyq;
         all &= nyq;
@@ -123,6 +144,7 @@ __device__ __forceinline__ bool asym_row(const HessPoly& hs, int ndim, const dou
 }
 
 __device__ __forceinline__ void conj_freq(int ndim, const double f[3], double fc[3]) {
+#pragma unroll
     for (int j = 0; j < 3; ++j) fc[j] = (j < ndim && f[j] != 0.5) ? -f[j] : f[j];
 }
 
@@ -229,67 +251,90 @@ __global__ void k_kupdate(KUpdateArgs<T> a) {
     }
 }
 
-// RK4.Step's stage algebra (rk4.go:35-127) fused with GetRHS; see Rk4StageArgs
-template <typename T>
-__global__ void k_rk4_stage(Rk4StageArgs<T> a) {
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
-        int kx, ky, kz;
-        kcoords(a.g, i, a.nmain, kx, ky, kz);
-        double f[3];
-        ref_freq(a.g, kx, ky, kz, f);
-        const double f2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
-        const double L = -(4.0 * kPi * kPi) * f2;
-        double rx = 0.0, ry = 0.0;
-        for (int t = 0; t < a.nterm; ++t) {
-            double m = a.coef[t] * ipow(L, a.pw[t]);
-            if (a.dmul[t]) m *= a.dmul[t][0];
-            if (a.tmode[t] == 1) {
-                const double fc = f[a.tcomp[t]];
+// RK4.Step's stage algebra (rk4.go:35-127) fused with GetRHS; see Rk4StageArgs.  A thread
+// takes EPT elements kStageThreads apart: the uniform data of each term is read once for
+// EPT elements, and the loads of all EPT elements are in flight together.
+template <typename T, int EPT>
+__global__ void __launch_bounds__(kStageThreads) k_rk4_stage(Rk4StageArgs<T> a) {
+    const long i0 = (long)blockIdx.x * (kStageThreads * EPT) + threadIdx.x;
+    double f[EPT][3], f2[EPT], L[EPT], rx[EPT], ry[EPT];
+    bool ok[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const long i = i0 + kStageThreads * e;
+        ok[e] = i < a.nk;
+        int kx = 0, ky = 0, kz = 0;
+        if (ok[e]) kcoords(a.g, i, a.nmain, kx, ky, kz);
+        ref_freq(a.g, kx, ky, kz, f[e]);
+        f2[e] = f[e][0] * f[e][0] + f[e][1] * f[e][1] + f[e][2] * f[e][2];
+        L[e] = -(4.0 * kPi * kPi) * f2[e];
+        rx[e] = 0.0;
+        ry[e] = 0.0;
+    }
+    for (int t = 0; t < a.nterm; ++t) {
+        const double c = a.coef[t];
+        const int pw = a.pw[t];
+        const double* dmp = a.dmul[t];
+        const double dm = dmp ? dmp[0] : 1.0;
+        const cpx<T>* br = a.brick[t];
+        const int mode = a.tmode[t], comp = a.tcomp[t];
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            if (!ok[e]) continue;
+            const long i = i0 + kStageThreads * e;
+            double m = c * ipow(L[e], pw);
+            if (dmp) m *= dm;
+            if (mode == 1) {
+                const double fc = (comp == 0) ? f[e][0] : (comp == 1) ? f[e][1] : f[e][2];
                 if (fabs(fabs(fc) - 0.5) > 1e-6) {
                     const double w = m * 2.0 * sin(kPi * fc);
-                    const cpx<T> b = a.brick[t][i];
-                    rx -= w * (double)b.y;
-                    ry += w * (double)b.x;
+                    const cpx<T> b = br[i];
+                    rx[e] -= w * (double)b.y;
+                    ry[e] += w * (double)b.x;
                 }
                 continue;
             }
-            if (a.brick[t]) {
-                const cpx<T> b = a.brick[t][i];
-                rx += m * (double)b.x;
-                ry += m * (double)b.y;
+            if (br) {
+                const cpx<T> b = br[i];
+                rx[e] += m * (double)b.x;
+                ry[e] += m * (double)b.y;
             } else {
-                rx += m;
+                rx[e] += m;
             }
         }
+    }
+    const double gam = (a.stage == 3) ? a.dt : a.fc * a.dt;
+    const double bp = a.fp * a.dt;
+    const double bc = a.fc * a.dt;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        if (!ok[e]) continue;
+        const long i = i0 + kStageThreads * e;
         // KF is stored in T between the stage kernels of RK4.Step; round it the same way
-        rx = (double)(T)rx;
-        ry = (double)(T)ry;
-        const double D = denominator(a.denom, a.sv, a.hs, a.g.ndim, f, L, f2);
-        const double gam = (a.stage == 3) ? a.dt : a.fc * a.dt;
+        const double qx = (double)(T)rx[e];
+        const double qy = (double)(T)ry[e];
+        const double D = denominator(a.denom, a.sv, a.hs, a.g.ndim, f[e], L[e], f2[e]);
         double s = 1.0 / (1.0 - gam * D);
-        if (asym_row(a.hs, a.g.ndim, f)) {
+        if (asym_row(a.hs, a.g.ndim, f[e])) {
             double fcj[3];
-            conj_freq(a.g.ndim, f, fcj);
-            const double Dc = denominator(a.denom, a.sv, a.hs, a.g.ndim, fcj, L, f2);
+            conj_freq(a.g.ndim, f[e], fcj);
+            const double Dc = denominator(a.denom, a.sv, a.hs, a.g.ndim, fcj, L[e], f2[e]);
             s = 0.5 * (s + 1.0 / (1.0 - gam * Dc));
         }
-        const double bp = a.fp * a.dt;
         if (a.stage == 0) {
             const cpx<T> k = a.k[i];
             a.init[i] = k;
-            a.final[i] = mk<T>((T)((double)k.x + bp * rx), (T)((double)k.y + bp * ry));
-            const double bc = a.fc * a.dt;
-            a.kn[i] = mk<T>((T)(s * ((double)k.x + bc * rx)), (T)(s * ((double)k.y + bc * ry)));
+            a.final[i] = mk<T>((T)((double)k.x + bp * qx), (T)((double)k.y + bp * qy));
+            a.kn[i] = mk<T>((T)(s * ((double)k.x + bc * qx)), (T)(s * ((double)k.y + bc * qy)));
         } else if (a.stage < 3) {
             const cpx<T> fin = a.final[i], ini = a.init[i];
-            a.final[i] = mk<T>((T)((double)fin.x + bp * rx), (T)((double)fin.y + bp * ry));
-            const double bc = a.fc * a.dt;
-            a.kn[i] = mk<T>((T)(s * ((double)ini.x + bc * rx)), (T)(s * ((double)ini.y + bc * ry)));
+            a.final[i] = mk<T>((T)((double)fin.x + bp * qx), (T)((double)fin.y + bp * qy));
+            a.kn[i] = mk<T>((T)(s * ((double)ini.x + bc * qx)), (T)(s * ((double)ini.y + bc * qy)));
         } else {
-            if (a.filter) s *= vandeven(a.filter, sqrt(f2) * 2.0 / kPi);
+            if (a.filter) s *= vandeven(a.filter, sqrt(f2[e]) * 2.0 / kPi);
             const cpx<T> fin = a.final[i];
             // final is stored in T before the division (rk4.go:57-67)
-            const T fx = (T)((double)fin.x + bp * rx), fy = (T)((double)fin.y + bp * ry);
+            const T fx = (T)((double)fin.x + bp * qx), fy = (T)((double)fin.y + bp * qy);
             a.kn[i] = mk<T>((T)(s * (double)fx), (T)(s * (double)fy));
         }
     }
@@ -397,32 +442,50 @@ __global__ void k_kgrad(KGradArgs<T> a, int ndim) {
     }
 }
 
-// 3x3 LU with partial pivoting; returns false if singular
-__device__ bool solve3(double A[3][3], double b[3], double x[3]) {
+// 3x3 LU with partial pivoting; returns false if singular.  Every array index is a
+// compile-time constant (the pivot row is swapped in by selects), so A stays in registers.
+__device__ __forceinline__ bool solve3(double A[3][3], double b[3], double x[3]) {
     int piv[3] = {0, 1, 2};
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
         int p = k;
         double best = fabs(A[k][k]);
+#pragma unroll
         for (int i = k + 1; i < 3; ++i)
             if (fabs(A[i][k]) > best) { best = fabs(A[i][k]); p = i; }
         if (best == 0.0) return false;
-        if (p != k) {
-            for (int j = 0; j < 3; ++j) { double t = A[k][j]; A[k][j] = A[p][j]; A[p][j] = t; }
-            int t = piv[k]; piv[k] = piv[p]; piv[p] = t;
+#pragma unroll
+        for (int i = k + 1; i < 3; ++i) {
+            const bool sw = (i == p);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double t = A[k][j];
+                A[k][j] = sw ? A[i][j] : t;
+                A[i][j] = sw ? t : A[i][j];
+            }
+            const int t = piv[k];
+            piv[k] = sw ? piv[i] : t;
+            piv[i] = sw ? t : piv[i];
         }
+#pragma unroll
         for (int i = k + 1; i < 3; ++i) {
             A[i][k] /= A[k][k];
+#pragma unroll
             for (int j = k + 1; j < 3; ++j) A[i][j] -= A[i][k] * A[k][j];
         }
     }
     double y[3];
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
-        double s = b[piv[i]];
+        double s = (piv[i] == 0) ? b[0] : (piv[i] == 1) ? b[1] : b[2];
+#pragma unroll
         for (int j = 0; j < i; ++j) s -= A[i][j] * y[j];
         y[i] = s;
     }
+#pragma unroll
     for (int i = 2; i >= 0; --i) {
         double s = y[i];
+#pragma unroll
         for (int j = i + 1; j < 3; ++j) s -= A[i][j] * x[j];
         x[i] = s / A[i][i];
     }
@@ -431,41 +494,63 @@ __device__ bool solve3(double A[3][3], double b[3], double x[3]) {
 
 __device__ double elastic_kernel(const ElasticParams& ep, const double f[3]) {
     if (fabs(f[0]) < 1e-10 && fabs(f[1]) < 1e-10 && fabs(f[2]) < 1e-10) return 0.0;
+    // every loop has a fixed trip count (dim enters as a guard) so that the local arrays
+    // stay in registers
     double G[3][3];
     const double tp2 = 4.0 * kPi * kPi;
+#pragma unroll
     for (int m = 0; m < 3; ++m)
+#pragma unroll
         for (int n = 0; n < 3; ++n) {
             double e = 0.0;
+#pragma unroll
             for (int j = 0; j < 3; ++j)
+#pragma unroll
                 for (int l = 0; l < 3; ++l) e += ep.C[m * 27 + j * 9 + n * 3 + l] * f[j] * f[l];
             G[m][n] = e * tp2;
         }
     // force direction b_c = sum_j sigma*_cj f_j for c < Dim (u = -i 2 pi G^-1 b H)
     double b[3] = {0.0, 0.0, 0.0};
-    for (int c = 0; c < ep.dim; ++c)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (c >= ep.dim) continue;
+#pragma unroll
         for (int j = 0; j < 3; ++j) b[c] += ep.sig[c * 3 + j] * f[j];
+    }
     double v[3];
     if (!solve3(G, b, v)) return 0.0;
     double fm[3];
+#pragma unroll
     for (int c = 0; c < 3; ++c) fm[c] = (fabs(fabs(f[c]) - 0.5) < 1e-10) ? 0.0 : f[c];
     // eps_ab = 2 pi^2 (f'_b v_a + f'_a v_b) H ; K = sum_{a<=b<Dim} fac sigma*_ab eps_ab / H
     double K = 0.0;
-    for (int a = 0; a < ep.dim; ++a)
-        for (int bb = a; bb < ep.dim; ++bb) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int bb = a; bb < 3; ++bb) {
+            if (bb >= ep.dim) continue;
             const double fac = (a != bb) ? 2.0 : 1.0;
             K += fac * ep.sig[a * 3 + bb] * 2.0 * kPi * kPi * (fm[bb] * v[a] + fm[a] * v[bb]);
         }
     return K;
 }
 
+// K(k) depends only on the frequencies and the material, so it is tabulated once
 template <typename T>
-__global__ void k_kelastic(KElasticArgs<T> a) {
+__global__ void k_kelastic_table(KElasticArgs<T> a, double* __restrict__ ktab) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
         int kx, ky, kz;
         kcoords(a.g, i, a.nmain, kx, ky, kz);
         double f[3];
         ref_freq(a.g, kx, ky, kz, f);
-        const double K = elastic_kernel(a.ep, f);
+        ktab[i] = elastic_kernel(a.ep, f);
+    }
+}
+
+template <typename T>
+__global__ void k_kelastic(KElasticArgs<T> a) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
+        const double K = a.ktab[i];
         const cpx<T> s = a.src[i];
         a.dst[i] = mk<T>((T)(K * (double)s.x), (T)(K * (double)s.y));
     }
@@ -934,7 +1019,9 @@ hipError_t launch_kupdate(const KUpdateArgs<T>& a, hipStream_t s) {
 }
 template <typename T>
 hipError_t launch_rk4_stage(const Rk4StageArgs<T>& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_rk4_stage<T>, dim3(grid_for(a.nk)), dim3(256), 0, s, a);
+    constexpr int EPT = kStageEpt;
+    const long blocks = (a.nk + kStageThreads * EPT - 1) / (kStageThreads * EPT);
+    hipLaunchKernelGGL((k_rk4_stage<T, EPT>), dim3((unsigned)blocks), dim3(kStageThreads), 0, s, a);
     return hipGetLastError();
 }
 template <typename T>
@@ -953,7 +1040,13 @@ hipError_t launch_kgrad(const KGradArgs<T>& a, int ndim, hipStream_t s) {
     return hipGetLastError();
 }
 template <typename T>
+hipError_t launch_kelastic_table(const KElasticArgs<T>& a, double* ktab, hipStream_t s) {
+    hipLaunchKernelGGL(k_kelastic_table<T>, dim3(grid_for(a.nk)), dim3(256), 0, s, a, ktab);
+    return hipGetLastError();
+}
+template <typename T>
 hipError_t launch_kelastic(const KElasticArgs<T>& a, hipStream_t s) {
+    if (!a.ktab) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_kelastic<T>, dim3(grid_for(a.nk)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
@@ -1065,6 +1158,7 @@ hipError_t launch_reduce2(const double* a, long na, double ca, const double* b, 
     template hipError_t launch_volcons_update<T>(const double*, long, const T*, double, double*, hipStream_t); \
     template hipError_t launch_kgrad<T>(const KGradArgs<T>&, int, hipStream_t);                               \
     template hipError_t launch_kelastic<T>(const KElasticArgs<T>&, hipStream_t);                              \
+    template hipError_t launch_kelastic_table<T>(const KElasticArgs<T>&, double*, hipStream_t);               \
     template hipError_t launch_sqsum<T>(const T* const*, int, T, T*, long, hipStream_t);                      \
     template hipError_t launch_indicator<T>(const T*, T*, long, hipStream_t);                                 \
     template hipError_t launch_elastic_real<T>(const T*, const T*, double, T*, long, hipStream_t);            \

@@ -41,10 +41,38 @@ struct KPoly {
     int pow[kMaxKTerms];
 };
 
+// Division of 32-bit unsigned numerators by a fixed divisor d >= 1 with one multiply-high
+// (Granlund & Montgomery, "Division by invariant integers using multiplication", 1994,
+// Thm 4.2 with N = 32): l = ceil(log2 d) and m = ceil(2^(32+l) / d) = 2^32 + mp, so that
+// q = floor(m n / 2^(32+l)) = (n + umulhi(mp, n)) >> l for every n < 2^32.
+struct FastDiv {
+    unsigned mp = 0;
+    int l = 0;
+    unsigned d = 1;
+    static FastDiv make(unsigned d) {
+        FastDiv f;
+        f.d = d;
+        while ((1ull << f.l) < d) ++f.l;
+        const unsigned long long m = ((1ull << (32 + f.l)) + d - 1) / d;  // 2^32 <= m < 2^33
+        f.mp = (unsigned)(m - (1ull << 32));
+        return f;
+    }
+    __host__ __device__ unsigned div(unsigned n) const {
+        const unsigned hi = (unsigned)(((unsigned long long)mp * n) >> 32);  // v_mul_hi_u32
+        return (unsigned)(((unsigned long long)n + hi) >> l);
+    }
+};
+
 // geometry of the transform grid
 struct KGrid {
     int nx = 1, ny = 1, nz = 1;
     int ndim = 2;
+    // the k-space kernels index the half spectrum with 32-bit numerators (the grid keeps
+    // nk < 2^32) divided by nx / 2 and ny
+    FastDiv dxh, dy;
+    // Freq components (fftWrap.go:57-74) per axis: [ny] row (y) | [nx/2 + 1] column (x) |
+    // [nz] depth (z, zeros in 2D); each entry is float64(k) / float64(n), less 1 above 0.5
+    const double* ftab = nullptr;
 };
 
 template <typename T>
