@@ -456,6 +456,7 @@ private:
     // general path
     void general_step_euler();
     void general_step_rk4();
+    void sync_from_spectra();
     void ensure_k_fields();
     void sync_derived();
     void eval_user_terms_for(int eq);
@@ -509,6 +510,7 @@ private:
     std::vector<FieldEq> eqs_;
     Buf<double> filter_;
     Buf<double> ftab_;  // KGrid::ftab
+    bool row_sync_ = false;  // sync_from_spectra applies (dense derived fields of <= 2 fields)
     bool has_filter_ = false;
     std::vector<std::pair<int, Buf<cpx<T>>>> tw_;
     // general-path storage
@@ -730,6 +732,8 @@ Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gp
             dargs_.push_back(a);
             dcount_.push_back(0);
         }
+        row_sync_ = nf_ <= 2 && nd_ >= 1 && nd_ <= kMaxArr;
+        for (int d = 0; d < nd_; ++d) row_sync_ = row_sync_ && dargs_[d].dense && !dargs_[d].noise;
         pass_names_ = {"general_step"};
     }
     pass_ms_.assign(pass_names_.size(), 0.0);
@@ -1595,6 +1599,48 @@ void Engine<T>::general_step_euler() {
     c2r(in, out, T(1) / (T)g_.n);
 }
 
+// Stages 0-2 of RK4: the corrected fields' inverse transform and the derived fields'
+// forward transform meet in one row pass (k_rows_sync), so the real fields are not written
+// (the last stage writes them).  Used when every derived field is a dense polynomial of
+// the (at most two) fields.
+template <typename T>
+void Engine<T>::sync_from_spectra() {
+    std::vector<cpx<T>*> scr;
+    std::vector<const cpx<T>*> src;
+    for (int i = 0; i < nf_; ++i) {
+        scr.push_back(kscratch(i));
+        src.push_back(K_[i].get());
+    }
+    if (g_.ndim == 3) {
+        pencils(scr, src, 2, +1);
+        pencils(scr, {}, 1, +1);
+    } else {
+        pencils(scr, src, 1, +1);
+    }
+    RowSyncArgs<T> a;
+    a.tw = tw(g_.nx);
+    for (int i = 0; i < nf_; ++i) {
+        a.fmain[i] = scr[i];
+        a.fnyq[i] = scr[i] + g_.nmain;
+    }
+    a.scale = T(1) / (T)g_.n;
+    a.nd = nd_;
+    std::vector<cpx<T>*> out;
+    for (int d = 0; d < nd_; ++d) {
+        dargs_[d].counter = dcount_[d]++;
+        std::memcpy(a.dc[d], dargs_[d].dc, sizeof(a.dc[d]));
+        a.dmain[d] = K_[nf_ + d].get();
+        a.dnyq[d] = K_[nf_ + d].get() + g_.nmain;
+        out.push_back(K_[nf_ + d].get());
+    }
+    a.rows = g_.rows;
+    a.nx = g_.nx;
+    a.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
+    HIPCHK(launch_rows_sync<T>(a, st_));
+    pencils(out, {}, 1, -1);
+    if (g_.ndim == 3) pencils(out, {}, 2, -1);
+}
+
 template <typename T>
 void Engine<T>::general_step_rk4() {
     // RK4.Step (rk4.go:29-127) with each stage's GetRHS, PrepareNextCorrection and
@@ -1635,6 +1681,10 @@ void Engine<T>::general_step_rk4() {
             K_[i].swap(KF_[i]);
             in.push_back(K_[i].get());
             out.push_back(F_[i].get());
+        }
+        if (stage < 3 && row_sync_) {
+            sync_from_spectra();
+            continue;
         }
         c2r(in, out, T(1) / (T)g_.n);
         if (stage < 3) sync_derived();

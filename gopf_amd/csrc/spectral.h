@@ -154,6 +154,27 @@ struct RowR2CArgs {
     double dc[kMaxArr][4][4] = {};
 };
 
+// x-row pass between stages of the general stepper: the inverse transform of one or two
+// fields' rows (one transform, the fields as A + i B) gives the real rows, the dense
+// derived-field polynomials (RowR2CArgs::dc, f0 = field 0, f1 = field 1) are evaluated on
+// them and transformed forward in pairs into the derived fields' half spectra, so the real
+// fields never go through HBM (written only where fout is set)
+template <typename T>
+struct RowSyncArgs {
+    const cpx<T>* tw = nullptr;
+    const cpx<T>* fmain[2] = {};         // fields' half spectra (fmain[1] null: one field)
+    const cpx<T>* fnyq[2] = {};
+    T* fout[2] = {};                     // optional real fields
+    T scale = T(1);                      // inverse scale of the fields (1 / N)
+    int nd = 0;                          // derived arrays, 1 .. kMaxArr
+    double dc[kMaxArr][4][4] = {};
+    cpx<T>* dmain[kMaxArr] = {};
+    cpx<T>* dnyq[kMaxArr] = {};
+    long rows = 0;
+    int nx = 0;
+    int TW = 16;
+};
+
 // x-row C2R of half spectra into real rows (scaled)
 template <typename T>
 struct RowC2RArgs {
@@ -202,6 +223,7 @@ template <typename T> hipError_t launch_pencils(const PencilArgs<T>& a, int n, i
 template <typename T> hipError_t launch_pencil_update(const PencilUpdateArgs<T>& a, int n, hipStream_t s);
 template <typename T> hipError_t launch_rows_r2c(const RowR2CArgs<T>& a, hipStream_t s);
 template <typename T> hipError_t launch_rows_c2r(const RowC2RArgs<T>& a, hipStream_t s);
+template <typename T> hipError_t launch_rows_sync(const RowSyncArgs<T>& a, hipStream_t s);
 template <typename T> hipError_t launch_rows_fused(const RowFusedArgs<T>& a, hipStream_t s);
 
 bool fft_size_supported(int n);      // sizes with a device transform

@@ -711,18 +711,24 @@ __device__ __forceinline__ void load_herm_pair_lds(cpx<T>* v, const cpx<T>* m, c
     __syncthreads();
 }
 
-// the dense derived-field polynomial of array q at element off (the k_derived formula)
+// sum_ij dc[i][j] x0^i x1^j in T, Horner in x0 of polynomials in x1 (the k_derived formula)
+template <typename T>
+__device__ __forceinline__ T dense_poly(const double (&dc)[4][4], T x0, T x1) {
+    T acc = T(0);
+#pragma unroll
+    for (int p = 3; p >= 0; --p) {
+        const T inner = (((T)dc[p][3] * x1 + (T)dc[p][2]) * x1 + (T)dc[p][1]) * x1 + (T)dc[p][0];
+        acc = acc * x0 + inner;
+    }
+    return acc;
+}
+
+// the dense derived-field polynomial of array q at element off
 template <typename T>
 __device__ __forceinline__ T row_poly(const RowR2CArgs<T>& a, int q, long off) {
     const T x0 = a.pf0[q][off];
     const T x1 = a.pf1[q] ? a.pf1[q][off] : T(0);
-    T acc = T(0);
-#pragma unroll
-    for (int p = 3; p >= 0; --p) {
-        const T inner = (((T)a.dc[q][p][3] * x1 + (T)a.dc[q][p][2]) * x1 + (T)a.dc[q][p][1]) * x1 + (T)a.dc[q][p][0];
-        acc = acc * x0 + inner;
-    }
-    return acc;
+    return dense_poly<T>(a.dc[q], x0, x1);
 }
 
 template <typename T, int N>
@@ -797,6 +803,60 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_c2r(RowC2RArgs<T> a) {
                 if (two) a.out[q + 1][off] = v[e].y * a.scale;
             }
         }
+    }
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(kMaxBlock) k_rows_sync(RowSyncArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
+    constexpr int V = plain_variant<N>();
+    constexpr int E = LineShape<N, V>::E;
+    const int n = (N > 0) ? N : a.nx;
+    const int TT = (N > 0) ? LineShape<N, V>::T : n;
+    const LineMap lm = line_map(true, a.TW, TT, n);
+    const long row = (long)blockIdx.x * a.TW + lm.l;
+    const bool valid = row < a.rows;
+    const bool two = a.fmain[1] != nullptr;
+    cpx<T> v[E];
+    {
+        cpx<T> A[E];
+        load_herm<T, E>(A, a.fmain[0], a.fnyq[0], row, n, lm.t, TT, valid);
+        if (two) {
+            load_herm<T, E>(v, a.fmain[1], a.fnyq[1], row, n, lm.t, TT, valid);
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = mk<T>(A[e].x - v[e].y, A[e].y + v[e].x);  // A + i B
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = A[e];
+        }
+    }
+    line_fft<T, N, +1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
+    // the real rows, rounded to T as the inverse pass stores them
+    T x0[E], x1[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        x0[e] = v[e].x * a.scale;
+        x1[e] = two ? v[e].y * a.scale : T(0);
+        if (valid) {
+            const long off = row * n + lm.t + TT * e;
+            if (a.fout[0]) a.fout[0][off] = x0[e];
+            if (two && a.fout[1]) a.fout[1][off] = x1[e];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kMaxArr; q += 2) {
+        if (q >= a.nd) break;
+        const bool dtwo = q + 1 < a.nd;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const T d0 = valid ? dense_poly<T>(a.dc[q], x0[e], x1[e]) : T(0);
+            const T d1 = (valid && dtwo) ? dense_poly<T>(a.dc[q + 1], x0[e], x1[e]) : T(0);
+            v[e] = mk<T>(d0, d1);
+        }
+        line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
+        split_store<T, E>(v, lds + lm.off, n, lm.t, TT, row, valid, a.dmain[q], a.dnyq[q],
+                          dtwo ? a.dmain[q + 1] : nullptr, dtwo ? a.dnyq[q + 1] : nullptr);
     }
 }
 
@@ -1029,6 +1089,18 @@ hipError_t launch_rows_c2r(const RowC2RArgs<T>& a, hipStream_t s) {
 }
 
 template <typename T>
+hipError_t launch_rows_sync(const RowSyncArgs<T>& a, hipStream_t s) {
+    if (a.nd < 1 || a.nd > kMaxArr || !a.fmain[0]) return hipErrorInvalidValue;
+    const long blocks = rows_blocks(a.rows, a.TW);
+    const int thr = a.TW * threads_per_line(a.nx);
+    const size_t smem = lds_elems(a.nx, a.TW) * sizeof(cpx<T>);
+    return dispatch_size(a.nx, [&](auto NC) -> hipError_t {
+        constexpr int NN = decltype(NC)::value;
+        return launch_k(k_rows_sync<T, NN>, dim3(blocks), dim3(thr), smem, s, a);
+    });
+}
+
+template <typename T>
 hipError_t launch_rows_fused(const RowFusedArgs<T>& a, hipStream_t s) {
     const long blocks = rows_fused_blocks(a.rows, a.TW);
     const int thr = a.TW * threads_per_line(a.nx, a.variant);
@@ -1052,6 +1124,7 @@ hipError_t launch_rows_fused(const RowFusedArgs<T>& a, hipStream_t s) {
     template hipError_t launch_pencil_update<T>(const PencilUpdateArgs<T>&, int, hipStream_t);       \
     template hipError_t launch_rows_r2c<T>(const RowR2CArgs<T>&, hipStream_t);                      \
     template hipError_t launch_rows_c2r<T>(const RowC2RArgs<T>&, hipStream_t);                      \
+    template hipError_t launch_rows_sync<T>(const RowSyncArgs<T>&, hipStream_t);                    \
     template hipError_t launch_rows_fused<T>(const RowFusedArgs<T>&, hipStream_t);
 GPF_INST(float)
 GPF_INST(double)
