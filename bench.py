@@ -255,6 +255,24 @@ def cpu_baseline_c4(W, N: int, steps: int):
                       f"to 192^3 up to the FFT's log factor"}
 
 
+def c4_traffic(N: int, s: int) -> dict:
+    """Algorithmic HBM bytes of one step of the two-field precipitate RK4 step, every array
+    counted once per pass that reads or writes it.  R = a real array (N^3 s bytes), H = a
+    half spectrum ((N/2+1) N^2 complex values), K = the elastic multiplier table ((N/2+1) N^2
+    doubles).  Per step: the derived fields' transform (rows with the polynomials inline,
+    y, z) 2R+4H read / 6H written; the elastic term (indicator, transform of H, K(k)
+    multiply, inverse, H'(phi)(S - 2 e* H), transform) 5R+8H+K / 3R+9H; eight RK4 stage
+    passes (conc 13H / 8H, phase 17H / 8H); three between-stage syncs (z, y inverse of both
+    fields, the fused row pass, y, z forward of both derived fields) 30H / 30H; the last
+    inverse (z, y, rows) 6H / 4H+2R; the lagged-field copy R / R."""
+    R = N ** 3 * s
+    H = (N // 2 + 1) * N * N * 2 * s
+    K = (N // 2 + 1) * N * N * 8
+    rd = 8 * R + 78 * H + K
+    wr = 6 * R + 65 * H
+    return {"read": rd, "write": wr, "total": rd + wr, "per_voxel_step": (rd + wr) / N ** 3}
+
+
 def run_c4(G, W, args):
     """Two-field precipitate with the Khachaturyan elastic term, 192^3, RK4, dt 0.01, 500 steps."""
     N, dt, steps = 192, 0.01, args.steps
@@ -273,7 +291,11 @@ def run_c4(G, W, args):
         s.Propagate(steps)
         s.Synchronize()
         el = time.perf_counter() - t0
-        res[prec] = {"value": N ** 3 * steps / el, "ms_per_step": 1e3 * el / steps}
+        tr = c4_traffic(N, 4 if prec == "fp32" else 8)
+        res[prec] = {"value": N ** 3 * steps / el, "ms_per_step": 1e3 * el / steps,
+                     "algorithmic_bytes_per_voxel_step": tr["per_voxel_step"],
+                     "achieved_GBs": tr["total"] * steps / el / 1e9,
+                     "frac_of_8TBs": tr["total"] * steps / el / 8e12}
         fields[prec] = (s.Field("conc").real, s.Field("phase").real)
         print(f"# c4 {prec}: {1e3 * el / steps:.3f} ms/step", file=sys.stderr, flush=True)
         del s
@@ -282,6 +304,8 @@ def run_c4(G, W, args):
     out = {"metric": "voxel-steps/s, 192^3 two-field elastic RK4", "value": res["fp32"]["value"],
            "unit": "voxel-steps/s", "ms_per_step": res["fp32"]["ms_per_step"], "dtype": "f32",
            "fp64": res["fp64"], "speedup_fp32_vs_fp64": res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"],
+           "algorithmic_bytes_per_voxel_step": res["fp32"]["algorithmic_bytes_per_voxel_step"],
+           "achieved_GBs": res["fp32"]["achieved_GBs"], "frac_of_8TBs": res["fp32"]["frac_of_8TBs"],
            "err_vs_fp64_rel_l2": err,
            "config": {"workload": "precipitate_elastic_3d_192", "grid": dims, "dt": dt, "steps": steps,
                       "stepper": "RK4", "elastic": "cubic C11=110 C12=60 C44=30, misfit 0.01 I",
