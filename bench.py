@@ -160,19 +160,9 @@ def run_c1(G, W, args):
     return out
 
 
-def c3_sim_inputs(W, s: int, n: int):
-    """Sim s: SplitMix64(1000 + s); (M, kappa) = exp(U[ln 0.1, ln 10]) from the first two
-    draws, the initial field 0.5 + U(-0.05, 0.05) from the next n."""
-    u = W.splitmix64_uniform(1000 + s, n + 2)
-    lo, hi = np.log(0.1), np.log(10.0)
-    M = float(np.exp(lo + (hi - lo) * u[0]))
-    kappa = float(np.exp(lo + (hi - lo) * u[1]))
-    return M, kappa, 0.5 + 0.1 * (u[2:] - 0.5)
-
-
 def run_c3(G, W, args):
     """384 CH sims: 128 of 128^2, 192 of 256^2, 64 of 512^2, FP32-bulk, dt 0.1, 500 steps."""
-    classes = [((128, 128), 128), ((256, 256), 192), ((512, 512), 64)]
+    classes = W.C3_CLASSES
     dt, steps = 0.1, args.steps
     m = G.Model()
     m.AddField("conc", n=16)
@@ -184,7 +174,7 @@ def run_c3(G, W, args):
     vox = 0
     for s in range(len(b)):
         n = int(np.prod(b.shapes[s]))
-        M, kappa, c0 = c3_sim_inputs(W, s, n)
+        M, kappa, c0 = W.c3_sim_inputs(s, n)
         for k, v in W.ch_scalars(M, kappa).items():
             b.SetScalar(s, k, v)
         b.SetField(s, c0)
@@ -202,7 +192,7 @@ def run_c3(G, W, args):
                       "params": "(M, kappa) = exp(U[ln 0.1, ln 10]), SplitMix64 seed 1000 + sim"},
            "layout": b.Tuning()}
     if not args.no_cpu:
-        M, kappa, c0 = c3_sim_inputs(W, 128, 256 * 256)
+        M, kappa, c0 = W.c3_sim_inputs(128, 256 * 256)
         out["cpu_baseline"] = cpu_baseline_ch2d(256, 20, dt, c0, W.ch_scalars(M, kappa),
                                                 "one 256^2 sim of the sweep (sim 128)")
     return out

@@ -35,6 +35,20 @@ def ch_scalars(M: float = 1.0, kappa: float = 1.0) -> dict:
     return {"m2": 2.0 * M, "m6": 6.0 * M, "m4": 4.0 * M, "mk": M * kappa}
 
 
+# batched sweep: 128 sims of 128^2, 192 of 256^2, 64 of 512^2
+C3_CLASSES = [((128, 128), 128), ((256, 256), 192), ((512, 512), 64)]
+
+
+def c3_sim_inputs(s: int, n: int):
+    """Sim s of the sweep: SplitMix64(1000 + s); (M, kappa) = exp(U[ln 0.1, ln 10]) from the
+    first two draws, the initial field 0.5 + U(-0.05, 0.05) from the next n."""
+    u = splitmix64_uniform(1000 + s, n + 2)
+    lo, hi = np.log(0.1), np.log(10.0)
+    M = float(np.exp(lo + (hi - lo) * u[0]))
+    kappa = float(np.exp(lo + (hi - lo) * u[1]))
+    return M, kappa, 0.5 + 0.1 * (u[2:] - 0.5)
+
+
 # ---------------------------------------------------------------------------
 # two-field strain precipitate (examples/strain_single_precipitate/main.go:14-113,
 # without CONSERVE_PREC_VOL): conc and phase, cubic elasticity, RK4

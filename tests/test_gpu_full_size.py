@@ -173,3 +173,50 @@ def test_precipitate_192_fp32_fp64_and_cubic_symmetry():
         assert err <= 1e-5
         assert sym <= 1e-10
         assert moved > 1e-4
+
+
+def test_batch_384_sims_full_size():
+    """The batched sweep at its full size (128 sims of 128^2, 192 of 256^2, 64 of 512^2,
+    FP32-bulk, dt 0.1, 50 steps): every sim conserves its mass, and the first and last sim of
+    each size class agree with a single FP64 solver of that sim (relative L2 <= 1e-5).
+
+    The state's k = 0 mode is the FP64 mean, which the Cahn-Hilliard step leaves unchanged;
+    the read-back field is an FP32 inverse transform plus that mean, and the rounding of its
+    up to 512^2 values puts about 1e-10 to 1e-9 on their sum, hence the 1e-8 bound."""
+    dt, steps = 0.1, 50
+    m = G.Model()
+    m.AddField("conc", n=16)
+    for k, v in W.ch_scalars().items():
+        m.AddScalar(k, v)
+    m.AddEquation(W.CH_EQUATION)
+    m.Init()
+    b = G.Batch(m, W.C3_CLASSES, dt, precision="fp32")
+    assert len(b) == 384
+    inputs = []
+    for s in range(len(b)):
+        M, kappa, c0 = W.c3_sim_inputs(s, int(np.prod(b.shapes[s])))
+        for k, v in W.ch_scalars(M, kappa).items():
+            b.SetScalar(s, k, v)
+        b.SetField(s, c0)
+        inputs.append((M, kappa, c0.sum()))
+    b.Propagate(steps)
+    worst = 0.0
+    for s in range(len(b)):
+        worst = max(worst, abs(b.Field(s).real.sum() - inputs[s][2]) / inputs[s][2])
+    print(f"384 sims, {steps} steps: worst mass drift {worst:.3g}", flush=True)
+    assert worst <= 1e-8
+    for s in (0, 127, 128, 319, 320, 383):
+        dims = list(b.shapes[s])
+        M, kappa, c0 = W.c3_sim_inputs(s, int(np.prod(dims)))
+        sm = G.Model()
+        sm.AddField("conc", c0)
+        for k, v in W.ch_scalars(M, kappa).items():
+            sm.AddScalar(k, v)
+        sm.AddEquation(W.CH_EQUATION)
+        sm.Init()
+        single = G.Solver(sm, dims, dt, precision="fp64")
+        single.StepperPropagate(steps)
+        err = relL2(b.Field(s).real, single.Field(0).real)
+        print(f"sim {s} {dims}: batch fp32 vs single fp64 rel L2 {err:.3g}", flush=True)
+        assert err <= 1e-5
+        del single
