@@ -1491,25 +1491,48 @@ KElasticArgs<T> Engine<T>::elastic_args(size_t e) const {
     return a;
 }
 
-// HomogeneousModulusLinElast: FFT(H'(phi) (IFFT(K H^) / N - 2 e* H(phi))), once per step
+// HomogeneousModulusLinElast: FFT(H'(phi) (IFFT(K H^) / N - 2 e* H(phi))), once per step.
+// The indicator H(phi) = 3 phi^2 - 2 phi^3 is evaluated inside the row transform of phi
+// (a dense one-field polynomial), and the inverse row transform of K H^, the pointwise
+// H'(phi) (S - 2 e* H(phi)) and its forward row transform are one k_rows_sync pass, so
+// neither H, S nor the real source goes through HBM.
 template <typename T>
 void Engine<T>::eval_elastic() {
     for (size_t e = 0; e < Tel_.size(); ++e) {
         const UserTerm& ut = mcopy_.user_table[kHomogeneousModulus][e];
-        T* H = rscratch(1);
-        HIPCHK(launch_indicator<T>(lag_[e].get(), H, g_.n, st_));
         cpx<T>* Hk = kscratch(kMaxArr + 1);
-        r2c({H}, {Hk}, T(1));
+        DerivedArgs<T> ind;
+        ind.nf = 1;
+        ind.f[0] = lag_[e].get();
+        ind.dense = 1;
+        ind.dc[2][0] = 3.0;
+        ind.dc[3][0] = -2.0;
+        const std::vector<const DerivedArgs<T>*> prog{&ind};
+        r2c({nullptr}, {Hk}, T(1), &prog);
         KElasticArgs<T> a = elastic_args(e);
         a.src = Hk;
         a.dst = Hk;
         a.ktab = Ktab_[e].get();
         HIPCHK(launch_kelastic<T>(a, st_));
-        T* S = rscratch(2);
-        c2r({Hk}, {S}, T(1) / (T)g_.n);
-        T* P = rscratch(3);
-        HIPCHK(launch_elastic_real<T>(lag_[e].get(), S, 2.0 * energy_density(ut.C, ut.misfit), P, g_.n, st_));
-        r2c({P}, {Tel_[e].get()}, T(1));
+        if (g_.ndim == 3) pencils({Hk}, {}, 2, +1);
+        pencils({Hk}, {}, 1, +1);
+        RowSyncArgs<T> r;
+        r.tw = tw(g_.nx);
+        r.fmain[0] = Hk;
+        r.fnyq[0] = Hk + g_.nmain;
+        r.scale = T(1) / (T)g_.n;
+        r.mode = 1;
+        r.phi = lag_[e].get();
+        r.two_estar = 2.0 * energy_density(ut.C, ut.misfit);
+        r.nd = 1;
+        r.dmain[0] = Tel_[e].get();
+        r.dnyq[0] = Tel_[e].get() + g_.nmain;
+        r.rows = g_.rows;
+        r.nx = g_.nx;
+        r.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
+        HIPCHK(launch_rows_sync<T>(r, st_));
+        pencils({Tel_[e].get()}, {}, 1, -1);
+        if (g_.ndim == 3) pencils({Tel_[e].get()}, {}, 2, -1);
     }
 }
 

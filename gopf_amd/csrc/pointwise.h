@@ -313,10 +313,6 @@ template <typename T> hipError_t launch_volcons_update(const double* px, long nb
                                                        double* st, hipStream_t s);
 // out = factor * sum_d w_d^2
 template <typename T> hipError_t launch_sqsum(const T* const* w, int nw, T factor, T* out, long n, hipStream_t s);
-// indicator H(phi) = 3 phi^2 - 2 phi^3  (homoLinElast.go:10-12)
-template <typename T> hipError_t launch_indicator(const T* phi, T* out, long n, hipStream_t s);
-// P = H'(phi) (S - 2 e* H(phi))
-template <typename T> hipError_t launch_elastic_real(const T* phi, const T* S, double two_estar, T* out, long n, hipStream_t s);
 template <typename T> hipError_t launch_copy(const T* src, T* dst, long n, hipStream_t s);
 // per-block FP64 sums of f(x) = sum_q poly[q] x^q and of x over a real array
 // (x is shifted by the FP64 offset before both sums)

@@ -567,24 +567,6 @@ __global__ void k_sqsum(const T* w0, const T* w1, const T* w2, int nw, T factor,
 }
 
 template <typename T>
-__global__ void k_indicator(const T* phi, T* out, long n) {
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const T x = phi[i];
-        out[i] = T(3) * x * x - T(2) * x * x * x;
-    }
-}
-
-template <typename T>
-__global__ void k_elastic_real(const T* phi, const T* S, double two_estar, T* out, long n) {
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const T x = phi[i];
-        const T H = T(3) * x * x - T(2) * x * x * x;
-        const T dH = T(6) * x - T(6) * x * x;
-        out[i] = dH * (S[i] - (T)two_estar * H);
-    }
-}
-
-template <typename T>
 __global__ void k_copy(const T* src, T* dst, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         dst[i] = src[i];
@@ -1057,16 +1039,6 @@ hipError_t launch_sqsum(const T* const* w, int nw, T factor, T* out, long n, hip
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_indicator(const T* phi, T* out, long n, hipStream_t s) {
-    hipLaunchKernelGGL(k_indicator<T>, dim3(grid_for(n)), dim3(256), 0, s, phi, out, n);
-    return hipGetLastError();
-}
-template <typename T>
-hipError_t launch_elastic_real(const T* phi, const T* S, double two_estar, T* out, long n, hipStream_t s) {
-    hipLaunchKernelGGL(k_elastic_real<T>, dim3(grid_for(n)), dim3(256), 0, s, phi, S, two_estar, out, n);
-    return hipGetLastError();
-}
-template <typename T>
 hipError_t launch_copy(const T* src, T* dst, long n, hipStream_t s) {
     hipLaunchKernelGGL(k_copy<T>, dim3(grid_for(n)), dim3(256), 0, s, src, dst, n);
     return hipGetLastError();
@@ -1160,8 +1132,6 @@ hipError_t launch_reduce2(const double* a, long na, double ca, const double* b, 
     template hipError_t launch_kelastic<T>(const KElasticArgs<T>&, hipStream_t);                              \
     template hipError_t launch_kelastic_table<T>(const KElasticArgs<T>&, double*, hipStream_t);               \
     template hipError_t launch_sqsum<T>(const T* const*, int, T, T*, long, hipStream_t);                      \
-    template hipError_t launch_indicator<T>(const T*, T*, long, hipStream_t);                                 \
-    template hipError_t launch_elastic_real<T>(const T*, const T*, double, T*, long, hipStream_t);            \
     template hipError_t launch_copy<T>(const T*, T*, long, hipStream_t);                                      \
     template hipError_t launch_real_sums<T>(const T*, long, double, int, const double*, double*, double*,     \
                                             long, hipStream_t);                                               \

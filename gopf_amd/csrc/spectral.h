@@ -166,6 +166,12 @@ struct RowSyncArgs {
     const cpx<T>* fnyq[2] = {};
     T* fout[2] = {};                     // optional real fields
     T scale = T(1);                      // inverse scale of the fields (1 / N)
+    // mode 1, the elastic source of HomogeneousModulusLinElast (homoLinElast.go:91-127): one
+    // array, S = the inverse row of fmain[0], and dmain[0] receives the forward row of
+    // H'(phi) (S - two_estar H(phi)), phi read from the real rows of phi
+    int mode = 0;
+    const T* phi = nullptr;
+    double two_estar = 0.0;
     int nd = 0;                          // derived arrays, 1 .. kMaxArr
     double dc[kMaxArr][4][4] = {};
     cpx<T>* dmain[kMaxArr] = {};

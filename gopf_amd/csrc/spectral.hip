@@ -844,6 +844,24 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_sync(RowSyncArgs<T> a) {
             if (two && a.fout[1]) a.fout[1][off] = x1[e];
         }
     }
+    if (a.mode == 1) {
+        // P = H'(phi) (S - 2 e* H(phi)) in T on the inverse row, H = 3 phi^2 - 2 phi^3
+        // (homoLinElast.go:10-12, 91-127)
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            T P = T(0);
+            if (valid) {
+                const T x = a.phi[row * n + lm.t + TT * e];
+                const T H = T(3) * x * x - T(2) * x * x * x;
+                const T dH = T(6) * x - T(6) * x * x;
+                P = dH * (x0[e] - (T)a.two_estar * H);
+            }
+            v[e] = mk<T>(P, T(0));
+        }
+        line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
+        split_store<T, E>(v, lds + lm.off, n, lm.t, TT, row, valid, a.dmain[0], a.dnyq[0], nullptr, nullptr);
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < kMaxArr; q += 2) {
         if (q >= a.nd) break;
@@ -1091,6 +1109,7 @@ hipError_t launch_rows_c2r(const RowC2RArgs<T>& a, hipStream_t s) {
 template <typename T>
 hipError_t launch_rows_sync(const RowSyncArgs<T>& a, hipStream_t s) {
     if (a.nd < 1 || a.nd > kMaxArr || !a.fmain[0]) return hipErrorInvalidValue;
+    if (a.mode == 1 && (!a.phi || a.nd != 1 || a.fmain[1])) return hipErrorInvalidValue;
     const long blocks = rows_blocks(a.rows, a.TW);
     const int thr = a.TW * threads_per_line(a.nx);
     const size_t smem = lds_elems(a.nx, a.TW) * sizeof(cpx<T>);
