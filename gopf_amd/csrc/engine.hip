@@ -1728,13 +1728,35 @@ void Engine<T>::implicit_residual(const double* x, double* out) {
         const long n = g_.n;
         std::vector<const T*> in;
         std::vector<cpx<T>*> ko;
-        for (int i = 0; i < nf_; ++i) {
-            HIPCHK(launch_copy<T>(x + i * n, F_[i].get(), n, st_));
-            in.push_back(F_[i].get());
-            ko.push_back(K_[i].get());
+        if (row_sync_ && nf_ + nd_ <= kMaxArr) {
+            // the fields' transform and the dense derived fields' (their polynomials of x
+            // evaluated in the row pass) in one row launch and one launch per pencil axis;
+            // the real fields are not needed until the step takes the solution
+            std::vector<DerivedArgs<T>> px(nd_);
+            std::vector<const DerivedArgs<T>*> prog;
+            for (int i = 0; i < nf_; ++i) {
+                in.push_back(x + i * n);
+                ko.push_back(K_[i].get());
+                prog.push_back(nullptr);
+            }
+            for (int d = 0; d < nd_; ++d) {
+                dargs_[d].counter = dcount_[d]++;
+                px[d] = dargs_[d];
+                for (int f = 0; f < nf_; ++f) px[d].f[f] = x + f * n;
+                in.push_back(nullptr);
+                ko.push_back(K_[nf_ + d].get());
+                prog.push_back(&px[d]);
+            }
+            r2c(in, ko, T(1), &prog);
+        } else {
+            for (int i = 0; i < nf_; ++i) {
+                HIPCHK(launch_copy<T>(x + i * n, F_[i].get(), n, st_));
+                in.push_back(F_[i].get());
+                ko.push_back(K_[i].get());
+            }
+            r2c(in, ko, T(1));
+            sync_derived();
         }
-        r2c(in, ko, T(1));
-        sync_derived();
         std::vector<const cpx<T>*> rin;
         std::vector<T*> rout;
         for (int i = 0; i < nf_; ++i) {
