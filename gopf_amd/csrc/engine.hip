@@ -2369,10 +2369,15 @@ void Engine<T>::adjoint(long nsteps, long every, const double* target, double* l
         copy_k(SS[len].get(), S_.get());
         if (split_mean_) HIPCHK(launch_copy<double>(mean_.get(), segmean.get() + len, 1, st_));
     };
+    // Unless a scalar's gradient needs the source-derivative sums <q, dg/ds(c_n)> (kept in
+    // k_adj_lambda), the inverse rows of P and Q, lam_n = p + g'(c_n) q and the forward
+    // rows of lam_n are one k_rows_sync pass (mode 2), whose output is the next reverse
+    // step's FFT(lam); the first comes from lam_T before the sweep.
+    const bool fuse_rev = !(nscal > 0 && any_g) && ndg <= kMaxPoly;
     auto reverse = [&](long len) {
         for (long k = len - 1; k >= 0; --k) {
             const T* cn = need_real ? R[k].get() : nullptr;
-            r2c({lam.get()}, {Lk.get()}, T(1));
+            if (!fuse_rev) r2c({lam.get()}, {Lk.get()}, T(1));
             if (nscal > 0 && (need_c1 || need_c0)) {
                 // P, Q and the derivatives of D and E against the transforms of c_{n+1} and c_n
                 a.C1 = need_c1 ? SS[k + 1].get() : nullptr;
@@ -2382,6 +2387,32 @@ void Engine<T>::adjoint(long nsteps, long every, const double* target, double* l
                 HIPCHK(launch_adj_grad<T>(a, nblk, st_));
             } else {
                 HIPCHK(launch_adj_mult<T>(a, st_));
+            }
+            if (fuse_rev) {
+                if (g_.ndim == 3) pencils({Pk.get(), Qk.get()}, {}, 2, +1);
+                pencils({Pk.get(), Qk.get()}, {}, 1, +1);
+                RowSyncArgs<T> r;
+                r.tw = tw(g_.nx);
+                r.fmain[0] = Pk.get();
+                r.fnyq[0] = Pk.get() + g_.nmain;
+                r.fmain[1] = Qk.get();
+                r.fnyq[1] = Qk.get() + g_.nmain;
+                r.fout[0] = lam.get();
+                r.scale = T(1) / (T)n;
+                r.mode = 2;
+                r.phi = cn;
+                r.npoly = ndg;
+                for (int q = 0; q < ndg; ++q) r.gd[q] = dg[q];
+                r.nd = 1;
+                r.dmain[0] = Lk.get();
+                r.dnyq[0] = Lk.get() + g_.nmain;
+                r.rows = g_.rows;
+                r.nx = g_.nx;
+                r.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
+                HIPCHK(launch_rows_sync<T>(r, st_));
+                pencils({Lk.get()}, {}, 1, -1);
+                if (g_.ndim == 3) pencils({Lk.get()}, {}, 2, -1);
+                continue;
             }
             c2r({Pk.get(), Qk.get()}, {pr.get(), qr.get()}, T(1) / (T)n);
             if (need_real) {
@@ -2404,6 +2435,7 @@ void Engine<T>::adjoint(long nsteps, long every, const double* target, double* l
         HIPCHK(e);
         return ge;
     };
+    if (fuse_rev) r2c({lam.get()}, {Lk.get()}, T(1));
     hipGraphExec_t g_rec = nullptr, g_rev = nullptr;
     bool warm = false;  // the first segment runs eagerly: it creates every lazily built table
     for (long seg = nck - 1; seg >= 0; --seg) {

@@ -169,9 +169,14 @@ struct RowSyncArgs {
     // mode 1, the elastic source of HomogeneousModulusLinElast (homoLinElast.go:91-127): one
     // array, S = the inverse row of fmain[0], and dmain[0] receives the forward row of
     // H'(phi) (S - two_estar H(phi)), phi read from the real rows of phi
+    // mode 2, the adjoint's lam = p + g'(c) q (p, q the inverse rows of fmain[0], fmain[1];
+    // c the real rows of phi, or g' = 0 when phi is null; g' = sum_k gd[k] c^k in FP64):
+    // fout[0] receives lam and dmain[0] its forward row
     int mode = 0;
     const T* phi = nullptr;
     double two_estar = 0.0;
+    int npoly = 0;
+    double gd[kMaxPoly] = {};
     int nd = 0;                          // derived arrays, 1 .. kMaxArr
     double dc[kMaxArr][4][4] = {};
     cpx<T>* dmain[kMaxArr] = {};

@@ -838,11 +838,32 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_sync(RowSyncArgs<T> a) {
     for (int e = 0; e < E; ++e) {
         x0[e] = v[e].x * a.scale;
         x1[e] = two ? v[e].y * a.scale : T(0);
-        if (valid) {
+        if (valid && a.mode == 0) {
             const long off = row * n + lm.t + TT * e;
             if (a.fout[0]) a.fout[0][off] = x0[e];
             if (two && a.fout[1]) a.fout[1][off] = x1[e];
         }
+    }
+    if (a.mode == 2) {
+        // the k_adj_lambda formula on the inverse rows
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            T lam = T(0);
+            if (valid) {
+                const long off = row * n + lm.t + TT * e;
+                double d = 0.0;
+                if (a.phi) {
+                    const double x = (double)a.phi[off];
+                    for (int k = a.npoly - 1; k >= 0; --k) d = d * x + a.gd[k];
+                }
+                lam = (T)((double)x0[e] + d * (double)x1[e]);
+                a.fout[0][off] = lam;
+            }
+            v[e] = mk<T>(lam, T(0));
+        }
+        line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
+        split_store<T, E>(v, lds + lm.off, n, lm.t, TT, row, valid, a.dmain[0], a.dnyq[0], nullptr, nullptr);
+        return;
     }
     if (a.mode == 1) {
         // P = H'(phi) (S - 2 e* H(phi)) in T on the inverse row, H = 3 phi^2 - 2 phi^3
@@ -1110,6 +1131,8 @@ template <typename T>
 hipError_t launch_rows_sync(const RowSyncArgs<T>& a, hipStream_t s) {
     if (a.nd < 1 || a.nd > kMaxArr || !a.fmain[0]) return hipErrorInvalidValue;
     if (a.mode == 1 && (!a.phi || a.nd != 1 || a.fmain[1])) return hipErrorInvalidValue;
+    if (a.mode == 2 && (!a.fmain[1] || !a.fout[0] || a.nd != 1 || a.npoly < 0 || a.npoly > kMaxPoly))
+        return hipErrorInvalidValue;
     const long blocks = rows_blocks(a.rows, a.TW);
     const int thr = a.TW * threads_per_line(a.nx);
     const size_t smem = lds_elems(a.nx, a.TW) * sizeof(cpx<T>);
