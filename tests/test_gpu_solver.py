@@ -272,6 +272,40 @@ def test_two_field_rk4_parity(oracle_lib):
             assert maxrel(s.Field(f).real, o.field(f).real) <= 1e-10, (stepper, f)
 
 
+@pytest.mark.parametrize("case", ["one_field_3d", "two_fields_odd_2d"])
+def test_rk4_row_sync_parity(oracle_lib, case):
+    """RK4 with every derived field a dense polynomial, so the stages' row pass joins the
+    fields' inverse and the derived fields' forward transforms: one field (no second row
+    in the transform; derived conc^2, conc^3) at 24^3, and two fields with three derived
+    fields (psi^2, phi^2, phi*psi: an unpaired last one) on a 32 x 48 grid, against the
+    oracle."""
+    if case == "one_field_3d":
+        dims, n = [24, 24, 24], 24 ** 3
+        rng = np.random.default_rng(21)
+        fields = {"conc": 0.5 + 0.05 * rng.standard_normal(n)}
+        eqs = ["dconc/dt = ka*LAP conc - LAP conc^2 + kb*LAP conc^3 - conc^2*ka"]
+        scal = {"ka": 0.3, "kb": 0.5}
+        dt, steps = 0.02, 12
+    else:
+        dims, n = [32, 48], 32 * 48
+        rng = np.random.default_rng(22)
+        fields = {"phi": 0.5 + 0.05 * rng.standard_normal(n), "psi": 0.3 + 0.05 * rng.standard_normal(n)}
+        eqs = ["dphi/dt = ka*LAP phi - ka*phi^2 + kb*LAP psi^2", "dpsi/dt = kb*LAP psi + phi*psi - psi^2"]
+        scal = {"ka": 0.5, "kb": 1.0}
+        dt, steps = 0.05, 20
+    pm, om = pair_models(fields, scal, eqs, n)
+    s = G.Solver(pm, dims, dt, stepper="rk4")
+    assert not s.fused
+    s.StepperPropagate(steps)
+    from oracle.engine import OracleSolver
+    o = OracleSolver(om, dims, fields, dt, "rk4")
+    o.stepper_propagate(steps)
+    for f in fields:
+        err = maxrel(s.Field(f).real, o.field(f).real)
+        print(case, f, err, flush=True)
+        assert err <= 1e-10, (case, f)
+
+
 def test_short_field_name_panics_like_the_reference():
     m = G.Model()
     m.AddField("a", n=16)
