@@ -79,6 +79,57 @@ def time_steps(s, steps):
     return time.perf_counter() - t0
 
 
+class StreamTimer:
+    """HIP events recorded on the solver's own stream around the timed steps."""
+
+    def __init__(self):
+        import ctypes
+        self.C = ctypes
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.e0, self.e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        for e in (self.e0, self.e1):
+            if self.hip.hipEventCreate(ctypes.byref(e)) != 0:
+                raise RuntimeError("hipEventCreate failed")
+
+    def time(self, s, steps):
+        """(host seconds, device seconds) of `steps` steps of solver s."""
+        C = self.C
+        stream = C.c_void_p(s.Stream())
+        s.Synchronize()
+        t0 = time.perf_counter()
+        if self.hip.hipEventRecord(self.e0, stream) != 0:
+            raise RuntimeError("hipEventRecord failed")
+        s.StepperPropagate(steps)
+        if self.hip.hipEventRecord(self.e1, stream) != 0:
+            raise RuntimeError("hipEventRecord failed")
+        s.Synchronize()
+        host = time.perf_counter() - t0
+        if self.hip.hipEventSynchronize(self.e1) != 0:
+            raise RuntimeError("hipEventSynchronize failed")
+        ms = C.c_float()
+        if self.hip.hipEventElapsedTime(C.byref(ms), self.e0, self.e1) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return host, ms.value / 1e3
+
+
+def host_cpu():
+    """Host core count and CPU model, for the CPU baseline's record."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"nproc": avail, "cpu_model": model}
+
+
 def cpu_baseline(n_side: int, dt: float, c0: np.ndarray):
     """FP64 oracle (restatement of the reference's Euler step with complex128 FFTs) on one core."""
     from oracle import engine as E
@@ -97,9 +148,12 @@ def cpu_baseline(n_side: int, dt: float, c0: np.ndarray):
     t0 = time.perf_counter()
     o.stepper_propagate(1)
     el = time.perf_counter() - t0
-    return {"value": n / el, "unit": "voxel-steps/s", "cores": 1, "kind": "port",
-            "sample": f"1 Euler step of the {n_side}^3 CH workload by the FP64 C oracle "
-                      f"(3 forward + 1 inverse complex128 3D FFT, cmplx.Pow derived fields), {el:.1f} s"}
+    out = {"value": n / el, "unit": "voxel-steps/s", "cores": 1, "kind": "port",
+           "label": "restatement, not reference (the Go+FFTW reference cannot be built here)",
+           "sample": f"1 Euler step of the {n_side}^3 CH workload by the FP64 C oracle "
+                     f"(3 forward + 1 inverse complex128 3D FFT, cmplx.Pow derived fields), {el:.1f} s"}
+    out.update(host_cpu())
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -521,9 +575,11 @@ def main() -> int:
     copy_gbs = G.CopyBandwidth(copy_bytes, reps=10)
     print(f"# device copy of {copy_bytes} B: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
     print(f"# building FP32-bulk solver {dims}", file=sys.stderr, flush=True)
+    timer = StreamTimer()
     s32 = build_solver(G, W, n, dims, c0, "fp32", args.dt)
-    s32.StepperPropagate(args.warmup)   # the first step runs the lines-per-block autotuner
-    el32 = time_steps(s32, args.steps)
+    prep32 = s32.Prepare(args.steps)    # autotuner and step-graph capture, outside the timed region
+    s32.StepperPropagate(args.warmup)
+    el32, dev32 = timer.time(s32, args.steps)
     # per-pass device times over a second timed region
     tsteps = min(args.steps, 200)
     s32.EnableTiming(True)
@@ -531,14 +587,17 @@ def main() -> int:
     passes = s32.PassTimes()
     s32.EnableTiming(False)
     tuning = s32.Tuning()
-    print(f"# fp32: {1e3 * el32 / args.steps:.4f} ms/step; passes {passes}", file=sys.stderr, flush=True)
+    print(f"# fp32: {1e3 * el32 / args.steps:.4f} ms/step (events {1e3 * dev32 / args.steps:.4f}); "
+          f"prepare {prep32:.1f} ms; passes {passes}", file=sys.stderr, flush=True)
     res = {}
     if not args.no_fp64:
         s64 = build_solver(G, W, n, dims, c0, "fp64", args.dt)
+        prep64 = s64.Prepare(args.steps)
         s64.StepperPropagate(args.warmup)
-        el64 = time_steps(s64, args.steps)
+        el64, dev64 = timer.time(s64, args.steps)
         del s64
-        print(f"# fp64: {1e3 * el64 / args.steps:.4f} ms/step", file=sys.stderr, flush=True)
+        print(f"# fp64: {1e3 * el64 / args.steps:.4f} ms/step (events {1e3 * dev64 / args.steps:.4f}); "
+              f"prepare {prep64:.1f} ms", file=sys.stderr, flush=True)
         # accuracy at the stated step count, from the same initial field
         del s32
         e32 = build_solver(G, W, n, dims, c0, "fp32", args.dt)
@@ -553,8 +612,10 @@ def main() -> int:
         del e64
         m0 = float(np.sum(c0))
         res = {
-            "fp64": {"value": n * args.steps / el64, "ms_per_step": 1e3 * el64 / args.steps},
+            "fp64": {"value": n * args.steps / el64, "ms_per_step": 1e3 * el64 / args.steps,
+                     "ms_per_step_events": 1e3 * dev64 / args.steps, "prepare_ms": prep64},
             "speedup_fp32_vs_fp64": el64 / el32,
+            "speedup_fp32_vs_fp64_events": dev64 / dev32,
             "err_vs_fp64": {"steps": args.err_steps,
                             "rel_l2": float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64)),
                             "max_rel": float(np.abs(f32 - f64).max() / np.abs(f64).max())},
@@ -587,8 +648,12 @@ def main() -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"cahn_hilliard_3d_{N}", "grid": dims, "dt": args.dt, "M": 1.0, "kappa": 1.0,
                    "stepper": "semi-implicit Euler", "init": "0.5 + U(-0.05, 0.05), SplitMix64 seed 1",
-                   "precision": "fp32 bulk; fp64 k=0 mode, denominators, reductions"},
+                   "precision": "fp32 arrays and transforms; fp64 k=0 mode (mean), fp64-built "
+                                "implicit denominators and update multipliers (rounded to fp32), "
+                                "fp64 reductions"},
         "roofline": roof, "cpu_baseline": cpu,
+        "ms_per_step_events": 1e3 * dev32 / args.steps, "prepare_ms": prep32,
+        "passes_sum_ms": sum(v[0] / v[1] for v in passes.values()),
     }
     out.update(res)
     print(json.dumps(out))

@@ -23,6 +23,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgopf_amd.so")
 
+GPF_OK, GPF_EINVAL, GPF_EPARSE, GPF_EDEVICE, GPF_ESTATE = 0, 1, 2, 3, 4
 GPF_FP64, GPF_FP32 = 0, 1
 GPF_EULER, GPF_RK4, GPF_IMPLICIT_EULER, GPF_SDD = 0, 1, 2, 3
 GPF_IO_FLOAT64, GPF_IO_UINT8 = 0, 1
@@ -93,6 +94,7 @@ def _declare(L) -> None:
         "gpf_solver_destroy": (None, [_vp]),
         "gpf_solver_propagate": (C.c_int, [_vp, C.c_int]),
         "gpf_solver_stepper_propagate": (C.c_int, [_vp, C.c_int]),
+        "gpf_solver_prepare": (C.c_int, [_vp, C.c_int, _dp]),
         "gpf_solver_get_time": (C.c_int, [_vp, _dp]),
         "gpf_solver_get_field": (C.c_int, [_vp, C.c_int, _dp, C.c_long]),
         "gpf_solver_set_field": (C.c_int, [_vp, C.c_int, _dp, C.c_long]),
@@ -460,6 +462,14 @@ class Solver:
 
     def StepperPropagate(self, nsteps: int) -> None:
         _check(lib().gpf_solver_stepper_propagate(self._h, int(nsteps)))
+
+    def Prepare(self, nsteps: int = 0) -> float:
+        """Autotuner, monitor buffers and the replayed step graph ahead of the first step
+        (no reference counterpart); the fields and the time do not change.  Returns the host
+        milliseconds it took."""
+        ms = C.c_double()
+        _check(lib().gpf_solver_prepare(self._h, int(nsteps), C.byref(ms)))
+        return ms.value
 
     def Solve(self, nepochs: int, nsteps: int, callbacks=()) -> None:
         """Solver.Solve (solver.go:106-120) with host callbacks cb(solver, epoch); files of

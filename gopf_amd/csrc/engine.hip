@@ -24,6 +24,7 @@
 #include <cmath>
 #include <complex>
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -350,6 +351,10 @@ public:
     virtual ~EngineBase() = default;
     virtual void step() = 0;
     virtual void steps(long n) = 0;  // n steps; the fused path replays captured graphs
+    // everything the first steps would set up, without advancing the state: the
+    // autotuner, the monitor buffers for `upcoming` steps and the replayed step graph;
+    // returns the host milliseconds it took
+    virtual double prepare(long upcoming) = 0;
     virtual void on_step_finished() = 0;
     virtual void get_field(int idx, double* out) = 0;
     virtual void set_field(int idx, const double* in) = 0;
@@ -386,6 +391,7 @@ public:
 
     void step() override;
     void steps(long n) override;
+    double prepare(long upcoming) override;
     void on_step_finished() override;
     void get_field(int idx, double* out) override;
     void set_field(int idx, const double* in) override;
@@ -432,6 +438,7 @@ private:
     void fused_step();
     void prepare_fused(long upcoming);  // allocations ahead of launches (never during graph capture)
     void enqueue_fused();               // the launches of one step (capturable)
+    void capture_graph();               // instantiates graph_: kGraphSteps steps of enqueue_fused
     void drop_graph();
     hipGraphExec_t graph_ = nullptr;
     bool graph_mon_ = false;  // the monitor setting the graph was captured with
@@ -1371,6 +1378,40 @@ void Engine<T>::fused_step() {
     if (mon_) ++series_n_;
 }
 
+template <typename T>
+void Engine<T>::capture_graph() {
+    if (graph_ && graph_mon_ != mon_) drop_graph();
+    if (graph_) return;
+    graph_mon_ = mon_;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < kGraphSteps; ++k) enqueue_fused();
+    HIPCHK(hipStreamEndCapture(st_, &g));
+    const hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(e);
+    // the executable graph's device-side copy is made now, not at its first launch
+    HIPCHK(hipGraphUpload(graph_, st_));
+}
+
+// The autotuner (on scratch copies), the monitor's buffers and the step graph, ahead of
+// the first step, so that a caller's timed steps replay graphs only.  The state is not
+// advanced: the tuner's prime() recomputes W from S, which it already holds.
+template <typename T>
+double Engine<T>::prepare(long upcoming) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (fp_.ok) {
+        if (autotune_ && !tuned_) {
+            autotune();
+            drop_graph();
+        }
+        prepare_fused(std::max(upcoming, 0L));
+        if (!timing_ && !c_out_) capture_graph();
+        HIPCHK(hipStreamSynchronize(st_));
+    }
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 // n fused steps: launch-bound grids replay a captured graph of kGraphSteps steps (kernels
 // only: no memset or allocation is captured; the series slot comes from a device counter)
 template <typename T>
@@ -1387,18 +1428,8 @@ void Engine<T>::steps(long n) {
         for (long i = 0; i < n; ++i) fused_step();
         return;
     }
-    if (graph_ && graph_mon_ != mon_) drop_graph();
     prepare_fused(n);
-    if (!graph_) {
-        graph_mon_ = mon_;
-        hipGraph_t g = nullptr;
-        HIPCHK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < kGraphSteps; ++k) enqueue_fused();
-        HIPCHK(hipStreamEndCapture(st_, &g));
-        const hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        HIPCHK(e);
-    }
+    capture_graph();
     while (n >= kGraphSteps) {
         HIPCHK(hipGraphLaunch(graph_, st_));
         n -= kGraphSteps;
@@ -2484,6 +2515,7 @@ class FFTBase {
 public:
     virtual ~FFTBase() = default;
     virtual void exec(double* data, int dir) = 0;
+    virtual long size() const = 0;
 };
 
 template <typename T>
@@ -2497,23 +2529,13 @@ public:
         for (int v : {nx_, ny_, nz_})
             if (v > 1 && !fft_size_supported(v)) throw Error(GPF_EINVAL, "unsupported transform length " + std::to_string(v));
         n_ = (long)nx_ * ny_ * nz_;
-        HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
-        d_.alloc(n_);
-        for (int v : {nx_, ny_, nz_}) {
-            if (v <= 1) continue;
-            bool have = false;
-            for (auto& p : tw_) have |= (p.first == v);
-            if (have) continue;
-            auto h = twiddle_host<T>(v);
-            Buf<cpx<T>> b(v);
-            HIPCHK(hipMemcpy(b.get(), h.data(), v * sizeof(cpx<T>), hipMemcpyHostToDevice));
-            tw_.emplace_back(v, std::move(b));
-        }
     }
     ~FullFFT() override {
         if (st_) (void)hipStreamDestroy(st_);
     }
+    long size() const override { return n_; }
     void exec(double* data, int dir) override {
+        ensure();
         std::vector<cpx<T>> h(n_);
         for (long i = 0; i < n_; ++i) {
             h[i].x = (T)data[2 * i];
@@ -2544,6 +2566,23 @@ public:
     }
 
 private:
+    // device resources on first use: the plan itself (NewFFTW) is host data, so a
+    // length mismatch is reported before anything touches the device
+    void ensure() {
+        if (st_) return;
+        HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+        d_.alloc(n_);
+        for (int v : {nx_, ny_, nz_}) {
+            if (v <= 1) continue;
+            bool have = false;
+            for (auto& p : tw_) have |= (p.first == v);
+            if (have) continue;
+            auto h = twiddle_host<T>(v);
+            Buf<cpx<T>> b(v);
+            HIPCHK(hipMemcpy(b.get(), h.data(), v * sizeof(cpx<T>), hipMemcpyHostToDevice));
+            tw_.emplace_back(v, std::move(b));
+        }
+    }
     void pass(const Seg& s, int n, int dir) {
         PencilArgs<T> a;
         for (auto& p : tw_)
@@ -2859,6 +2898,15 @@ int gpf_solver_propagate(gpf_solver* s, int nsteps) {
             if (s->stepper != GPF_RK4) s->current_step++;
             s->eng->on_step_finished();
         }
+    });
+}
+
+int gpf_solver_prepare(gpf_solver* s, int nsteps, double* ms) {
+    return guarded([&] {
+        if (!s || nsteps < 0) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        const double t = s->eng->prepare(nsteps);
+        if (ms) *ms = t;
     });
 }
 
@@ -3229,7 +3277,6 @@ int gpf_batch_tuning(gpf_batch* b, int* tws, int maxn) {
 int gpf_fft_create(const int* dims, int ndim, int precision, int device, gpf_fft** out) {
     return guarded([&] {
         if (!dims || !out) throw Error(GPF_EINVAL, "bad argument");
-        set_device(device);
         auto f = std::make_unique<gpf_fft>();
         f->device = device;
         if (precision == GPF_FP32) f->impl.reset(new FullFFT<float>(dims, ndim));
@@ -3238,12 +3285,19 @@ int gpf_fft_create(const int* dims, int ndim, int precision, int device, gpf_fft
     });
 }
 
-void gpf_fft_destroy(gpf_fft* f) { delete f; }
+void gpf_fft_destroy(gpf_fft* f) {
+    if (!f) return;
+    try {
+        set_device(f->device);
+    } catch (...) {
+    }
+    delete f;
+}
 
 int gpf_fft_forward(gpf_fft* f, double* data, long n) {
     return guarded([&] {
         if (!f || !data) throw Error(GPF_EINVAL, "bad argument");
-        (void)n;
+        if (n != f->impl->size()) throw Error(GPF_EINVAL, "buffer length does not match the plan");
         set_device(f->device);
         f->impl->exec(data, -1);
     });
@@ -3252,7 +3306,7 @@ int gpf_fft_forward(gpf_fft* f, double* data, long n) {
 int gpf_fft_inverse(gpf_fft* f, double* data, long n) {
     return guarded([&] {
         if (!f || !data) throw Error(GPF_EINVAL, "bad argument");
-        (void)n;
+        if (n != f->impl->size()) throw Error(GPF_EINVAL, "buffer length does not match the plan");
         set_device(f->device);
         f->impl->exec(data, +1);
     });

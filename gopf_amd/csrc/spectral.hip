@@ -392,70 +392,6 @@ __device__ __forceinline__ void update_multipliers(const PencilUpdateArgs<T>& u,
     }
 }
 
-template <int DEG>
-__device__ __forceinline__ float horner_f(const float* c, float L) {
-    float v = c[DEG];
-#pragma unroll
-    for (int p = DEG - 1; p >= 0; --p) v = v * L + c[p];
-    return v;
-}
-
-// FP32-bulk multipliers without a filter: the same formulas in FP32 arithmetic (s1 and s2
-// are rounded to FP32 in any case; m1 - 1 is formed without cancellation, so its relative
-// error stays a few FP32 ulps at every k).  The k = 0 values, which advance the FP64 mean,
-// come from the L^0 coefficients in FP64.
-template <int E, int DEG>
-__device__ __forceinline__ void update_multipliers_f32(const PencilUpdateArgs<float>& u, const MultLine& ml,
-                                                       const double* dsim, const double* lsim, float* s1a,
-                                                       float* s2a, double& m1_0, double& m2_0) {
-    // coefficients: the argument struct's, read at fixed indices, then the sim's row if any
-    // (never a pointer that may point into the argument struct: that copies it to scratch)
-    double dd[DEG + 1], dl[DEG + 1];
-#pragma unroll
-    for (int p = 0; p <= DEG; ++p) {
-        dd[p] = u.dd[p];
-        dl[p] = u.dl[p];
-    }
-    if (dsim) {
-#pragma unroll
-        for (int p = 0; p <= DEG; ++p) {
-            dd[p] = dsim[p];
-            dl[p] = lsim[p];
-        }
-    }
-    float dc[DEG + 1], lc[DEG + 1];
-#pragma unroll
-    for (int p = 0; p <= DEG; ++p) {
-        dc[p] = (float)dd[p];
-        lc[p] = (float)dl[p];
-    }
-    {
-        const double D0 = dd[0], E0 = dl[0];
-        const double A0 = 1.0 / (1.0 - u.dt * D0);
-        m1_0 = 1.0 + u.dt * (E0 + D0) * A0;
-        m2_0 = (u.src_lap == 0) ? A0 * u.dt : 0.0;
-    }
-    const float dt = (float)u.dt, scale = (float)u.scale, invv = (float)ml.invv, fixed2 = (float)ml.fixed2;
-    const float c4pi2 = (float)(4.0 * kPi * kPi);
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const int p = ml.t + ml.TT * e;
-        const float fv = (float)wrapk(p, ml.nv) * invv;
-        const float L = -c4pi2 * (fixed2 + fv * fv);
-        const float D = horner_f<DEG>(dc, L);
-        const float Ek = horner_f<DEG>(lc, L);
-        float Lp = 1.f;
-#pragma unroll
-        for (int q = 0; q < DEG; ++q)
-            if (q < u.src_lap) Lp *= L;
-        const float den = 1.f - dt * D;
-        float A0 = __builtin_amdgcn_rcpf(den);
-        A0 = A0 * (2.f - den * A0);  // one Newton step: correctly rounded to within an ulp
-        s1a[e] = dt * (Ek + D) * A0;
-        s2a[e] = A0 * dt * Lp * scale;
-    }
-}
-
 // MON: the energy monitor's partial sums are compiled in (a separate instantiation, so
 // that the common path does not carry their registers)
 template <typename T, int N, bool MON>
@@ -499,19 +435,11 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencil_update(PencilUpdateArgs<T>
     T s1a[E], s2a[E];
     double m1_0 = 1.0, m2_0 = 0.0;
     const MultLine ml{lm.t, TT, nv, invv, fixed2};
-    bool done = false;
-    if constexpr (sizeof(T) == 4) {
-        if (!u.filter && u.dense) {
-            if (u.dense == 2) update_multipliers_f32<E, 2>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
-            else update_multipliers_f32<E, 4>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
-            done = true;
-        }
-    }
-    if (!done) {
-        if (u.dense == 2) update_multipliers<T, E, 2>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
-        else if (u.dense == 4) update_multipliers<T, E, 4>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
-        else update_multipliers<T, E, 0>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
-    }
+    // the multipliers and their denominators 1 - dt D(k) are built in FP64 at both
+    // precisions; FP32-bulk rounds only the finished s1 = m1 - 1 and s2 to FP32
+    if (u.dense == 2) update_multipliers<T, E, 2>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
+    else if (u.dense == 4) update_multipliers<T, E, 4>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
+    else update_multipliers<T, E, 0>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
     // 2) explicit source: forward transform along this axis, scale by its multiplier
     cpx<T> gs[E];
     double g0 = 0.0;

@@ -155,6 +155,12 @@ void gpf_solver_destroy(gpf_solver* s);
 int gpf_solver_propagate(gpf_solver* s, int nsteps);
 /* Euler.Propagate / RK4.Propagate (pf/euler.go:55-59, pf/rk4.go:130-135) */
 int gpf_solver_stepper_propagate(gpf_solver* s, int nsteps);
+/* No reference counterpart (the reference plans nothing ahead of Step): does ahead of time
+ * what the first Step would, without advancing the fields or the time -- the autotuner
+ * (when gpf_options.autotune is set), the energy monitor's buffers for nsteps steps and
+ * the captured step graph that Propagate replays -- so that later steps only launch.
+ * *ms (may be NULL) receives the host milliseconds it took.  A no-op on the general path. */
+int gpf_solver_prepare(gpf_solver* s, int nsteps, double* ms);
 /* TimeStepper.GetTime (pf/euler.go:50-52, pf/rk4.go:139-141) */
 int gpf_solver_get_time(gpf_solver* s, double* t);
 /* copy field idx (Model.Fields order) to / from host complex128 (imag set to 0 / ignored) */
@@ -272,10 +278,13 @@ int gpf_device_copy_bandwidth(long bytes, int reps, int device, double* ms_per_c
 
 /* ---------------- pf.FourierTransform (pfutil.FFTWWrapper) ---------------- */
 
-/* pfutil.NewFFTW (pfutil/fftWrap.go:16-23) */
+/* pfutil.NewFFTW (pfutil/fftWrap.go:16-23).  The plan is host data; device buffers are
+ * taken on the first transform. */
 int gpf_fft_create(const int* dims, int ndim, int precision, int device, gpf_fft** out);
 void gpf_fft_destroy(gpf_fft* f);
-/* FFTWWrapper.FFT / IFFT (pfutil/fftWrap.go:26-39): in place on host complex128, unnormalised */
+/* FFTWWrapper.FFT / IFFT (pfutil/fftWrap.go:26-39): in place on host complex128, unnormalised.
+ * n is the number of complex values in data and must equal the plan's size (GPF_EINVAL
+ * otherwise; FFTWWrapper copies min(len) values, pfutil/fftWrap.go:27,35). */
 int gpf_fft_forward(gpf_fft* f, double* data, long n);
 int gpf_fft_inverse(gpf_fft* f, double* data, long n);
 /* FFTWWrapper.Freq (pfutil/fftWrap.go:57-74): out[ndim] */

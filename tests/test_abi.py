@@ -3,6 +3,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -60,3 +61,26 @@ def test_errors_are_reported_not_raised_as_crashes():
     assert ei.value.code == 2 and "leibniz" in str(ei.value)
     with pytest.raises(G.GpfError):
         G.Solver(m, [2, 2], 0.1)  # model not initialised
+
+
+def test_fft_rejects_a_buffer_of_the_wrong_length():
+    """FFTWWrapper.FFT/IFFT copy between the plan's buffer and the caller's slice
+    (pfutil/fftWrap.go:26-39); the C entry points take the slice's length and refuse any
+    length other than the plan's size before touching memory or the device."""
+    import gopf_amd as G
+    L = G.lib()
+    h = ctypes.c_void_p()
+    dims = (ctypes.c_int * 2)(8, 16)
+    assert L.gpf_fft_create(dims, 2, G.GPF_FP64, 0, ctypes.byref(h)) == 0
+    try:
+        short = (ctypes.c_double * (2 * 64))()
+        for fn in (L.gpf_fft_forward, L.gpf_fft_inverse):
+            assert fn(h, short, 64) == G.GPF_EINVAL
+            assert "does not match the plan" in L.gpf_last_error().decode()
+            assert fn(h, short, 129) == G.GPF_EINVAL
+        ft = G.FourierTransform([8, 16])
+        with pytest.raises(G.GpfError) as ei:
+            ft.FFT(np.zeros(100, dtype=np.complex128))
+        assert ei.value.code == G.GPF_EINVAL
+    finally:
+        L.gpf_fft_destroy(h)
