@@ -43,6 +43,15 @@ class _Options(C.Structure):
                 ("fused", C.c_int), ("device", C.c_int), ("autotune", C.c_int)]
 
 
+class _Reductions(C.Structure):
+    _fields_ = [("sum", C.c_double), ("sum_sq", C.c_double), ("min", C.c_double), ("max", C.c_double),
+                ("nonfinite", C.c_long)]
+
+
+def _red_dict(r: "_Reductions") -> Dict[str, float]:
+    return {"sum": r.sum, "sum_sq": r.sum_sq, "min": r.min, "max": r.max, "nonfinite": int(r.nonfinite)}
+
+
 class _Term(C.Structure):
     _fields_ = [("kind", C.c_int), ("coef_re", C.c_double), ("coef_im", C.c_double), ("brick", C.c_int),
                 ("has_lap", C.c_int), ("lap", C.c_int), ("user", C.c_int)]
@@ -100,6 +109,8 @@ def _declare(L) -> None:
         "gpf_solver_set_field": (C.c_int, [_vp, C.c_int, _dp, C.c_long]),
         "gpf_solver_energy": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_double, _dp, _dp]),
         "gpf_solver_monitor_energy": (C.c_int, [_vp, C.c_int, _dp, C.c_double]),
+        "gpf_solver_reduce": (C.c_int, [_vp, C.c_int, C.POINTER(_Reductions)]),
+        "gpf_reduce": (C.c_int, [_vp, C.c_long, C.c_int, C.c_int, _vp, C.POINTER(_Reductions)]),
         "gpf_solver_energy_series": (C.c_int, [_vp, _dp, _dp, C.c_long, _lp]),
         "gpf_solver_synchronize": (C.c_int, [_vp]),
         "gpf_solver_is_fused": (C.c_int, [_vp]),
@@ -507,6 +518,13 @@ class Solver:
         _check(lib().gpf_solver_energy(self._h, self._idx(f), p.size, _dptr(p), float(kappa), C.byref(e), C.byref(m)))
         return e.value, m.value
 
+    def Reduce(self, f: Union[int, str]) -> Dict[str, float]:
+        """FP64 sum, sum of squares, min, max and the count of non-finite values of a field
+        (the device form of a driver's NaN scan, examples/solverStability/main.go:58-67)."""
+        r = _Reductions()
+        _check(lib().gpf_solver_reduce(self._h, self._idx(f), C.byref(r)))
+        return _red_dict(r)
+
     def MonitorEnergy(self, poly: Sequence[float], kappa: float) -> None:
         p = np.ascontiguousarray(np.asarray(poly, dtype=np.float64))
         _check(lib().gpf_solver_monitor_energy(self._h, p.size, _dptr(p), float(kappa)))
@@ -739,6 +757,15 @@ class Batch:
         v = (C.c_int * 64)()
         k = lib().gpf_batch_tuning(self._h, v, 64)
         return [v[i] for i in range(min(k, 64))]
+
+
+def Reduce(data_ptr: int, n: int, precision: str = "fp64", device: int = 0, stream: int = 0) -> Dict[str, float]:
+    """gpf_reduce over n device values at data_ptr (float for "fp32", double for "fp64")."""
+    r = _Reductions()
+    prec = GPF_FP32 if precision in ("fp32", "float32") else GPF_FP64
+    _check(lib().gpf_reduce(_vp(int(data_ptr)), int(n), prec, int(device), _vp(int(stream)) if stream else None,
+                            C.byref(r)))
+    return _red_dict(r)
 
 
 def CopyBandwidth(nbytes: int, reps: int = 20, device: int = 0) -> float:

@@ -1,6 +1,8 @@
 """Build libgopf_amd.so (gfx950) in-tree with hipcc.
 
-Run ``python -m gopf_amd.build`` or call ``build()``; every call recompiles.
+Run ``python -m gopf_amd.build`` or call ``build()``; every call recompiles every source.
+``python -m gopf_amd.build --incremental`` recompiles only the sources whose object is
+older than the source or one of the headers it includes (from hipcc's dependency files).
 """
 from __future__ import annotations
 
@@ -13,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgopf_amd.so")
-SOURCES = ["spectral.hip", "pointwise.hip", "krylov.hip", "io.hip", "engine.hip", "batch.hip", "model.cpp"]
+SOURCES = ["spectral.hip", "pointwise.hip", "krylov.hip", "io.hip", "reduce.hip", "engine.hip", "batch.hip", "model.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = ["-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
@@ -22,22 +24,42 @@ FLAGS = ["-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
          "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
 
 
+def _obj(src: str) -> str:
+    return os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+
+
+def _stale(src: str) -> bool:
+    """True when the object is missing or older than a file it was built from."""
+    obj = _obj(src)
+    dep = obj + ".d"
+    if not (os.path.exists(obj) and os.path.exists(dep)):
+        return True
+    t = os.path.getmtime(obj)
+    with open(dep) as f:
+        text = f.read().replace("\\\n", " ")
+    files = text.split(":", 1)[1].split() if ":" in text else [os.path.join(CSRC, src)]
+    return any(not os.path.exists(p) or os.path.getmtime(p) > t for p in files)
+
+
 def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + FLAGS + ["-I", os.path.join(HERE, "..", "include"), "-c", os.path.join(CSRC, src), "-o", obj]
+    obj = _obj(src)
+    dep = ["-MMD", "-MF", obj + ".d"]
+    cmd = [HIPCC] + FLAGS + dep + ["-I", os.path.join(HERE, "..", "include"), "-c", os.path.join(CSRC, src), "-o", obj]
     if src.endswith(".cpp"):
-        cmd = [HIPCC, "-std=c++17", "-O2", "-fPIC", "-Wall", "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC, "-std=c++17", "-O2", "-fPIC", "-Wall"] + dep + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     return obj
 
 
-def build(verbose: bool = True) -> str:
+def build(verbose: bool = True, incremental: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    workers = min(len(SOURCES), int(os.environ.get("MAX_JOBS", "8")))
+    todo = [s for s in SOURCES if not incremental or _stale(s)]
+    workers = max(1, min(len(todo), int(os.environ.get("MAX_JOBS", "8"))))
     with ThreadPoolExecutor(max_workers=workers) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        list(ex.map(_compile, todo))
+    objs = [_obj(s) for s in SOURCES]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -45,10 +67,10 @@ def build(verbose: bool = True) -> str:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} ({len(todo)} of {len(SOURCES)} sources compiled)")
     return LIB
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv[1:])
     sys.exit(0)

@@ -40,6 +40,7 @@
 #include "common.h"
 #include "fft_core.h"
 #include "io.h"
+#include "reduce.h"
 #include "krylov.h"
 #include "model.h"
 #include "pointwise.h"
@@ -360,6 +361,8 @@ public:
     virtual void set_field(int idx, const double* in) = 0;
     virtual void energy(int idx, int npoly, const double* poly, double kappa, double* e, double* mass) = 0;
     virtual void monitor(int npoly, const double* poly, double kappa) = 0;
+    // FP64 sum, sum of squares, min, max and non-finite count of field idx (reduce.h order)
+    virtual void reduce(int idx, double* out) = 0;
     virtual long series(double* e, double* m, long cap) = 0;
     virtual bool fused() const = 0;
     virtual void enable_timing(bool on) = 0;
@@ -397,6 +400,7 @@ public:
     void set_field(int idx, const double* in) override;
     void energy(int idx, int npoly, const double* poly, double kappa, double* e, double* mass) override;
     void monitor(int npoly, const double* poly, double kappa) override;
+    void reduce(int idx, double* out) override;
     long series(double* e, double* m, long cap) override;
     bool fused() const override { return fp_.ok; }
     void enable_timing(bool on) override { timing_ = on; }
@@ -1078,6 +1082,26 @@ void Engine<T>::energy(int idx, int npoly, const double* poly, double kappa, dou
     HIPCHK(hipStreamSynchronize(st_));
     *e = h[0];
     *mass = h[1];
+}
+
+// FP64 reductions of field idx as the host reads it: the fused path's real field is the
+// row-pass inverse of the state plus its FP64 mean
+template <typename T>
+void Engine<T>::reduce(int idx, double* out) {
+    if (idx < 0 || idx >= nf_) throw Error(GPF_EINVAL, "field index out of range");
+    const T* c;
+    if (fp_.ok) {
+        T* tmp = rscratch(0);
+        c2r({S_.get()}, {tmp}, T(1));
+        c = tmp;
+    } else {
+        c = F_[idx].get();
+    }
+    Buf<double> part(kRedOut * reduce_blocks(g_.n)), res(kRedOut);
+    HIPCHK(launch_field_reduce<T>(c, g_.n, (fp_.ok && split_mean_) ? mean_.get() : nullptr, part.get(), res.get(),
+                                  st_));
+    HIPCHK(hipMemcpyAsync(out, res.get(), kRedOut * sizeof(double), hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
 }
 
 template <typename T>
@@ -2963,6 +2987,42 @@ int gpf_solver_energy(gpf_solver* s, int idx, int npoly, const double* poly, dou
         if (!s || !energy || !mass || (npoly && !poly)) throw Error(GPF_EINVAL, "bad argument");
         set_device(s->device);
         s->eng->energy(idx, npoly, poly, kappa, energy, mass);
+    });
+}
+
+static void to_reductions(const double* r, gpf_reductions* out) {
+    out->sum = r[kRedSum];
+    out->sum_sq = r[kRedSumSq];
+    out->min = r[kRedMin];
+    out->max = r[kRedMax];
+    out->nonfinite = (long)r[kRedNonFinite];
+}
+
+int gpf_solver_reduce(gpf_solver* s, int idx, gpf_reductions* out) {
+    return guarded([&] {
+        if (!s || !out) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        double r[kRedOut];
+        s->eng->reduce(idx, r);
+        to_reductions(r, out);
+    });
+}
+
+int gpf_reduce(const void* data, long n, int precision, int device, void* stream, gpf_reductions* out) {
+    return guarded([&] {
+        if ((!data && n > 0) || n < 0 || !out) throw Error(GPF_EINVAL, "bad argument");
+        if (precision != GPF_FP32 && precision != GPF_FP64) throw Error(GPF_EINVAL, "unknown precision");
+        set_device(device);
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        Buf<double> part(kRedOut * reduce_blocks(n)), res(kRedOut);
+        if (precision == GPF_FP32)
+            HIPCHK(launch_field_reduce<float>(static_cast<const float*>(data), n, nullptr, part.get(), res.get(), st));
+        else
+            HIPCHK(launch_field_reduce<double>(static_cast<const double*>(data), n, nullptr, part.get(), res.get(), st));
+        double r[kRedOut];
+        HIPCHK(hipMemcpyAsync(r, res.get(), sizeof(r), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        to_reductions(r, out);
     });
 }
 

@@ -170,6 +170,25 @@ int gpf_solver_set_field(gpf_solver* s, int idx, const double* in, long n);
  * idx, in FP64 (compensated).  No reference counterpart: a monitor (cf. pf/monitor.go). */
 int gpf_solver_energy(gpf_solver* s, int idx, int npoly, const double* poly, double kappa,
                       double* energy, double* mass);
+/* FP64 reductions of a real field (Neumaier-compensated sums; min and max over the finite
+ * values, +inf / -inf when there is none; the count of NaN and +-inf values).  No reference
+ * counterpart: it replaces the host scan for NaN that a gopf driver runs over
+ * m.Fields[i].Data to stop an unstable run (examples/solverStability/main.go:58-67). */
+typedef struct {
+    double sum;        /* sum_x c(x): the field's mass */
+    double sum_sq;     /* sum_x c(x)^2 */
+    double min;
+    double max;
+    long nonfinite;    /* values that are NaN or +-inf (left out of the other four) */
+} gpf_reductions;
+
+/* the reductions of field idx (Model.Fields order) as gpf_solver_get_field would return it */
+int gpf_solver_reduce(gpf_solver* s, int idx, gpf_reductions* out);
+/* the reductions of n real values of the given precision (GPF_FP32: float, GPF_FP64:
+ * double) in device memory of `device`, computed on `stream` (a hipStream_t; NULL is the
+ * default stream), which is synchronised before return.  Allocates its partial sums. */
+int gpf_reduce(const void* data, long n, int precision, int device, void* stream, gpf_reductions* out);
+
 /* record the energy of field 0 before every step into a device series (fused path) */
 int gpf_solver_monitor_energy(gpf_solver* s, int npoly, const double* poly, double kappa);
 int gpf_solver_energy_series(gpf_solver* s, double* energy, double* mass, long cap, long* n);
