@@ -217,27 +217,45 @@ __device__ __forceinline__ void dftR(cpx<T>* a) {
 __device__ __forceinline__ int lds_pad(int p) { return p + (p >> 4); }
 constexpr int lds_line(int n) { return n + n / 16 + 1; }
 
+// twiddles of stage S: butterfly b, leg m (1 <= m < R) takes w^(k m TWSTEP), k = j mod NS
+template <int N, int V, int S>
+struct StageTw {
+    static constexpr int R = Sched<N, V>::R[S];
+    static constexpr int NB = Sched<N, V>::E / R;
+    static constexpr int NS = Span<N, V, S>::v;
+    static constexpr int TWSTEP = N / (NS * R);
+    static constexpr int count = (NS > 1) ? NB * (R - 1) : 1;
+};
+
+template <typename T, int N, int V, int S>
+__device__ __forceinline__ void load_stage_tw(cpx<T>* w, int t, const cpx<T>* __restrict__ tw) {
+    using ST = StageTw<N, V, S>;
+    if constexpr (ST::NS > 1) {
+        constexpr int TT = Sched<N, V>::T;
+#pragma unroll
+        for (int b = 0; b < ST::NB; ++b) {
+            const int k = (t + b * TT) % ST::NS;
+#pragma unroll
+            for (int m = 1; m < ST::R; ++m) w[b * (ST::R - 1) + m - 1] = tw[(k * m * ST::TWSTEP) % N];
+        }
+    }
+}
+
+// stage S's butterflies on registers a, with the stage's twiddles w (load_stage_tw)
 template <typename T, int N, int V, int S, int DIR>
-__device__ __forceinline__ void stage_compute(cpx<T>* a, int t, const cpx<T>* __restrict__ tw) {
-    constexpr int R = Sched<N, V>::R[S];
-    constexpr int E = Sched<N, V>::E;
-    constexpr int TT = Sched<N, V>::T;
-    constexpr int NB = E / R;
-    constexpr int NS = Span<N, V, S>::v;
-    constexpr int TWSTEP = N / (NS * R);
+__device__ __forceinline__ void stage_compute(cpx<T>* a, const cpx<T>* w) {
+    using ST = StageTw<N, V, S>;
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        if constexpr (NS > 1) {
-            const int j = t + b * TT;
-            const int k = j % NS;
+    for (int b = 0; b < ST::NB; ++b) {
+        if constexpr (ST::NS > 1) {
 #pragma unroll
-            for (int m = 1; m < R; ++m) {
-                cpx<T> w = tw[(k * m * TWSTEP) % N];
-                if (DIR > 0) w.y = -w.y;
-                a[b * R + m] = mul(a[b * R + m], w);
+            for (int m = 1; m < ST::R; ++m) {
+                cpx<T> x = w[b * (ST::R - 1) + m - 1];
+                if (DIR > 0) x.y = -x.y;
+                a[b * ST::R + m] = mul(a[b * ST::R + m], x);
             }
         }
-        dftR<R, DIR>(&a[b * R]);
+        dftR<ST::R, DIR>(&a[b * ST::R]);
     }
 }
 
@@ -252,6 +270,10 @@ __device__ __forceinline__ void exchange_and_compute(cpx<T>* a, cpx<T>* lds, int
         constexpr int NSP = Span<N, V, S - 1>::v;
         constexpr int R = Sched<N, V>::R[S];
         constexpr int NB = E / R;
+        // the stage's twiddles are requested before the exchange, so that their latency
+        // overlaps the LDS round trip and the barriers
+        cpx<T> w[StageTw<N, V, S>::count];
+        load_stage_tw<T, N, V, S>(w, t, tw);
 #pragma unroll
         for (int b = 0; b < NBP; ++b) {
             const int j = t + b * TT;
@@ -267,7 +289,7 @@ __device__ __forceinline__ void exchange_and_compute(cpx<T>* a, cpx<T>* lds, int
             for (int m = 0; m < R; ++m) a[b * R + m] = lds[lds_pad(j + m * (N / R)) * es];
         }
         __syncthreads();
-        stage_compute<T, N, V, S, DIR>(a, t, tw);
+        stage_compute<T, N, V, S, DIR>(a, w);
         exchange_and_compute<T, N, V, S + 1, DIR>(a, lds, es, t, tw);
     }
 }
@@ -288,7 +310,11 @@ __device__ __forceinline__ void fft_canon(cpx<T>* v, cpx<T>* lds, int es, int t,
     for (int b = 0; b < NB0; ++b)
 #pragma unroll
         for (int m = 0; m < R0; ++m) a[b * R0 + m] = v[b + m * NB0];
-    stage_compute<T, N, V, 0, DIR>(a, t, tw);
+    {
+        cpx<T> w[StageTw<N, V, 0>::count];
+        load_stage_tw<T, N, V, 0>(w, t, tw);
+        stage_compute<T, N, V, 0, DIR>(a, w);
+    }
     exchange_and_compute<T, N, V, 1, DIR>(a, lds, es, t, tw);
 #pragma unroll
     for (int b = 0; b < NBL; ++b)

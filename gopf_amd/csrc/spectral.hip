@@ -524,14 +524,26 @@ __device__ __forceinline__ void split_store_rows(const cpx<T>* v, cpx<T>* L, int
     __syncthreads();
     cpx<T>* a1 = gmain + r1 * nh + t;
     constexpr int J = (E * TTc / 2) / TTc;  // = E / 2 k values per thread below nh
+    cpx<T> A[J], B[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
         const int k = t + TTc * j;
         const cpx<T> z = L[lds_pad(k)];
         const cpx<T> zm = L[lds_pad((n - k) & (n - 1))];
         // A = (z + conj zm) / 2,  B = (z - conj zm) / (2 i)
-        if (v1) a1[TTc * j] = mk<T>(T(0.5) * (z.x + zm.x), T(0.5) * (z.y - zm.y));
-        if (v2) a1[TTc * j + nh] = mk<T>(T(0.5) * (z.y + zm.y), T(-0.5) * (z.x - zm.x));
+        A[j] = mk<T>(T(0.5) * (z.x + zm.x), T(0.5) * (z.y - zm.y));
+        B[j] = mk<T>(T(0.5) * (z.y + zm.y), T(-0.5) * (z.x - zm.x));
+    }
+    if (v1 && v2) {
+        // every line but possibly the last: one branch for all stores
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            a1[TTc * j] = A[j];
+            a1[TTc * j + nh] = B[j];
+        }
+    } else if (v1) {
+#pragma unroll
+        for (int j = 0; j < J; ++j) a1[TTc * j] = A[j];
     }
     if (t == 0) {
         const cpx<T> z = L[lds_pad(nh)];
@@ -601,29 +613,45 @@ __device__ __forceinline__ void load_herm_pair(cpx<T>* v, const cpx<T>* m, const
 // positions are loaded and staged in the line's LDS (row r1 at [0, nh], row r1 + 1 at
 // [nh + 1, n + 1]), and the mirrored positions are read back from LDS.  The line's LDS
 // region (lds_line(n) >= n + 2 elements for n >= 16) is free again on return.
+// Every lane issues all of its loads before it uses any of them: a lane whose position is
+// mirrored, or whose row is past the end, loads from a valid address of a valid row and
+// discards the value, so no load sits behind a branch or waits for the one before it.
 template <typename T, int E, int TTc>
 __device__ __forceinline__ void load_herm_pair_lds(cpx<T>* v, const cpx<T>* m, const cpx<T>* q, long r1, int n, int t,
                                                    bool v1, bool v2, cpx<T>* L) {
     const int nh = n / 2;
-    const cpx<T>* a1 = m + r1 * nh + t;
+    const long ra = v1 ? r1 : 0, rb = v2 ? r1 + 1 : ra;
+    const cpx<T>* pa = m + ra * nh;
+    const cpx<T>* pb = m + rb * nh;
+    const cpx<T> zero = mk<T>(T(0), T(0));
+    cpx<T> A[E], B[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int lo = TTc * e, hi = lo + TTc - 1;
+        if (lo > nh) continue;  // mirrored for every lane
         const int p = t + lo;
-        if (lo > nh) continue;  // mirrored: filled below
-        cpx<T> A = mk<T>(T(0), T(0)), B = A;
-        if (hi < nh || p < nh) {
-            if (v1) A = a1[lo];
-            if (v2) B = a1[lo + nh];
-        } else if (p == nh) {
-            if (v1) A = q[r1];
-            if (v2) B = q[r1 + 1];
+        if (hi < nh) {
+            A[e] = pa[p];
+            B[e] = pb[p];
         } else {
-            continue;
+            // the element that straddles nh: direct below it, the x-Nyquist value at it
+            const cpx<T>* sa = (p < nh) ? pa + p : (p == nh ? q + ra : pa);
+            const cpx<T>* sb = (p < nh) ? pb + p : (p == nh ? q + rb : pb);
+            A[e] = *sa;
+            B[e] = *sb;
         }
-        L[p] = A;
-        L[nh + 1 + p] = B;
-        v[e] = mk<T>(A.x - B.y, A.y + B.x);
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int lo = TTc * e, hi = lo + TTc - 1;
+        if (lo > nh) continue;
+        const int p = t + lo;
+        const cpx<T> a = v1 ? A[e] : zero, b = v2 ? B[e] : zero;
+        if (hi < nh || p <= nh) {
+            L[p] = a;
+            L[nh + 1 + p] = b;
+            v[e] = mk<T>(a.x - b.y, a.y + b.x);
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -632,8 +660,8 @@ __device__ __forceinline__ void load_herm_pair_lds(cpx<T>* v, const cpx<T>* m, c
         const int p = t + lo;
         if (hi < nh) continue;
         if (lo > nh || p > nh) {
-            const cpx<T> A = conjg(L[n - p]), B = conjg(L[nh + 1 + n - p]);
-            v[e] = mk<T>(A.x - B.y, A.y + B.x);
+            const cpx<T> a = conjg(L[n - p]), b = conjg(L[nh + 1 + n - p]);
+            v[e] = mk<T>(a.x - b.y, a.y + b.x);
         }
     }
     __syncthreads();
