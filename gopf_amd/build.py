@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgopf_amd.so")
-SOURCES = ["spectral.hip", "pointwise.hip", "krylov.hip", "io.hip", "reduce.hip", "engine.hip", "batch.hip", "model.cpp"]
+SOURCES = ["spectral_rows.hip", "spectral_update.hip", "spectral_pencils.hip", "spectral.hip", "pointwise.hip", "krylov.hip", "io.hip", "reduce.hip", "engine.hip", "batch.hip", "model.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = ["-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",

@@ -1,0 +1,12 @@
+// spectral_rows.hip -- instantiations of k_rows_fused (every length, schedule, with and without the monitor).
+#include "spectral_kernels.h"
+
+namespace gpf {
+
+#define GPF_INST(T) \
+    template hipError_t launch_rows_fused<T>(const RowFusedArgs<T>&, hipStream_t);
+GPF_INST(float)
+GPF_INST(double)
+#undef GPF_INST
+
+}  // namespace gpf

@@ -1,0 +1,12 @@
+// spectral_update.hip -- instantiations of k_pencil_update (every length, with and without the monitor).
+#include "spectral_kernels.h"
+
+namespace gpf {
+
+#define GPF_INST(T) \
+    template hipError_t launch_pencil_update<T>(const PencilUpdateArgs<T>&, int, hipStream_t);
+GPF_INST(float)
+GPF_INST(double)
+#undef GPF_INST
+
+}  // namespace gpf
