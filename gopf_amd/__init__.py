@@ -111,6 +111,7 @@ def _declare(L) -> None:
         "gpf_solver_monitor_energy": (C.c_int, [_vp, C.c_int, _dp, C.c_double]),
         "gpf_solver_reduce": (C.c_int, [_vp, C.c_int, C.POINTER(_Reductions)]),
         "gpf_reduce": (C.c_int, [_vp, C.c_long, C.c_int, C.c_int, _vp, C.POINTER(_Reductions)]),
+        "gpf_reduce_noalloc": (C.c_int, [_vp, C.c_long, C.c_int, C.c_int, _vp, _vp]),
         "gpf_solver_energy_series": (C.c_int, [_vp, _dp, _dp, C.c_long, _lp]),
         "gpf_solver_synchronize": (C.c_int, [_vp]),
         "gpf_solver_is_fused": (C.c_int, [_vp]),
@@ -766,6 +767,14 @@ def Reduce(data_ptr: int, n: int, precision: str = "fp64", device: int = 0, stre
     _check(lib().gpf_reduce(_vp(int(data_ptr)), int(n), prec, int(device), _vp(int(stream)) if stream else None,
                             C.byref(r)))
     return _red_dict(r)
+
+
+def ReduceNoAlloc(data_ptr: int, n: int, precision: str, device: int, stream: int, result_ptr: int) -> None:
+    """gpf_reduce_noalloc: the result's five doubles are written to device memory at
+    result_ptr on `stream`, which is not synchronised."""
+    prec = GPF_FP32 if precision in ("fp32", "float32") else GPF_FP64
+    _check(lib().gpf_reduce_noalloc(_vp(int(data_ptr)), int(n), prec, int(device),
+                                    _vp(int(stream)) if stream else None, _vp(int(result_ptr))))
 
 
 def CopyBandwidth(nbytes: int, reps: int = 20, device: int = 0) -> float:

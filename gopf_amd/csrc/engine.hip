@@ -3026,6 +3026,20 @@ int gpf_reduce(const void* data, long n, int precision, int device, void* stream
     });
 }
 
+int gpf_reduce_noalloc(const void* data, long n, int precision, int device, void* stream, double* result) {
+    return guarded([&] {
+        if ((!data && n > 0) || n < 0 || !result) throw Error(GPF_EINVAL, "bad argument");
+        if (precision != GPF_FP32 && precision != GPF_FP64) throw Error(GPF_EINVAL, "unknown precision");
+        if (n > kReduceStaticMaxN)
+            throw Error(GPF_EINVAL, "gpf_reduce_noalloc: n = " + std::to_string(n) + " exceeds its maximum of " +
+                                        std::to_string(kReduceStaticMaxN) + " values (use gpf_reduce)");
+        set_device(device);
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        if (precision == GPF_FP32) field_reduce_static<float>(static_cast<const float*>(data), n, result, st);
+        else field_reduce_static<double>(static_cast<const double*>(data), n, result, st);
+    });
+}
+
 int gpf_solver_monitor_energy(gpf_solver* s, int npoly, const double* poly, double kappa) {
     return guarded([&] {
         if (!s || (npoly && !poly)) throw Error(GPF_EINVAL, "bad argument");

@@ -22,6 +22,9 @@ enum ReduceSlot : int {
 // partials; one block then combines the partials in block order.  The partition depends on
 // n only, so every entry point that reduces the same array gets the same bits.
 constexpr long kReduceChunk = 65536;
+// partial blocks the no-allocation entry point keeps in its static device array
+constexpr long kReduceStaticBlocks = 4096;
+constexpr long kReduceStaticMaxN = kReduceChunk * kReduceStaticBlocks;  // 2^28 values
 
 inline long reduce_blocks(long n) { return n <= 0 ? 1 : (n + kReduceChunk - 1) / kReduceChunk; }
 
@@ -29,5 +32,12 @@ inline long reduce_blocks(long n) { return n <= 0 ? 1 : (n + kReduceChunk - 1) /
 // kRedOut * reduce_blocks(n) doubles; out (device) receives kRedOut doubles
 template <typename T>
 hipError_t launch_field_reduce(const T* x, long n, const double* mean, double* partial, double* out, hipStream_t s);
+
+// The same reduction with its partials in the library's static device array of the current
+// device (no allocation); n <= kReduceStaticMaxN.  out is device memory of kRedOut doubles,
+// written on stream s without a host synchronisation.  Calls that overlap on one device
+// share the array and race.
+template <typename T>
+void field_reduce_static(const T* x, long n, double* out, hipStream_t s);
 
 }  // namespace gpf

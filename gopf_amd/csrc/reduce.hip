@@ -4,15 +4,24 @@
 // into kRedOut partials, k_reduce_final combines the partials in block order.  Sums are
 // Neumaier-compensated FP64 whatever the array's type; the order of every addition is
 // fixed by n, so results are reproducible bit for bit.
+//
+// The no-allocation path keeps the partials in `scratch`, a static device array; each
+// device holds its own copy, whose address is looked up once per device.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 
 #include "common.h"
 #include "gpf.h"
 #include "reduce.h"
 
 namespace gpf {
+
+// The no-allocation path's partials: kReduceStaticBlocks blocks of kRedOut doubles
+// (160 KiB per device, held until the process ends).  In host code this name is not a
+// device address; only scratch_ptr()'s result is passed to kernels.
+static __device__ __align__(256) double scratch[kReduceStaticBlocks * kRedOut];
 
 namespace {
 
@@ -116,6 +125,24 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce_final(const double* __re
     }
 }
 
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device, looked up once per device
+double* scratch_ptr() {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw Error(GPF_EINVAL, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
 }  // namespace
 
 template <typename T>
@@ -126,7 +153,21 @@ hipError_t launch_field_reduce(const T* x, long n, const double* mean, double* p
     return hipGetLastError();
 }
 
+template <typename T>
+void field_reduce_static(const T* x, long n, double* out, hipStream_t s) {
+    if (n < 0) throw Error(GPF_EINVAL, "negative length");
+    if (n > kReduceStaticMaxN)
+        throw Error(GPF_EINVAL, "gpf_reduce_noalloc: n = " + std::to_string(n) + " exceeds its maximum of " +
+                                    std::to_string(kReduceStaticMaxN) + " values (use gpf_reduce)");
+    double* part = scratch_ptr();
+    // every partial slot the final pass reads is written by this call's first pass, so
+    // nothing left by an earlier call needs clearing
+    HIPCHK(launch_field_reduce<T>(x, n, nullptr, part, out, s));
+}
+
 template hipError_t launch_field_reduce<float>(const float*, long, const double*, double*, double*, hipStream_t);
 template hipError_t launch_field_reduce<double>(const double*, long, const double*, double*, double*, hipStream_t);
+template void field_reduce_static<float>(const float*, long, double*, hipStream_t);
+template void field_reduce_static<double>(const double*, long, double*, hipStream_t);
 
 }  // namespace gpf

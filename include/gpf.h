@@ -188,6 +188,14 @@ int gpf_solver_reduce(gpf_solver* s, int idx, gpf_reductions* out);
  * double) in device memory of `device`, computed on `stream` (a hipStream_t; NULL is the
  * default stream), which is synchronised before return.  Allocates its partial sums. */
 int gpf_reduce(const void* data, long n, int precision, int device, void* stream, gpf_reductions* out);
+/* No reference counterpart.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops): gpf_reduce with its partial sums in a static
+ * device array of the library (160 KiB per device, held until the process ends) and the
+ * result left in device memory: result[0..4] = sum, sum_sq, min, max, nonfinite (as a
+ * double), written on `stream` without synchronising it.  It calls no allocating routine.
+ * n may be at most 2^28 (268435456) values; larger n fails with GPF_EINVAL.  Every call on
+ * one device shares the array: calls that overlap on one device race. */
+int gpf_reduce_noalloc(const void* data, long n, int precision, int device, void* stream, double* result);
 
 /* record the energy of field 0 before every step into a device series (fused path) */
 int gpf_solver_monitor_energy(gpf_solver* s, int npoly, const double* poly, double kappa);
