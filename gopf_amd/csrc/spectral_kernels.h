@@ -527,8 +527,39 @@ __device__ __forceinline__ void split_store_rows(const cpx<T>* v, cpx<T>* L, int
 #pragma unroll
     for (int e = 0; e < E; ++e) L[lds_pad(t + TTc * e)] = v[e];
     __syncthreads();
-    cpx<T>* a1 = gmain + r1 * nh + t;
     constexpr int J = (E * TTc / 2) / TTc;  // = E / 2 k values per thread below nh
+    if constexpr (sizeof(T) == 4 && J % 2 == 0) {
+        // FP32: lane t stores k = 2t, 2t + 1 (+ 2 TTc j) of each row as one 16-byte value
+        float4* fa = reinterpret_cast<float4*>(gmain + r1 * nh) + t;
+        float4* fb = reinterpret_cast<float4*>(gmain + (r1 + 1) * nh) + t;
+        float4 A4[J / 2], B4[J / 2];
+#pragma unroll
+        for (int j = 0; j < J / 2; ++j) {
+            const int k = 2 * t + 2 * TTc * j;
+            const cpx<T> z0 = L[lds_pad(k)], z1 = L[lds_pad(k + 1)];
+            const cpx<T> m0 = L[lds_pad((n - k) & (n - 1))], m1 = L[lds_pad((n - k - 1) & (n - 1))];
+            A4[j] = make_float4(0.5f * (z0.x + m0.x), 0.5f * (z0.y - m0.y), 0.5f * (z1.x + m1.x), 0.5f * (z1.y - m1.y));
+            B4[j] = make_float4(0.5f * (z0.y + m0.y), -0.5f * (z0.x - m0.x), 0.5f * (z1.y + m1.y), -0.5f * (z1.x - m1.x));
+        }
+        if (v1 && v2) {
+#pragma unroll
+            for (int j = 0; j < J / 2; ++j) {
+                fa[TTc * j] = A4[j];
+                fb[TTc * j] = B4[j];
+            }
+        } else if (v1) {
+#pragma unroll
+            for (int j = 0; j < J / 2; ++j) fa[TTc * j] = A4[j];
+        }
+        if (t == 0) {
+            const cpx<T> z = L[lds_pad(nh)];
+            if (v1) gnyq[r1] = mk<T>(z.x, T(0));
+            if (v2) gnyq[r1 + 1] = mk<T>(z.y, T(0));
+        }
+        __syncthreads();
+        return;
+    }
+    cpx<T>* a1 = gmain + r1 * nh + t;
     cpx<T> A[J], B[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
@@ -626,6 +657,50 @@ __device__ __forceinline__ void load_herm_pair_lds(cpx<T>* v, const cpx<T>* m, c
                                                    bool v1, bool v2, cpx<T>* L) {
     const int nh = n / 2;
     const long ra = v1 ? r1 : 0, rb = v2 ? r1 + 1 : ra;
+    if constexpr (sizeof(T) == 4 && (TTc * E) % (4 * TTc) == 0) {
+        // FP32: lane t loads positions 2t, 2t + 1 (+ 2 TTc e) of each row as one 16-byte
+        // value; every canonical element then comes from LDS
+        constexpr int NL = (TTc * E / 2) / (2 * TTc);  // 16-byte loads per row below nh
+        const float4* fa = reinterpret_cast<const float4*>(m + ra * nh) + t;
+        const float4* fb = reinterpret_cast<const float4*>(m + rb * nh) + t;
+        float4 A4[NL], B4[NL];
+#pragma unroll
+        for (int e = 0; e < NL; ++e) {
+            A4[e] = fa[TTc * e];
+            B4[e] = fb[TTc * e];
+        }
+        const cpx<T> qa = q[ra], qb = q[rb];
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int e = 0; e < NL; ++e) {
+            const int p = 2 * t + 2 * TTc * e;
+            const float4 a4 = v1 ? A4[e] : z4, b4 = v2 ? B4[e] : z4;
+            L[p] = mk<T>(a4.x, a4.y);
+            L[p + 1] = mk<T>(a4.z, a4.w);
+            L[nh + 1 + p] = mk<T>(b4.x, b4.y);
+            L[nh + 2 + p] = mk<T>(b4.z, b4.w);
+        }
+        if (t == 0) {
+            L[nh] = v1 ? qa : mk<T>(T(0), T(0));
+            L[n + 1] = v2 ? qb : mk<T>(T(0), T(0));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int p = t + TTc * e;
+            cpx<T> a, b;
+            if (p <= nh) {
+                a = L[p];
+                b = L[nh + 1 + p];
+            } else {
+                a = conjg(L[n - p]);
+                b = conjg(L[nh + 1 + n - p]);
+            }
+            v[e] = mk<T>(a.x - b.y, a.y + b.x);
+        }
+        __syncthreads();
+        return;
+    }
     const cpx<T>* pa = m + ra * nh;
     const cpx<T>* pb = m + rb * nh;
     const cpx<T> zero = mk<T>(T(0), T(0));
@@ -930,17 +1005,22 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
 #pragma unroll
         for (int q = 0; q < kMaxPoly; ++q) gp[q] = (T)gd[q];
     }
+    // the block's copy of the twiddle table: the transforms' per-stage twiddles are then
+    // LDS reads rather than gathers from global memory
+    cpx<T>* twl = lds + lds_elems(n, a.TW);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) twl[i] = a.tw[i];
+    __syncthreads();
     cpx<T> v[E];
     if constexpr (N >= 16)
         load_herm_pair_lds<T, E, LineShape<N, V>::T>(v, a.s_main, a.s_nyq, r1, n, lm.t, v1, v2, lds + lm.off);
     else
         load_herm_pair<T, E, (N > 0 ? LineShape<N, V>::T : 0)>(v, a.s_main, a.s_nyq, r1, n, lm.t, TT, v1, v2);
-    line_fft<T, N, +1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
+    line_fft<T, N, +1, V>(v, lds + lm.off, 1, n, lm.t, twl);
     KSum sb, sm;
     if (a.npoly <= 4) rows_pointwise<T, E, 4, MON>(v, gp, a, r1, n, lm.t, TT, v1, v2, mu, sb, sm);
     else rows_pointwise<T, E, kMaxPoly, MON>(v, gp, a, r1, n, lm.t, TT, v1, v2, mu, sb, sm);
     if (MON) {
-        double* red = reinterpret_cast<double*>(lds + lds_elems(n, a.TW));
+        double* red = reinterpret_cast<double*>(lds + lds_elems(n, a.TW) + n);
         const double b = block_sum(sb.get(), red);
         const double ms = block_sum(sm.get(), red);
         if (threadIdx.x == 0) {
@@ -948,7 +1028,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
             a.mass_partial[blockIdx.x] = ms;
         }
     }
-    line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
+    line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, twl);
     if constexpr (N >= 16 && (N & (N - 1)) == 0 && E % 2 == 0)
         split_store_rows<T, E, LineShape<N, V>::T>(v, lds + lm.off, n, lm.t, r1, v1, v2, a.g_main, a.g_nyq);
     else
@@ -1080,7 +1160,8 @@ template <typename T>
 hipError_t launch_rows_fused(const RowFusedArgs<T>& a, hipStream_t s) {
     const long blocks = rows_fused_blocks(a.rows, a.TW);
     const int thr = a.TW * threads_per_line(a.nx, a.variant);
-    size_t smem = lds_elems(a.nx, a.TW) * sizeof(cpx<T>);
+    // exchange lines, the block's copy of the twiddle table, the monitor's partial sums
+    size_t smem = (lds_elems(a.nx, a.TW) + a.nx) * sizeof(cpx<T>);
     if (a.bulk_partial) smem += (size_t)thr * sizeof(double);
     return dispatch_size(a.nx, [&](auto NC) -> hipError_t {
         constexpr int NN = decltype(NC)::value;
