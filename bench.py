@@ -45,29 +45,48 @@ def pass_bytes(name: str, nx: int, ny: int, nz: int, s: int) -> float:
     return {"rows_fused": 2, "pencils_fwd": 2, "pencil_update": 4, "pencils_inv": 2}[name] * arr
 
 
-def pmc_traffic(workload: str, kernel: str):
+TUNING_FILE = os.path.join(ROOT, "gopf_amd", "tuning", "gfx950.json")
+
+
+def pmc_traffic(workload: str, kernel: str, layout: dict):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
     (profiles/pmc_traffic.json: FETCH_SIZE scaled by the copy-kernel calibration, plus
-    WRITE_SIZE), or None when the summary does not cover this workload."""
+    WRITE_SIZE), and None with the reason when the summary does not cover this workload or
+    was measured under another launch layout than `layout`."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
-        return float(d["workloads"][workload]["kernels"][kernel]["hbm_bytes"])
+        w = d["workloads"][workload]
+        if w.get("layout") != layout:
+            return None, "the committed counters were measured under another launch layout"
+        return float(w["kernels"][kernel]["hbm_bytes"]), None
     except (OSError, KeyError, ValueError, TypeError):
+        return None, "no committed counter summary for this workload and kernel"
+
+
+def stored_layout(workload: str, precision: str):
+    """The launch layout persisted for this workload and precision (autotuned once on an
+    MI355X, then reused, so that every run and every profile uses the same kernels)."""
+    try:
+        with open(TUNING_FILE) as f:
+            return json.load(f)[workload][precision]
+    except (OSError, KeyError, ValueError):
         return None
 
 
-def build_solver(G, W, n, dims, c0, precision, dt):
+def build_solver(G, W, n, dims, c0, precision, dt, layout=None):
     m = G.Model()
     m.AddField("conc", c0)
     for k, v in W.ch_scalars(1.0, 1.0).items():
         m.AddScalar(k, v)
     m.AddEquation(W.CH_EQUATION)
     m.Init()
-    s = G.Solver(m, dims, dt, precision=precision, autotune=True)
+    s = G.Solver(m, dims, dt, precision=precision, autotune=layout is None)
     if not s.fused:
         raise RuntimeError("the Cahn-Hilliard model did not compile to the fused path")
+    if layout is not None:
+        s.SetTuning(layout)
     return s
 
 
@@ -544,6 +563,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle timing")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--err-steps", type=int, default=1000, help="steps of the FP32-vs-FP64 accuracy run")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the autotuner instead of applying the stored launch layout")
+    ap.add_argument("--save-tuning", default=None,
+                    help="write the layouts this run used to this JSON file (stored-layout format)")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "ie", "io"],
                     help="c2 (default): 256^3 CH headline; c1 128^2 CH FP64 with energy series; c3 batched "
                          "sweep; c4 192^3 elastic RK4; c5 512^2 adjoint; ie 128^2 CH ImplicitEuler; io snapshots")
@@ -571,12 +594,15 @@ def main() -> int:
     dims = [N, N, N]
     n = N ** 3
     c0 = W.ch_initial(1, n)
-    copy_bytes = 1 << 30
-    copy_gbs = G.CopyBandwidth(copy_bytes, reps=10)
-    print(f"# device copy of {copy_bytes} B: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    workload = f"cahn_hilliard_3d_{N}"
+    probe_bytes = 1 << 30
+    bw = {k: G.DeviceBandwidth(probe_bytes, k) for k in ("copy", "read", "write")}
+    print(f"# device bandwidth over {probe_bytes} B: copy {bw['copy']:.0f}, read {bw['read']:.0f}, "
+          f"write {bw['write']:.0f} GB/s", file=sys.stderr, flush=True)
+    lay = {p: (None if args.autotune else stored_layout(workload, p)) for p in ("fp32", "fp64")}
     print(f"# building FP32-bulk solver {dims}", file=sys.stderr, flush=True)
     timer = StreamTimer()
-    s32 = build_solver(G, W, n, dims, c0, "fp32", args.dt)
+    s32 = build_solver(G, W, n, dims, c0, "fp32", args.dt, lay["fp32"])
     prep32 = s32.Prepare(args.steps)    # autotuner and step-graph capture, outside the timed region
     s32.StepperPropagate(args.warmup)
     el32, dev32 = timer.time(s32, args.steps)
@@ -591,8 +617,9 @@ def main() -> int:
           f"prepare {prep32:.1f} ms; passes {passes}", file=sys.stderr, flush=True)
     res = {}
     if not args.no_fp64:
-        s64 = build_solver(G, W, n, dims, c0, "fp64", args.dt)
+        s64 = build_solver(G, W, n, dims, c0, "fp64", args.dt, lay["fp64"])
         prep64 = s64.Prepare(args.steps)
+        tuning64 = s64.Tuning()
         s64.StepperPropagate(args.warmup)
         el64, dev64 = timer.time(s64, args.steps)
         del s64
@@ -600,12 +627,12 @@ def main() -> int:
               f"prepare {prep64:.1f} ms", file=sys.stderr, flush=True)
         # accuracy at the stated step count, from the same initial field
         del s32
-        e32 = build_solver(G, W, n, dims, c0, "fp32", args.dt)
+        e32 = build_solver(G, W, n, dims, c0, "fp32", args.dt, tuning)
         e32.StepperPropagate(args.err_steps)
         f32 = e32.Field(0).real
         _, m32 = e32.Energy(0, W.CH_BULK_POLY, 1.0)
         del e32
-        e64 = build_solver(G, W, n, dims, c0, "fp64", args.dt)
+        e64 = build_solver(G, W, n, dims, c0, "fp64", args.dt, tuning64)
         e64.StepperPropagate(args.err_steps)
         f64 = e64.Field(0).real
         _, m64 = e64.Energy(0, W.CH_BULK_POLY, 1.0)
@@ -630,14 +657,20 @@ def main() -> int:
     algo = pass_bytes(name, N, N, N, 4)
     achieved = algo / avg_s / 1e9
     step_bytes = sum(pass_bytes(k, N, N, N, 4) for k in passes)
+    traffic, why = pmc_traffic(f"{workload}_fp32", name, tuning)
     roof = {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic(f"cahn_hilliard_3d_{N}_fp32", name),
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "bytes_per_launch": algo,
             "avg_launch_ms": ms / cnt, "lines_per_block": tuning,
+            "layout": "stored (gopf_amd/tuning/gfx950.json)" if lay["fp32"] else "autotuned in this run",
             "passes_ms_per_launch": {k: v[0] / v[1] for k, v in passes.items()},
+            "passes_GBs": {k: pass_bytes(k, N, N, N, 4) / (v[0] / v[1] * 1e-3) / 1e9 for k, v in passes.items()},
             "step_effective_GBs": step_bytes * args.steps / el32 / 1e9,
             "step_frac": step_bytes * args.steps / el32 / 1e9 / HBM_PEAK_GBS,
-            "measured_copy_GBs": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs}
+            "measured_GBs": bw, "frac_of_measured_copy": achieved / bw["copy"],
+            "step_frac_of_measured_copy": step_bytes * args.steps / el32 / 1e9 / bw["copy"]}
+    if why:
+        roof["traffic_note"] = why
     cpu = None
     if not args.no_cpu:
         print("# timing the CPU oracle (one step)", file=sys.stderr, flush=True)
@@ -656,6 +689,9 @@ def main() -> int:
         "passes_sum_ms": sum(v[0] / v[1] for v in passes.values()),
     }
     out.update(res)
+    if args.save_tuning and not args.no_fp64:
+        with open(args.save_tuning, "w") as f:
+            json.dump({workload: {"fp32": tuning, "fp64": tuning64}}, f, indent=1)
     print(json.dumps(out))
     return 0
 

@@ -119,6 +119,7 @@ def _declare(L) -> None:
         "gpf_solver_pass_times": (C.c_int, [_vp, C.c_int, _dp, _lp, C.c_char_p, C.c_int]),
         "gpf_solver_stream": (_vp, [_vp]),
         "gpf_solver_tuning": (C.c_int, [_vp, _ip, C.c_int]),
+        "gpf_solver_set_tuning": (C.c_int, [_vp, _ip, C.c_int]),
         "gpf_solver_set_filter": (C.c_int, [_vp, _dp, C.c_long]),
         "gpf_solver_set_newton_krylov": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int]),
         "gpf_solver_newton_stats": (C.c_int, [_vp, _ip, _ip, _ip, _dp]),
@@ -140,6 +141,7 @@ def _declare(L) -> None:
         "gpf_solver_adjoint": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_long, _dp, _dp, C.c_int,
                                          C.POINTER(C.c_char_p), _dp]),
         "gpf_device_copy_bandwidth": (C.c_int, [C.c_long, C.c_int, C.c_int, _dp]),
+        "gpf_device_bandwidth": (C.c_int, [C.c_long, C.c_int, C.c_int, C.c_int, _dp]),
         "gpf_batch_create": (C.c_int, [_vp, C.c_int, _ip, _ip, C.c_double, C.POINTER(_Options), C.POINTER(_vp)]),
         "gpf_batch_destroy": (None, [_vp]),
         "gpf_batch_num_sims": (C.c_int, [_vp]),
@@ -569,6 +571,15 @@ class Solver:
                 "pencils_fwd_lines_per_thread": v[6], "pencils_inv_lines_per_thread": v[7],
                 "pencils_fwd_schedule": v[8], "pencils_inv_schedule": v[9]}
 
+    TUNING_KEYS = ("rows_fused", "pencils_fwd", "pencil_update", "pencils_inv", "update_axis", "rows_schedule",
+                   "pencils_fwd_lines_per_thread", "pencils_inv_lines_per_thread", "pencils_fwd_schedule",
+                   "pencils_inv_schedule")
+
+    def SetTuning(self, t: Dict[str, object]) -> None:
+        """Applies a launch layout returned by Tuning() instead of the autotuner."""
+        v = [int({"z": 2, "y": 1}[t[k]]) if k == "update_axis" else int(t[k]) for k in self.TUNING_KEYS]
+        _check(lib().gpf_solver_set_tuning(self._h, _ints(v), len(v)))
+
     def Stream(self) -> int:
         return int(lib().gpf_solver_stream(self._h) or 0)
 
@@ -782,6 +793,17 @@ def CopyBandwidth(nbytes: int, reps: int = 20, device: int = 0) -> float:
     ms = C.c_double()
     _check(lib().gpf_device_copy_bandwidth(int(nbytes), int(reps), int(device), C.byref(ms)))
     return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
+
+
+def DeviceBandwidth(nbytes: int, kind: str = "copy", reps: int = 10, device: int = 0) -> float:
+    """Measured device bandwidth in GB/s: "copy" (nontemporal, read + write bytes),
+    "read" or "write"."""
+    k = {"copy": 0, "read": 1, "write": 2}[kind]
+    ms = C.c_double()
+    _check(lib().gpf_device_bandwidth(int(nbytes), int(reps), k, int(device), C.byref(ms)))
+    unit = 4 * 256 * 4 * 4
+    moved = (int(nbytes) // unit) * unit * (2 if k == 0 else 1)
+    return moved / (ms.value * 1e-3) / 1e9
 
 
 def Freq(dims: Sequence[int], i: int) -> List[float]:

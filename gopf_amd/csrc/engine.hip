@@ -187,9 +187,12 @@ struct Geometry {
     }
 };
 
+// a block of tw lines fits: threads, and LDS for the exchange lines, the row pass's copy of
+// the twiddle table and the monitor's FP64 partial sums (160 KiB per workgroup on gfx950)
 static bool tw_fits(int n, int tw, size_t elem, int variant = 0) {
     const int T = fft_threads_per_line(n, variant);
-    return tw * T <= kMaxBlockThreads && (size_t)(n + n / 16 + 1) * (tw + 1) * elem <= 150 * 1024;
+    const size_t lds = ((size_t)(n + n / 16 + 1) * (tw + 1) + n) * elem + (size_t)kMaxBlockThreads * sizeof(double);
+    return tw * T <= kMaxBlockThreads && lds <= 160 * 1024;
 }
 
 static int choose_tw(int n, int narr, size_t elem, int min_tw, int variant = 0) {
@@ -370,6 +373,7 @@ public:
     virtual hipStream_t stream() const = 0;
     virtual long size() const = 0;
     virtual int tuning(int* tws, int maxn) = 0;
+    virtual void set_tuning(const int* tws, int n) = 0;
     virtual void set_filter(const double* table) = 0;
     virtual void adjoint(long nsteps, long every, const double* target, double* loss, double* grad_field,
                          int nscal, const char* const* names, double* grad_scalars) = 0;
@@ -408,6 +412,7 @@ public:
     hipStream_t stream() const override { return st_; }
     long size() const override { return g_.n; }
     int tuning(int* tws, int maxn) override;
+    void set_tuning(const int* tws, int n) override;
     void set_filter(const double* table) override;
     void adjoint(long nsteps, long every, const double* target, double* loss, double* grad_field, int nscal,
                  const char* const* names, double* grad_scalars) override;
@@ -452,7 +457,8 @@ private:
     void prime();
     void pencils_tw(const std::vector<cpx<T>*>& dst, int axis, int dir, int tw, int lpt = 1, int var = 0);
     void autotune();
-    bool autotune_ = false, tuned_ = false;
+    void warm(double ms_min);  // the step's passes on scratch copies for at least ms_min ms
+    bool autotune_ = false, tuned_ = false, warmed_ = false;
     int tw_rows_ = 0, tw_pf_ = 0, tw_pu_ = 0, tw_pi_ = 0;
     int rows_var_ = 0;  // transform schedule of the row pass (0 plain, 1 low-register)
     int lpt_pf_ = 1, lpt_pi_ = 1;  // lines per thread of the forward / inverse pencil passes
@@ -1300,6 +1306,7 @@ void Engine<T>::autotune() {
     HIPCHK(hipEventDestroy(e1));
     HIPCHK(hipStreamSynchronize(st_));
     tuned_ = true;
+    warmed_ = true;  // tens of milliseconds of the step's kernels have just run
     prime();  // the work array follows the chosen update axis
 }
 
@@ -1325,6 +1332,45 @@ int Engine<T>::tuning(int* tws, int maxn) {
     const int v[10] = {tw_rows_, tw_pf_, tw_pu_, tw_pi_, upd_axis(), rows_var_, lpt_pf_, lpt_pi_, var_pf_, var_pi_};
     for (int i = 0; i < 10 && i < maxn; ++i) tws[i] = v[i];
     return 10;
+}
+
+// a launch layout read back with tuning(), applied instead of running the autotuner (the
+// same layout on every run of a workload, as FFTW's wisdom)
+template <typename T>
+void Engine<T>::set_tuning(const int* v, int n) {
+    if (!fp_.ok) throw Error(GPF_EINVAL, "launch layouts apply to the fused path only");
+    if (n != 10) throw Error(GPF_EINVAL, "a launch layout has 10 values (gpf_solver_tuning order)");
+    const int axis = (g_.ndim == 3) ? v[4] : 1;
+    if (axis != 1 && axis != 2) throw Error(GPF_EINVAL, "update axis must be 1 (y) or 2 (z)");
+    const int other = (axis == 2) ? 1 : 2;
+    const int nu = axis == 2 ? g_.nz : g_.ny, no = other == 2 ? g_.nz : g_.ny;
+    const size_t el = sizeof(cpx<T>);
+    auto ok_tw = [](int tw) { return tw == 4 || tw == 8 || tw == 16 || tw == 32 || tw == 64; };
+    const int rows_var = v[5], lpf = v[6], lpi = v[7], vpf = v[8], vpi = v[9];
+    bool ok = ok_tw(v[0]) && ok_tw(v[2]) && (rows_var == 0 || rows_var == 1) && tw_fits(g_.nx, v[0], el, rows_var) &&
+              tw_fits(nu, v[2], el, 1);
+    if (g_.ndim == 3) {
+        ok = ok && ok_tw(v[3]) && (vpi == 0 || vpi == 1) && (lpi == 1 || (lpi == 2 && sizeof(T) == 4)) &&
+             tw_fits(no, v[3], el, vpi);
+        if (fp_.has_src)
+            ok = ok && ok_tw(v[1]) && (vpf == 0 || vpf == 1) && (lpf == 1 || (lpf == 2 && sizeof(T) == 4)) &&
+                 tw_fits(no, v[1], el, vpf);
+    }
+    if (!ok) throw Error(GPF_EINVAL, "launch layout not admissible for this grid and precision");
+    HIPCHK(hipStreamSynchronize(st_));
+    tw_rows_ = v[0];
+    tw_pf_ = v[1];
+    tw_pu_ = v[2];
+    tw_pi_ = v[3];
+    upd_axis_ = axis;
+    rows_var_ = rows_var;
+    lpt_pf_ = lpf;
+    lpt_pi_ = lpi;
+    var_pf_ = vpf;
+    var_pi_ = vpi;
+    tuned_ = true;
+    drop_graph();
+    prime();  // the work array follows the update axis
 }
 
 template <typename T>
@@ -1418,8 +1464,37 @@ void Engine<T>::capture_graph() {
     HIPCHK(hipGraphUpload(graph_, st_));
 }
 
+// Runs the step's passes, in the chosen launch layout, on scratch copies of the state
+// until ms_min milliseconds have passed (at most 2000 steps): an idle device runs its first
+// milliseconds of work at lower clocks, which the autotuner's timing runs otherwise absorb.
+template <typename T>
+void Engine<T>::warm(double ms_min) {
+    if (!fp_.ok) return;
+    Buf<cpx<T>> Sb(g_.nk), Wb(g_.nk);
+    Buf<double> mu(1);
+    HIPCHK(hipMemcpyAsync(Sb.get(), S_.get(), g_.nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+    HIPCHK(hipMemcpyAsync(Wb.get(), W_.get(), g_.nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+    HIPCHK(hipMemcpyAsync(mu.get(), mean_.get(), sizeof(double), hipMemcpyDeviceToDevice, st_));
+    const int nu = axis_len(upd_axis());
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < 200; ++k) {
+        for (int j = 0; j < 10; ++j) {
+            if (fp_.has_src) {
+                if (g_.ndim == 3) pencils_tw({Wb.get()}, other_axis(), +1, tw_pi_, lpt_pi_, var_pi_);
+                HIPCHK(launch_rows_fused<T>(rows_args(Wb.get(), mu.get(), tw_rows_, rows_var_), st_));
+                if (g_.ndim == 3) pencils_tw({Wb.get()}, other_axis(), -1, tw_pf_, lpt_pf_, var_pf_);
+            }
+            HIPCHK(launch_pencil_update<T>(update_args(Sb.get(), Wb.get(), mu.get(), tw_pu_), nu, st_));
+        }
+        HIPCHK(hipStreamSynchronize(st_));
+        if (std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() >= ms_min) break;
+    }
+    warmed_ = true;
+}
+
 // The autotuner (on scratch copies), the monitor's buffers and the step graph, ahead of
-// the first step, so that a caller's timed steps replay graphs only.  The state is not
+// the first step, so that a caller's timed steps replay graphs only; without the tuner,
+// the warm-up on scratch copies brings the device to its steady clocks.  The state is not
 // advanced: the tuner's prime() recomputes W from S, which it already holds.
 template <typename T>
 double Engine<T>::prepare(long upcoming) {
@@ -1429,6 +1504,7 @@ double Engine<T>::prepare(long upcoming) {
             autotune();
             drop_graph();
         }
+        if (!warmed_) warm(30.0);
         prepare_fused(std::max(upcoming, 0L));
         if (!timing_ && !c_out_) capture_graph();
         HIPCHK(hipStreamSynchronize(st_));
@@ -3106,6 +3182,43 @@ int gpf_device_copy_bandwidth(long bytes, int reps, int device, double* ms_per_c
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         (void)hipStreamDestroy(st);
+    });
+}
+
+int gpf_device_bandwidth(long bytes, int reps, int kind, int device, double* ms_per_pass) {
+    return guarded([&] {
+        if (bytes < (1L << 20) || reps < 1 || kind < 0 || kind > 2 || !ms_per_pass)
+            throw Error(GPF_EINVAL, "bad argument");
+        set_device(device);
+        const long unit = 4L * 256 * 4;  // floats per grid-stride step of one thread block row
+        const long n = (bytes / (long)sizeof(float)) / unit * unit;
+        Buf<float> a(n), b(n);
+        hipStream_t st;
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipMemsetAsync(a.get(), 0, n * sizeof(float), st));
+        HIPCHK(hipMemsetAsync(b.get(), 0, n * sizeof(float), st));
+        HIPCHK(launch_bw(kind, a.get(), b.get(), n, st));
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r) HIPCHK(launch_bw(kind, a.get(), b.get(), n, st));
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        *ms_per_pass = ms / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(st);
+    });
+}
+
+int gpf_solver_set_tuning(gpf_solver* s, const int* tws, int n) {
+    return guarded([&] {
+        if (!s || !tws) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        s->eng->set_tuning(tws, n);
     });
 }
 

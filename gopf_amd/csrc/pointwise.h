@@ -323,6 +323,7 @@ template <typename T> hipError_t launch_kgrad_energy(const cpx<T>* C, KGrid g, l
                                                      double* partial, long nblocks, hipStream_t s);
 // vectorised copy of n floats (n a multiple of 4): the bandwidth probe kernel
 hipError_t launch_copy_bw(const float* src, float* dst, long n, hipStream_t s);
+hipError_t launch_bw(int kind, const float* src, float* dst, long n, hipStream_t s);
 // single-block reduction of partials; out[0] = sum a * ca + sum b * cb
 hipError_t launch_reduce2(const double* a, long na, double ca, const double* b, long nb, double cb,
                           double* out, hipStream_t s);

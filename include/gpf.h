@@ -213,6 +213,10 @@ int gpf_solver_pass_times(gpf_solver* s, int maxpass, double* ms, long* launches
  * of the forward and inverse pencil passes and their transform schedules; returns the
  * number of values written */
 int gpf_solver_tuning(gpf_solver* s, int* tws, int maxn);
+/* applies a launch layout of 10 values as gpf_solver_tuning returns them (from an earlier
+ * run of the same grid and precision) instead of running the autotuner; GPF_EINVAL when
+ * the layout is not admissible for this grid and precision */
+int gpf_solver_set_tuning(gpf_solver* s, const int* tws, int n);
 /* the HIP stream the solver launches on (hipStream_t) */
 void* gpf_solver_stream(gpf_solver* s);
 /* TimeStepper.SetFilter (pf/euler.go:62-64, pf/rk4.go:138-140) with the filter as the
@@ -302,6 +306,12 @@ int gpf_solver_adjoint(gpf_solver* s, int nsteps, int checkpoint_every, const do
  * copy bandwidth the roofline is also reported against, and the known-size kernel
  * the HBM traffic counters are calibrated with). */
 int gpf_device_copy_bandwidth(long bytes, int reps, int device, double* ms_per_copy);
+/* Achievable-bandwidth probes over `bytes` bytes (at least 1 MiB; rounded down to whole
+ * 16 KiB blocks): kind 0 copies with nontemporal 16-byte loads and stores (bytes read +
+ * bytes written per pass), kind 1 only reads, kind 2 only writes.  Average milliseconds of
+ * one pass over reps passes.  No reference counterpart: the measured rates the roofline
+ * fractions are also reported against. */
+int gpf_device_bandwidth(long bytes, int reps, int kind, int device, double* ms_per_pass);
 
 /* ---------------- pf.FourierTransform (pfutil.FFTWWrapper) ---------------- */
 
