@@ -20,18 +20,22 @@ import numpy as np
 from . import parser as P
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# GPF_ORACLE_LIB selects another build of the same source (the sanitizer test's)
+LIB_PATH = os.environ.get("GPF_ORACLE_LIB", os.path.join(_HERE, "liboracle.so"))
 _lib = None
 
 
-def build(force: bool = False) -> str:
-    """Compile oracle/pf_oracle.c into oracle/liboracle.so with gcc."""
+def build(force: bool = False, path: Optional[str] = None, sanitize: bool = False) -> str:
+    """Compile oracle/pf_oracle.c into oracle/liboracle.so (or `path`) with gcc; sanitize
+    adds AddressSanitizer and UndefinedBehaviorSanitizer (UB aborts)."""
     src = os.path.join(_HERE, "pf_oracle.c")
-    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
-        cmd = ["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
-               "-o", LIB_PATH, src, "-lm"]
+    out = path or LIB_PATH
+    if force or not os.path.exists(out) or os.path.getmtime(out) < os.path.getmtime(src):
+        san = ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-g"]
+        cmd = ["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared"] + (san if sanitize else []) + \
+              ["-o", out, src, "-lm"]
         subprocess.check_call(cmd)
-    return LIB_PATH
+    return out
 
 
 def lib():
