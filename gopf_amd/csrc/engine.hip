@@ -96,6 +96,59 @@ static std::vector<cpx<T>> twiddle_host(int n) {
     return w;
 }
 
+// Bluestein tables of a length-n transform of direction dir (+-1) with convolution length m:
+// the chirp w_j = exp(dir pi i j^2 / n), j < n (the exponent reduced mod 2n exactly, in
+// integers, before the angle is formed), and the forward m-point transform of the
+// zero-padded conj(w) (index j and m - j), divided by m.  Built in long double with an
+// iterative radix-2 transform.
+static int bluestein_length(int n) {
+    int m = 1;
+    while (m < 2 * n - 1) m *= 2;
+    return m;
+}
+
+template <typename T>
+static void bluestein_tables(int n, int dir, int m, std::vector<cpx<T>>& chirp, std::vector<cpx<T>>& filt) {
+    const long double pi = 3.14159265358979323846264338327950288L;
+    std::vector<std::complex<long double>> w(n), b(m, std::complex<long double>(0.0L, 0.0L));
+    for (long j = 0; j < n; ++j) {
+        const long q = (j * j) % (2L * n);
+        const long double ang = (long double)dir * pi * (long double)q / (long double)n;
+        w[j] = std::complex<long double>(cosl(ang), sinl(ang));
+    }
+    for (int j = 0; j < n; ++j) {
+        b[j] = std::conj(w[j]);
+        if (j > 0) b[m - j] = std::conj(w[j]);
+    }
+    // forward transform of b (sign -1), bit-reversal then butterflies
+    for (int i = 1, r = 0; i < m; ++i) {
+        int bit = m >> 1;
+        for (; r & bit; bit >>= 1) r ^= bit;
+        r ^= bit;
+        if (i < r) std::swap(b[i], b[r]);
+    }
+    for (int len = 2; len <= m; len <<= 1) {
+        const long double ang = -2.0L * pi / (long double)len;
+        for (int i = 0; i < m; i += len)
+            for (int k = 0; k < len / 2; ++k) {
+                const std::complex<long double> tw(cosl(ang * k), sinl(ang * k));
+                const std::complex<long double> u = b[i + k], v = b[i + k + len / 2] * tw;
+                b[i + k] = u + v;
+                b[i + k + len / 2] = u - v;
+            }
+    }
+    chirp.resize(n);
+    filt.resize(m);
+    for (int j = 0; j < n; ++j) {
+        chirp[j].x = (T)w[j].real();
+        chirp[j].y = (T)w[j].imag();
+    }
+    for (int j = 0; j < m; ++j) {
+        filt[j].x = (T)(b[j].real() / (long double)m);
+        filt[j].y = (T)(b[j].imag() / (long double)m);
+    }
+}
+
 // Vandeven table (pf/vandeven.go:13-27)
 static void vandeven_table(int order, double* data) {
     data[0] = 1.0;
@@ -2618,6 +2671,10 @@ public:
     virtual long size() const = 0;
 };
 
+// the longest length the FourierTransform drop-in transforms without a compiled schedule
+// (Bluestein with a convolution of at most 2048 points)
+constexpr int kBluesteinMax = 1024;
+
 template <typename T>
 class FullFFT : public FFTBase {
 public:
@@ -2627,7 +2684,9 @@ public:
         ny_ = (ndim >= 2) ? dims[ndim - 2] : 1;
         nz_ = (ndim == 3) ? dims[0] : 1;
         for (int v : {nx_, ny_, nz_})
-            if (v > 1 && !fft_size_supported(v)) throw Error(GPF_EINVAL, "unsupported transform length " + std::to_string(v));
+            if (v > 1 && !fft_size_supported(v) && v > kBluesteinMax)
+                throw Error(GPF_EINVAL, "unsupported transform length " + std::to_string(v) +
+                                            " (lengths without a compiled schedule must be <= 1024)");
         n_ = (long)nx_ * ny_ * nz_;
     }
     ~FullFFT() override {
@@ -2672,18 +2731,65 @@ private:
         if (st_) return;
         HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
         d_.alloc(n_);
-        for (int v : {nx_, ny_, nz_}) {
-            if (v <= 1) continue;
-            bool have = false;
-            for (auto& p : tw_) have |= (p.first == v);
-            if (have) continue;
+        auto upload_tw = [&](int v) {
+            for (auto& p : tw_)
+                if (p.first == v) return;
             auto h = twiddle_host<T>(v);
             Buf<cpx<T>> b(v);
             HIPCHK(hipMemcpy(b.get(), h.data(), v * sizeof(cpx<T>), hipMemcpyHostToDevice));
             tw_.emplace_back(v, std::move(b));
+        };
+        for (int v : {nx_, ny_, nz_}) {
+            if (v <= 1) continue;
+            if (fft_size_supported(v)) {
+                upload_tw(v);
+                continue;
+            }
+            bool have = false;
+            for (auto& bt : blue_) have |= (bt.n == v);
+            if (have) continue;
+            const int m = bluestein_length(v);
+            upload_tw(m);
+            for (int dir : {-1, +1}) {
+                std::vector<cpx<T>> ch, fl;
+                bluestein_tables<T>(v, dir, m, ch, fl);
+                Blue bt;
+                bt.n = v;
+                bt.dir = dir;
+                bt.m = m;
+                bt.chirp.alloc(v);
+                bt.filt.alloc(m);
+                HIPCHK(hipMemcpy(bt.chirp.get(), ch.data(), v * sizeof(cpx<T>), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(bt.filt.get(), fl.data(), m * sizeof(cpx<T>), hipMemcpyHostToDevice));
+                blue_.push_back(std::move(bt));
+            }
         }
     }
+    const cpx<T>* twid(int n) const {
+        for (auto& p : tw_)
+            if (p.first == n) return p.second.get();
+        return nullptr;
+    }
     void pass(const Seg& s, int n, int dir) {
+        if (!fft_size_supported(n)) {
+            // no compiled schedule and longer than the brute-force limit: Bluestein
+            for (auto& bt : blue_) {
+                if (bt.n != n || bt.dir != dir) continue;
+                BluesteinArgs<T> b;
+                b.p.tw = twid(bt.m);
+                b.p.narr = 1;
+                b.p.arr[0][0] = d_.get();
+                b.p.seg[0] = s;
+                b.p.nseg = 1;
+                b.p.TW = choose_tw(bt.m, 1, sizeof(cpx<T>), s.sb == 1 ? 8 : 1);
+                b.m = bt.m;
+                b.chirp = bt.chirp.get();
+                b.filt = bt.filt.get();
+                HIPCHK(launch_bluestein<T>(b, n, st_));
+                return;
+            }
+            throw Error(GPF_EINVAL, "no Bluestein plan for length " + std::to_string(n));
+        }
         PencilArgs<T> a;
         for (auto& p : tw_)
             if (p.first == n) a.tw = p.second.get();
@@ -2700,6 +2806,11 @@ private:
     hipStream_t st_ = nullptr;
     Buf<cpx<T>> d_;
     std::vector<std::pair<int, Buf<cpx<T>>>> tw_;
+    struct Blue {
+        int n = 0, dir = 0, m = 0;
+        Buf<cpx<T>> chirp, filt;
+    };
+    std::vector<Blue> blue_;
 };
 
 }  // namespace gpf
@@ -3453,6 +3564,14 @@ int gpf_batch_synchronize(gpf_batch* b) {
         if (!b) throw Error(GPF_EINVAL, "bad argument");
         set_device(b->device);
         b->impl->synchronize();
+    });
+}
+
+int gpf_batch_set_layout(gpf_batch* b, const int* v, int n) {
+    return guarded([&] {
+        if (!b || !v) throw Error(GPF_EINVAL, "bad argument");
+        set_device(b->device);
+        b->impl->set_layout(v, n);
     });
 }
 

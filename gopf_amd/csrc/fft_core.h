@@ -80,8 +80,39 @@ GPF_SCHED_V(96, 1, 12, 4, 4, 2, 3)
 GPF_SCHED_V(192, 1, 12, 4, 4, 4, 3)
 GPF_SCHED_V(384, 1, 12, 4, 4, 4, 2, 3)
 GPF_SCHED_V(768, 1, 12, 4, 4, 4, 4, 3)
+// radix-5 lengths (E = 20 or 10: a multiple of every radix of the schedule)
+GPF_SCHED(80, 20, 4, 4, 5)
+GPF_SCHED(100, 20, 4, 5, 5)
+GPF_SCHED(160, 20, 4, 4, 2, 5)
+GPF_SCHED(200, 20, 4, 2, 5, 5)
+GPF_SCHED(250, 10, 2, 5, 5, 5)
+GPF_SCHED(320, 20, 4, 4, 4, 5)
+GPF_SCHED(400, 20, 4, 4, 5, 5)
+GPF_SCHED(500, 20, 4, 5, 5, 5)
+GPF_SCHED(640, 20, 4, 4, 4, 2, 5)
+GPF_SCHED(800, 20, 4, 4, 2, 5, 5)
+GPF_SCHED(1000, 20, 4, 2, 5, 5, 5)
+GPF_SCHED(1280, 20, 4, 4, 4, 4, 5)
+GPF_SCHED(1600, 20, 4, 4, 4, 5, 5)
+GPF_SCHED(2000, 20, 4, 4, 5, 5, 5)
+GPF_SCHED(2560, 20, 4, 4, 4, 4, 2, 5)
+// radix-7 lengths
+GPF_SCHED(112, 28, 4, 4, 7)
+GPF_SCHED(224, 28, 4, 4, 2, 7)
+GPF_SCHED(448, 28, 4, 4, 4, 7)
+GPF_SCHED(896, 28, 4, 4, 4, 2, 7)
+// 2^11 (the largest power of two with a compiled schedule)
+GPF_SCHED(2048, 16, 16, 16, 8)
+GPF_SCHED_V(2048, 1, 8, 8, 8, 8, 4)
 #undef GPF_SCHED
 #undef GPF_SCHED_V
+
+// every length with a compiled schedule, for dispatch tables: X(n) per length
+#define GPF_FAST_LENGTHS(X)                                                                        \
+    X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)                          \
+    X(12) X(24) X(48) X(96) X(192) X(384) X(768)                                                   \
+    X(80) X(100) X(160) X(200) X(250) X(320) X(400) X(500) X(640) X(800) X(1000) X(1280) X(1600)   \
+    X(2000) X(2560) X(112) X(224) X(448) X(896)
 
 // span before stage S
 template <int N, int V, int S>
@@ -124,6 +155,58 @@ __device__ __forceinline__ void dft3(cpx<T>* a) {
     a[0] = add(a[0], sm);
     a[1] = add(b, r);
     a[2] = sub(b, r);
+}
+
+// length 5: X_k = x_0 + sum_{j=1,2} (x_j + x_{5-j}) cos(2 pi jk/5)
+//               + DIR i sum_{j=1,2} (x_j - x_{5-j}) sin(2 pi jk/5)
+template <int DIR, typename T>
+__device__ __forceinline__ void dft5(cpx<T>* a) {
+    const T c1 = T(0.30901699437494742410229341718281906);   // cos(2 pi / 5)
+    const T c2 = T(-0.80901699437494742410229341718281906);  // cos(4 pi / 5)
+    const T s1 = T(DIR) * T(0.95105651629515357211643933337938214);  // DIR sin(2 pi / 5)
+    const T s2 = T(DIR) * T(0.58778525229247312916870595463907277);  // DIR sin(4 pi / 5)
+    const cpx<T> t1 = add(a[1], a[4]), t2 = add(a[2], a[3]);
+    const cpx<T> u1 = sub(a[1], a[4]), u2 = sub(a[2], a[3]);
+    const cpx<T> x0 = a[0];
+    a[0] = add(x0, add(t1, t2));
+    const cpx<T> p1 = mk<T>(x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y);
+    const cpx<T> p2 = mk<T>(x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y);
+    // i (s1 u1 + s2 u2) and i (s2 u1 - s1 u2)
+    const cpx<T> q1 = mk<T>(-(s1 * u1.y + s2 * u2.y), s1 * u1.x + s2 * u2.x);
+    const cpx<T> q2 = mk<T>(-(s2 * u1.y - s1 * u2.y), s2 * u1.x - s1 * u2.x);
+    a[1] = add(p1, q1);
+    a[4] = sub(p1, q1);
+    a[2] = add(p2, q2);
+    a[3] = sub(p2, q2);
+}
+
+// length 7, the same pairing of x_j and x_{7-j}; cos(2 pi m/7) and sin(2 pi m/7) for
+// m = jk mod 7 reduce to c1..c3 and +-s1..s3
+template <int DIR, typename T>
+__device__ __forceinline__ void dft7(cpx<T>* a) {
+    const T c1 = T(0.62348980185873353052500488400423981);   // cos(2 pi / 7)
+    const T c2 = T(-0.22252093395631440428890256449679476);  // cos(4 pi / 7)
+    const T c3 = T(-0.90096886790241912623610231950744505);  // cos(6 pi / 7)
+    const T s1 = T(DIR) * T(0.78183148246802980870844452667405775);  // DIR sin(2 pi / 7)
+    const T s2 = T(DIR) * T(0.97492791218182360701813168299393122);  // DIR sin(4 pi / 7)
+    const T s3 = T(DIR) * T(0.43388373911755812047576833284835875);  // DIR sin(6 pi / 7)
+    const cpx<T> t1 = add(a[1], a[6]), t2 = add(a[2], a[5]), t3 = add(a[3], a[4]);
+    const cpx<T> u1 = sub(a[1], a[6]), u2 = sub(a[2], a[5]), u3 = sub(a[3], a[4]);
+    const cpx<T> x0 = a[0];
+    a[0] = add(x0, add(t1, add(t2, t3)));
+    // k = 1: (c1, c2, c3), (s1, s2, s3); k = 2: (c2, c3, c1), (s2, -s3, -s1); k = 3: (c3, c1, c2), (s3, -s1, s2)
+    const cpx<T> p1 = mk<T>(x0.x + c1 * t1.x + c2 * t2.x + c3 * t3.x, x0.y + c1 * t1.y + c2 * t2.y + c3 * t3.y);
+    const cpx<T> p2 = mk<T>(x0.x + c2 * t1.x + c3 * t2.x + c1 * t3.x, x0.y + c2 * t1.y + c3 * t2.y + c1 * t3.y);
+    const cpx<T> p3 = mk<T>(x0.x + c3 * t1.x + c1 * t2.x + c2 * t3.x, x0.y + c3 * t1.y + c1 * t2.y + c2 * t3.y);
+    const T r1x = s1 * u1.x + s2 * u2.x + s3 * u3.x, r1y = s1 * u1.y + s2 * u2.y + s3 * u3.y;
+    const T r2x = s2 * u1.x - s3 * u2.x - s1 * u3.x, r2y = s2 * u1.y - s3 * u2.y - s1 * u3.y;
+    const T r3x = s3 * u1.x - s1 * u2.x + s2 * u3.x, r3y = s3 * u1.y - s1 * u2.y + s2 * u3.y;
+    a[1] = mk<T>(p1.x - r1y, p1.y + r1x);
+    a[6] = mk<T>(p1.x + r1y, p1.y - r1x);
+    a[2] = mk<T>(p2.x - r2y, p2.y + r2x);
+    a[5] = mk<T>(p2.x + r2y, p2.y - r2x);
+    a[3] = mk<T>(p3.x - r3y, p3.y + r3x);
+    a[4] = mk<T>(p3.x + r3y, p3.y - r3x);
 }
 
 template <int DIR, typename T>
@@ -196,6 +279,10 @@ __device__ __forceinline__ void dftR(cpx<T>* a) {
         dft3<DIR>(a);
     } else if constexpr (R == 4) {
         dft4<DIR>(a[0], a[1], a[2], a[3]);
+    } else if constexpr (R == 5) {
+        dft5<DIR>(a);
+    } else if constexpr (R == 7) {
+        dft7<DIR>(a);
     } else if constexpr (R == 8) {
         dft8<DIR>(a);
     } else if constexpr (R == 16) {

@@ -90,6 +90,19 @@ struct PencilArgs {
     int variant = 0;                     // transform schedule of k_pencils: 0 plain, 1 low-register
 };
 
+// Bluestein transform of pencils of any length n <= 1024 (those without a compiled
+// schedule): X_k = w_k sum_j (x_j w_j) conj(w_{k-j}), w_j = exp(DIR pi i j^2 / n), a
+// circular convolution of length M (a power of two >= 2n - 1) computed with two M-point
+// transforms.  chirp holds w_j (n values), filt the forward M-point transform of the
+// zero-padded conj(w) divided by M.
+template <typename T>
+struct BluesteinArgs {
+    PencilArgs<T> p;                     // tiles, segments and the M-point twiddle table (p.tw)
+    int m = 0;                           // convolution length M
+    const cpx<T>* chirp = nullptr;
+    const cpx<T>* filt = nullptr;
+};
+
 // fused update pass of the single-field fast path, along the last axis of the step:
 //   W (source g, transformed along every other axis) --forward--> G^
 //   S <- m1 S + m2 G^ / N            (S: the state spectrum F(c - mean) / N, in place)
@@ -230,7 +243,27 @@ struct RowFusedArgs {
 };
 
 // launchers (implemented in spectral.hip); n is the line length.
+// Grouped launches of a batch of size classes: class c owns blocks [first[c], first[c] +
+// blocks[c]) of the launch (first[c] a multiple of 8, first[ncls] the launch's block count);
+// rows / upd point to device arrays of the classes' arguments.
+constexpr int kMaxGroup = 8;
+template <typename T>
+struct BatchGroupArgs {
+    int ncls = 0;
+    int first[kMaxGroup + 1] = {};
+    int blocks[kMaxGroup] = {};
+    const RowFusedArgs<T>* rows = nullptr;
+    const PencilUpdateArgs<T>* upd = nullptr;
+};
+template <typename T> hipError_t launch_rows_fused_group(const BatchGroupArgs<T>& g, int threads, size_t smem,
+                                                         hipStream_t s);
+template <typename T> hipError_t launch_pencil_update_group(const BatchGroupArgs<T>& g, int threads, size_t smem,
+                                                            hipStream_t s);
+bool group_length(int n);  // transform lengths the grouped launches are compiled for
+
 template <typename T> hipError_t launch_pencils(const PencilArgs<T>& a, int n, int dir, hipStream_t s);
+template <typename T, int DIR> hipError_t launch_pencils_dir(const PencilArgs<T>& a, int n, hipStream_t s);
+template <typename T> hipError_t launch_bluestein(const BluesteinArgs<T>& a, int n, hipStream_t s);
 template <typename T> hipError_t launch_pencil_update(const PencilUpdateArgs<T>& a, int n, hipStream_t s);
 template <typename T> hipError_t launch_rows_r2c(const RowR2CArgs<T>& a, hipStream_t s);
 template <typename T> hipError_t launch_rows_c2r(const RowC2RArgs<T>& a, hipStream_t s);

@@ -1,15 +1,28 @@
-// spectral.hip -- host helpers of the spectral kernels (supported lengths, grid sizes) and the
-// instantiations of the row transforms k_rows_r2c, k_rows_c2r and k_rows_sync.  The other
-// kernel families are instantiated in spectral_pencils.hip, spectral_update.hip and
-// spectral_rows.hip; the kernels themselves are in spectral_kernels.h.
+// spectral.hip -- host helpers of the spectral kernels (supported lengths, grid sizes, the
+// direction switch of the pencil transforms).  The kernel families are instantiated in units
+// of their own, one per family and precision (spectral_pencils*.hip, spectral_update*.hip,
+// spectral_rows*.hip, spectral_r2c*.hip, spectral_c2r*.hip, spectral_sync*.hip), so that they
+// compile in parallel; the kernels themselves are in spectral_kernels.h.
 #include "spectral_kernels.h"
 
 namespace gpf {
 
 bool fft_size_fast(int n) {
     switch (n) {
-        case 2: case 4: case 8: case 16: case 32: case 64: case 128: case 256: case 512: case 1024:
-        case 12: case 24: case 48: case 96: case 192: case 384: case 768:
+#define GPF_C(NN) case NN:
+        GPF_FAST_LENGTHS(GPF_C)
+#undef GPF_C
+            return true;
+        default:
+            return false;
+    }
+}
+
+bool group_length(int n) {
+    switch (n) {
+#define GPF_C(NN) case NN:
+        GPF_GROUP_LENGTHS(GPF_C)
+#undef GPF_C
             return true;
         default:
             return false;
@@ -33,12 +46,18 @@ long pencil_blocks(const Seg* seg, int nseg, int TW, int* tiles0) {
     return tot;
 }
 
-#define GPF_INST(T)                                                                  \
-    template hipError_t launch_rows_r2c<T>(const RowR2CArgs<T>&, hipStream_t);     \
-    template hipError_t launch_rows_c2r<T>(const RowC2RArgs<T>&, hipStream_t);     \
-    template hipError_t launch_rows_sync<T>(const RowSyncArgs<T>&, hipStream_t);
-GPF_INST(float)
-GPF_INST(double)
-#undef GPF_INST
+// instantiated in spectral_pencils_{fwd,inv}{,64}.hip, not here
+extern template hipError_t launch_pencils_dir<float, -1>(const PencilArgs<float>&, int, hipStream_t);
+extern template hipError_t launch_pencils_dir<float, +1>(const PencilArgs<float>&, int, hipStream_t);
+extern template hipError_t launch_pencils_dir<double, -1>(const PencilArgs<double>&, int, hipStream_t);
+extern template hipError_t launch_pencils_dir<double, +1>(const PencilArgs<double>&, int, hipStream_t);
+
+template <typename T>
+hipError_t launch_pencils(const PencilArgs<T>& a, int n, int dir, hipStream_t s) {
+    return dir < 0 ? launch_pencils_dir<T, -1>(a, n, s) : launch_pencils_dir<T, +1>(a, n, s);
+}
+template hipError_t launch_pencils<float>(const PencilArgs<float>&, int, int, hipStream_t);
+template hipError_t launch_pencils<double>(const PencilArgs<double>&, int, int, hipStream_t);
+
 
 }  // namespace gpf

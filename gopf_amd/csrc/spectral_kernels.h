@@ -154,10 +154,11 @@ __device__ __forceinline__ int xcd_tile(int b, int nblk) {
     return x * q + min(x, r) + (b >> 3);
 }
 
+// tile of block b of a launch of nblk blocks (a grouped launch passes a class's own range)
 template <typename T>
-__device__ __forceinline__ TileInfo decode_tile(const PencilArgs<T>& a) {
+__device__ __forceinline__ TileInfo decode_tile(const PencilArgs<T>& a, int b, int nblk) {
     TileInfo ti;
-    int tile = xcd_tile(blockIdx.x, gridDim.x);
+    int tile = xcd_tile(b, nblk);
     const bool s1 = tile >= a.tiles0;
     ti.s = s1 ? 1 : 0;
     if (s1) tile -= a.tiles0;
@@ -286,7 +287,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencils(PencilArgs<T> a) {
     constexpr int E = LineShape<N, V>::E;
     const int n = (N > 0) ? N : a.n;
     const int TT = (N > 0) ? LineShape<N, V>::T : n;
-    const TileInfo ti = decode_tile(a);
+    const TileInfo ti = decode_tile(a, blockIdx.x, gridDim.x);
     const LineMap lm = line_map(ti.es == 1, a.TW, TT, n);
 #pragma unroll
     for (int q = 0; q < kMaxArr; ++q) {
@@ -324,6 +325,43 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencils(PencilArgs<T> a) {
                 }
             }
         }
+    }
+}
+
+// Bluestein pencils: a length-n line (n at run time) is multiplied by the chirp, zero-padded
+// to M, transformed forward, multiplied by the chirp filter's spectrum, transformed back,
+// and multiplied by the chirp again; positions p >= n of the M-point line are discarded.
+template <typename T, int M>
+__global__ void __launch_bounds__(kMaxBlock) k_bluestein(BluesteinArgs<T> b) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
+    constexpr int V = plain_variant<M>();
+    constexpr int E = LineShape<M, V>::E;
+    constexpr int TT = LineShape<M, V>::T;
+    const PencilArgs<T>& a = b.p;
+    const int n = a.n;
+    const TileInfo ti = decode_tile(a, blockIdx.x, gridDim.x);
+    const LineMap lm = line_map(ti.es == 1, a.TW, TT, M);
+    const bool valid = lm.l < ti.nvalid;
+    const long base = ti.base + (long)lm.l * ti.sb;
+    cpx<T>* out = pick(a.arr[0][0], a.arr[0][1], ti.s);
+    cpx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int p = lm.t + TT * e;
+        cpx<T> x = mk<T>(T(0), T(0));
+        if (valid && p < n) x = mul(out[base + (long)p * ti.es], b.chirp[p]);
+        v[e] = x;
+    }
+    fft_canon<T, M, -1, V>(v, lds + lm.off, lm.es, lm.t, a.tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = mul(v[e], b.filt[lm.t + TT * e]);
+    fft_canon<T, M, +1, V>(v, lds + lm.off, lm.es, lm.t, a.tw);
+    if (!valid) return;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int p = lm.t + TT * e;
+        if (p < n) out[base + (long)p * ti.es] = mul(v[e], b.chirp[p]);
     }
 }
 
@@ -400,15 +438,15 @@ __device__ __forceinline__ void update_multipliers(const PencilUpdateArgs<T>& u,
 // MON: the energy monitor's partial sums are compiled in (a separate instantiation, so
 // that the common path does not carry their registers)
 template <typename T, int N, bool MON>
-__global__ void __launch_bounds__(kMaxBlock) k_pencil_update(PencilUpdateArgs<T> u) {
-    extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void pencil_update_body(const PencilUpdateArgs<T>& u, int blk, int nblk,
+                                                   unsigned char* smem) {
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
     constexpr int V = 1;  // low-register schedule: two lines and the multipliers stay in registers
     constexpr int E = LineShape<N, V>::E;
     const PencilArgs<T>& a = u.p;
     const int n = (N > 0) ? N : a.n;
     const int TT = (N > 0) ? LineShape<N, V>::T : n;
-    const TileInfo ti = decode_tile(a);
+    const TileInfo ti = decode_tile(a, blk, nblk);
     const LineMap lm = line_map(ti.es == 1, a.TW, TT, n);
     const bool valid = lm.l < ti.nvalid;
     const long base = ti.base + (long)lm.l * ti.sb;
@@ -429,8 +467,10 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencil_update(PencilUpdateArgs<T>
     const int nv = (vary == 0) ? u.g.nx : (vary == 1) ? u.g.ny : u.g.nz;
     const double invv = (vary == 0) ? inx : (vary == 1) ? iny : inz;
     const double wk = (k0 == 0 || 2 * k0 == u.g.nx) ? 1.0 : 2.0;  // Hermitian weight of the kx plane
-    // batched sims: this line's sim selects its coefficients and mean
-    const int sim = (ti.sim_src == 0) ? ti.ia : (ti.sim_src == 1) ? bl : -1;
+    // batched sims: this line's sim selects its coefficients and mean.  A line past the end
+    // of a tile of the per-sim segment (bl >= nb = number of sims) takes the tile's first
+    // sim, so that its discarded multipliers read coefficients inside the tables
+    const int sim = (ti.sim_src == 0) ? ti.ia : (ti.sim_src == 1) ? (valid ? bl : ti.b0) : -1;
     const double* dsim = (sim >= 0) ? u.denom_sim + sim * kMaxKTerms : nullptr;
     const double* lsim = (sim >= 0) ? u.lin_sim + sim * kMaxKTerms : nullptr;
     double* mean = (u.mean && sim >= 0) ? u.mean + sim : u.mean;
@@ -483,11 +523,17 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencil_update(PencilUpdateArgs<T>
         // the state is FFT(c) / N: |FFT(c)|^2 = |S|^2 / scale^2
         double* red = reinterpret_cast<double*>(lds + lds_elems(n, a.TW));
         const double tot = block_sum(eg, red);
-        if (threadIdx.x == 0) u.egrad_partial[blockIdx.x] = tot / (u.scale * u.scale);
+        if (threadIdx.x == 0) u.egrad_partial[blk] = tot / (u.scale * u.scale);
     }
     // 4) the next step's row-pass input: the new state inverse-transformed along this axis
     line_fft<T, N, +1, V>(c, lds + lm.off, lm.es, n, lm.t, a.tw);
     store_line<T, E>(c, warr, base, ti.es, lm.t, TT, valid);
+}
+
+template <typename T, int N, bool MON>
+__global__ void __launch_bounds__(kMaxBlock) k_pencil_update(PencilUpdateArgs<T> u) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    pencil_update_body<T, N, MON>(u, blockIdx.x, gridDim.x, smem);
 }
 
 // x-rows: t fastest; the half spectrum split needs Z(k) and Z(n-k) from other threads
@@ -975,15 +1021,14 @@ __device__ __forceinline__ void rows_pointwise(cpx<T>* v, const T* gp, const Row
 
 // V: transform schedule (0 plain, 1 low-register); MON: monitor partial sums compiled in
 template <typename T, int N, int V, bool MON>
-__global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
-    extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void rows_fused_body(const RowFusedArgs<T>& a, int blk, unsigned char* smem) {
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
     constexpr int E = LineShape<N, V>::E;
     const int n = (N > 0) ? N : a.nx;
     const int TT = (N > 0) ? LineShape<N, V>::T : n;
     const LineMap lm = line_map(true, a.TW, TT, n);
     // this line carries rows r1 and r1 + 1
-    const long r1 = 2 * ((long)blockIdx.x * a.TW + lm.l);
+    const long r1 = 2 * ((long)blk * a.TW + lm.l);
     const bool v1 = r1 < a.rows, v2 = r1 + 1 < a.rows;
     const long sim = (a.rows_per_sim > 0 && v1) ? r1 / a.rows_per_sim : -1;
     const double mu = a.mean ? a.mean[sim >= 0 ? sim : 0] : 0.0;
@@ -1024,8 +1069,8 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
         const double b = block_sum(sb.get(), red);
         const double ms = block_sum(sm.get(), red);
         if (threadIdx.x == 0) {
-            a.bulk_partial[blockIdx.x] = b;
-            a.mass_partial[blockIdx.x] = ms;
+            a.bulk_partial[blk] = b;
+            a.mass_partial[blk] = ms;
         }
     }
     line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, twl);
@@ -1036,12 +1081,79 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
                           v2 ? a.g_nyq + 1 : nullptr);
 }
 
+template <typename T, int N, int V, bool MON>
+__global__ void __launch_bounds__(kMaxBlock) k_rows_fused(RowFusedArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    rows_fused_body<T, N, V, MON>(a, blockIdx.x, smem);
+}
+
+// ---------------------------------------------------------------------------
+// grouped launches of a batch: one launch advances every size class.  The classes' block
+// ranges follow one another, each starting at a multiple of 8 so that the XCD-aware tile
+// order of a class matches the XCD of its blocks (block b runs on XCD b mod 8); blocks past
+// a class's own count (the padding) exit at once.  A class's arguments come from a device
+// array, read uniformly by the block; its compiled length is selected by a switch.
+// ---------------------------------------------------------------------------
+#define GPF_GROUP_LENGTHS(X) X(64) X(128) X(256) X(512) X(1024)
+
+// class of block b and b's index inside the class (unrolled selects: the argument struct is
+// never indexed dynamically)
+template <typename T>
+__device__ __forceinline__ int group_class(const BatchGroupArgs<T>& g, int b, int& local, int& count) {
+    int c = 0, first = 0, cnt = g.blocks[0];
+#pragma unroll
+    for (int k = 1; k < kMaxGroup; ++k)
+        if (k < g.ncls && b >= g.first[k]) {
+            c = k;
+            first = g.first[k];
+            cnt = g.blocks[k];
+        }
+    local = b - first;
+    count = cnt;
+    return c;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kMaxBlock) k_rows_fused_group(BatchGroupArgs<T> g) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    int blk, cnt;
+    const int c = group_class(g, blockIdx.x, blk, cnt);
+    if (blk >= cnt) return;
+    const RowFusedArgs<T> a = g.rows[c];
+    switch (a.nx * 2 + a.variant) {
+#define GPF_G(NN)                                                                           \
+    case NN * 2 + 1: rows_fused_body<T, NN, 1, false>(a, blk, smem); break;                 \
+    case NN * 2:                                                                           \
+        if constexpr (plain_variant<NN>() == 0) rows_fused_body<T, NN, 0, false>(a, blk, smem); \
+        break;
+        GPF_GROUP_LENGTHS(GPF_G)
+#undef GPF_G
+        default: break;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kMaxBlock) k_pencil_update_group(BatchGroupArgs<T> g) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    int blk, cnt;
+    const int c = group_class(g, blockIdx.x, blk, cnt);
+    if (blk >= cnt) return;
+    // read in place (a copy of the argument struct would live in scratch memory)
+    const PencilUpdateArgs<T>& u = g.upd[c];
+    switch (u.p.n) {
+#define GPF_G(NN) \
+    case NN: pencil_update_body<T, NN, false>(u, blk, cnt, smem); break;
+        GPF_GROUP_LENGTHS(GPF_G)
+#undef GPF_G
+        default: break;
+    }
+}
+
 // v: 0 = plain pass, 1 = update pass
 int threads_per_line(int n, int v = 0) {
     switch (n) {
 #define GPF_T(NN) case NN: return (v || plain_variant<NN>()) ? Sched<NN, 1>::T : Sched<NN>::T;
-        GPF_T(2) GPF_T(4) GPF_T(8) GPF_T(16) GPF_T(32) GPF_T(64) GPF_T(128) GPF_T(256) GPF_T(512)
-        GPF_T(1024) GPF_T(12) GPF_T(24) GPF_T(48) GPF_T(96) GPF_T(192) GPF_T(384) GPF_T(768)
+        GPF_FAST_LENGTHS(GPF_T)
 #undef GPF_T
         default: return n;  // brute-force DFT: one thread per output
     }
@@ -1051,8 +1163,7 @@ template <typename F>
 hipError_t dispatch_size(int n, F&& f) {
     switch (n) {
 #define GPF_D(NN) case NN: return f(std::integral_constant<int, NN>{});
-        GPF_D(2) GPF_D(4) GPF_D(8) GPF_D(16) GPF_D(32) GPF_D(64) GPF_D(128) GPF_D(256) GPF_D(512)
-        GPF_D(1024) GPF_D(12) GPF_D(24) GPF_D(48) GPF_D(96) GPF_D(192) GPF_D(384) GPF_D(768)
+        GPF_FAST_LENGTHS(GPF_D)
 #undef GPF_D
         default:
             if (n >= 1 && n <= 64) return f(std::integral_constant<int, 0>{});
@@ -1074,8 +1185,10 @@ hipError_t launch_k(K kernel, dim3 grid, dim3 block, size_t smem, hipStream_t s,
 
 }  // namespace
 
-template <typename T>
-hipError_t launch_pencils(const PencilArgs<T>& a0, int n, int dir, hipStream_t s) {
+// the pencils of one direction (the two directions are instantiated in separate units)
+template <typename T, int DIR>
+hipError_t launch_pencils_dir(const PencilArgs<T>& a0, int n, hipStream_t s) {
+    constexpr int dir = DIR;
     PencilArgs<T> a = a0;
     a.n = n;
     if (a.lpt != 1 && a.lpt != 2) return hipErrorInvalidValue;
@@ -1088,19 +1201,55 @@ hipError_t launch_pencils(const PencilArgs<T>& a0, int n, int dir, hipStream_t s
         auto go = [&](auto VC) -> hipError_t {
             constexpr int VV = decltype(VC)::value;
             if constexpr (sizeof(T) == 4) {
-                if (a.lpt == 2) {
-                    if (dir < 0) return launch_k(k_pencils<T, NN, -1, VV, 2>, dim3(blocks), dim3(thr), smem, s, a);
-                    return launch_k(k_pencils<T, NN, +1, VV, 2>, dim3(blocks), dim3(thr), smem, s, a);
-                }
+                if (a.lpt == 2) return launch_k(k_pencils<T, NN, DIR, VV, 2>, dim3(blocks), dim3(thr), smem, s, a);
             }
-            if (dir < 0) return launch_k(k_pencils<T, NN, -1, VV, 1>, dim3(blocks), dim3(thr), smem, s, a);
-            return launch_k(k_pencils<T, NN, +1, VV, 1>, dim3(blocks), dim3(thr), smem, s, a);
+            return launch_k(k_pencils<T, NN, DIR, VV, 1>, dim3(blocks), dim3(thr), smem, s, a);
         };
         if constexpr (plain_variant<NN>() == 0) {
             if (!a.variant) return go(std::integral_constant<int, 0>{});
         }
         return go(std::integral_constant<int, 1>{});
     });
+}
+
+// a grouped launch: the caller has set first/blocks/ncls, the per-class argument arrays
+// (device memory, with each class's lines per block set for `threads` threads) and the
+// dynamic LDS size smem, the largest any class needs
+template <typename T>
+hipError_t launch_rows_fused_group(const BatchGroupArgs<T>& g, int threads, size_t smem, hipStream_t s) {
+    if (g.ncls < 1 || g.ncls > kMaxGroup || threads > kMaxBlock) return hipErrorInvalidValue;
+    return launch_k(k_rows_fused_group<T>, dim3(g.first[g.ncls]), dim3(threads), smem, s, g);
+}
+
+template <typename T>
+hipError_t launch_pencil_update_group(const BatchGroupArgs<T>& g, int threads, size_t smem, hipStream_t s) {
+    if (g.ncls < 1 || g.ncls > kMaxGroup || threads > kMaxBlock) return hipErrorInvalidValue;
+    return launch_k(k_pencil_update_group<T>, dim3(g.first[g.ncls]), dim3(threads), smem, s, g);
+}
+
+
+template <typename T>
+hipError_t launch_bluestein(const BluesteinArgs<T>& b0, int n, hipStream_t s) {
+    BluesteinArgs<T> b = b0;
+    b.p.n = n;
+    b.p.lpt = 1;
+    if (n < 2 || 2 * n - 1 > b.m) return hipErrorInvalidValue;
+    const long blocks = pencil_blocks(b.p.seg, b.p.nseg, b.p.TW, &b.p.tiles0);
+    const size_t smem = lds_elems(b.m, b.p.TW) * sizeof(cpx<T>);
+    auto go = [&](auto MC) -> hipError_t {
+        constexpr int MM = decltype(MC)::value;
+        constexpr int V = plain_variant<MM>();
+        const int thr = b.p.TW * LineShape<MM, V>::T;
+        return launch_k(k_bluestein<T, MM>, dim3(blocks), dim3(thr), smem, s, b);
+    };
+    switch (b.m) {
+        case 128: return go(std::integral_constant<int, 128>{});
+        case 256: return go(std::integral_constant<int, 256>{});
+        case 512: return go(std::integral_constant<int, 512>{});
+        case 1024: return go(std::integral_constant<int, 1024>{});
+        case 2048: return go(std::integral_constant<int, 2048>{});
+        default: return hipErrorInvalidValue;
+    }
 }
 
 template <typename T>

@@ -1,12 +1,9 @@
-// spectral_pencils.hip -- instantiations of k_pencils (every length, direction, schedule, lines per thread).
+// spectral_pencils.hip -- FP32 instantiations of the Bluestein pencils k_bluestein (convolution lengths
+// 128 to 2048); the plain pencils are in spectral_pencils_fwd.hip and spectral_pencils_inv.hip.
 #include "spectral_kernels.h"
 
 namespace gpf {
 
-#define GPF_INST(T) \
-    template hipError_t launch_pencils<T>(const PencilArgs<T>&, int, int, hipStream_t);
-GPF_INST(float)
-GPF_INST(double)
-#undef GPF_INST
+template hipError_t launch_bluestein<float>(const BluesteinArgs<float>&, int, hipStream_t);
 
 }  // namespace gpf

@@ -1,0 +1,9 @@
+// spectral_pencils_inv.hip -- FP32 instantiations of the inverse pencil transforms k_pencils (every
+// length, schedule, lines per thread); a unit of its own so that the families compile in parallel.
+#include "spectral_kernels.h"
+
+namespace gpf {
+
+template hipError_t launch_pencils_dir<float, +1>(const PencilArgs<float>&, int, hipStream_t);
+
+}  // namespace gpf

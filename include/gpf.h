@@ -344,11 +344,19 @@ int gpf_batch_num_sims(const gpf_batch* b);
 int gpf_batch_set_scalar(gpf_batch* b, int sim, const char* name, double value);
 int gpf_batch_set_field(gpf_batch* b, int sim, const double* data, long n);
 int gpf_batch_get_field(gpf_batch* b, int sim, double* out, long n);
-/* nsteps Euler steps of every sim; blocks of 10 steps run as one replayed HIP graph */
+/* nsteps Euler steps of every sim; blocks of 10 steps run as one replayed HIP graph.  With
+ * gpf_options.autotune, the first call times the launch layouts on scratch copies: lines per
+ * block and transform schedule of each class, and the packing -- one row launch and one
+ * update launch per class, or one grouped row launch and one grouped update launch that
+ * advance every class (classes of power-of-two sides 64..1024) -- and keeps the fastest. */
 int gpf_batch_propagate(gpf_batch* b, int nsteps);
 int gpf_batch_synchronize(gpf_batch* b);
-/* lines per block of the row and update pass of each class (2 values per class) */
+/* the launch layout: per class (rows lines per block, update lines per block, rows transform
+ * schedule), then the threads per block of the grouped launches, 0 when each class launches
+ * on its own; returns the number of values (3 per class + 1) */
 int gpf_batch_tuning(gpf_batch* b, int* tws, int maxn);
+/* applies a layout in gpf_batch_tuning's format instead of the tuner (n = 3 per class + 1) */
+int gpf_batch_set_layout(gpf_batch* b, const int* v, int n);
 
 /* ---------------- pf.Vandeven ---------------- */
 

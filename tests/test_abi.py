@@ -84,3 +84,12 @@ def test_fft_rejects_a_buffer_of_the_wrong_length():
         assert ei.value.code == G.GPF_EINVAL
     finally:
         L.gpf_fft_destroy(h)
+
+
+def test_fft_lengths_beyond_the_plans_are_refused():
+    """Lengths above 1024 without a compiled schedule (here the prime 1031) have no plan;
+    NewFFTW's counterpart refuses them on the host, before any device call."""
+    import gopf_amd as G
+    with pytest.raises(G.GpfError) as ei:
+        G.FourierTransform([1031, 4])
+    assert ei.value.code == G.GPF_EINVAL and "1031" in str(ei.value)

@@ -20,8 +20,14 @@ def test_pass_byte_model_and_committed_traffic_agree():
     arr32 = (256 // 2 + 1) * 256 * 256 * 8
     assert sum(bench.pass_bytes(k, 256, 256, 256, 4) for k in
                ("rows_fused", "pencils_fwd", "pencil_update", "pencils_inv")) == 10 * arr32
+    layout = bench.stored_layout("cahn_hilliard_3d_256", "fp32")
+    assert layout is not None
     for k in ("rows_fused", "pencils_fwd", "pencil_update", "pencils_inv"):
-        t = bench.pmc_traffic("cahn_hilliard_3d_256_fp32", k)
-        assert t is not None
+        t, why = bench.pmc_traffic("cahn_hilliard_3d_256_fp32", k, layout)
+        assert t is not None, why
         # measured HBM bytes within 10 % of the algorithmic bytes
         assert abs(t / bench.pass_bytes(k, 256, 256, 256, 4) - 1.0) < 0.10, k
+    # counters measured under another launch layout are refused
+    other = dict(layout, rows_fused=layout["rows_fused"] * 2)
+    t, why = bench.pmc_traffic("cahn_hilliard_3d_256_fp32", "rows_fused", other)
+    assert t is None and "layout" in why

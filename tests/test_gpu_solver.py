@@ -417,3 +417,52 @@ def test_fused_ch_sizes_match_oracle(oracle_lib, dims, steps, precision):
         assert maxrel(got, ref) <= 1e-10
     else:
         assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-5
+
+
+@pytest.mark.parametrize("dims", [[80, 100], [160, 320], [16, 2048], [2048, 16], [112, 224], [80, 80, 80]])
+def test_ch_radix5_radix7_and_2048_lengths_match_oracle(oracle_lib, dims):
+    """The fused Cahn-Hilliard step on grids whose transform lengths use the radix-5 and
+    radix-7 stages and the 2^11 schedule (rows, pencils and the update pass), 10 FP64 steps
+    against the oracle (euler.go:16-47)."""
+    import gopf_amd as G
+    from gopf_amd import workloads as W
+    from oracle import parser as P
+    from oracle.engine import OracleSolver
+    print("ch lengths", dims, flush=True)
+    n = int(np.prod(dims))
+    c0 = W.ch_initial(3, n)
+    m = G.Model()
+    m.AddField("conc", c0)
+    om = P.Model()
+    om.add_field("conc", n)
+    for k, v in W.ch_scalars().items():
+        m.AddScalar(k, v)
+        om.add_scalar(k, v)
+    m.AddEquation(W.CH_EQUATION)
+    om.add_equation(W.CH_EQUATION)
+    m.Init()
+    om.init()
+    s = G.Solver(m, dims, 0.1, precision="fp64")
+    assert s.fused
+    s.StepperPropagate(10)
+    o = OracleSolver(om, dims, {"conc": c0}, 0.1)
+    o.stepper_propagate(10)
+    ref = o.field("conc").real
+    err = np.abs(s.Field(0).real - ref).max() / np.abs(ref).max()
+    assert err <= 1e-10, (dims, err)
+
+
+def test_odd_fastest_extent_is_refused():
+    """The device stores real fields as half spectra of the fastest axis, packed two rows per
+    transform; an odd fastest extent (a 9 x 15 grid) is refused with a message."""
+    import gopf_amd as G
+    from gopf_amd import workloads as W
+    m = G.Model()
+    m.AddField("conc", W.ch_initial(3, 135))
+    for k, v in W.ch_scalars().items():
+        m.AddScalar(k, v)
+    m.AddEquation(W.CH_EQUATION)
+    m.Init()
+    with pytest.raises(G.GpfError) as ei:
+        G.Solver(m, [9, 15], 0.1)
+    assert ei.value.code == G.GPF_EINVAL and "even" in str(ei.value)
