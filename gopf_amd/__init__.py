@@ -151,6 +151,7 @@ def _declare(L) -> None:
         "gpf_batch_propagate": (C.c_int, [_vp, C.c_int]),
         "gpf_batch_synchronize": (C.c_int, [_vp]),
         "gpf_batch_tuning": (C.c_int, [_vp, _ip, C.c_int]),
+        "gpf_batch_set_layout": (C.c_int, [_vp, _ip, C.c_int]),
         "gpf_fft_create": (C.c_int, [_ip, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
         "gpf_fft_destroy": (None, [_vp]),
         "gpf_fft_forward": (C.c_int, [_vp, _dp, C.c_long]),
@@ -765,10 +766,24 @@ class Batch:
     def Synchronize(self) -> None:
         _check(lib().gpf_batch_synchronize(self._h))
 
-    def Tuning(self) -> List[int]:
+    def SetLayout(self, layout: Dict[str, object]) -> None:
+        """Applies a layout returned by Tuning() instead of the tuner."""
+        v = []
+        for c in layout["classes"]:
+            v += [int(c["rows_tw"]), int(c["update_tw"]), int(c["rows_schedule"])]
+        v.append(int(layout["grouped_threads"]))
+        _check(lib().gpf_batch_set_layout(self._h, _ints(v), len(v)))
+
+    def Tuning(self) -> Dict[str, object]:
+        """The launch layout: per class the row and update lines per block and the row
+        transform schedule; grouped_threads > 0 when one launch per pass advances every class."""
         v = (C.c_int * 64)()
         k = lib().gpf_batch_tuning(self._h, v, 64)
-        return [v[i] for i in range(min(k, 64))]
+        vals = [v[i] for i in range(min(k, 64))]
+        ncls = (len(vals) - 1) // 3
+        return {"classes": [{"rows_tw": vals[3 * c], "update_tw": vals[3 * c + 1], "rows_schedule": vals[3 * c + 2]}
+                            for c in range(ncls)],
+                "grouped_threads": vals[-1] if vals else 0}
 
 
 def Reduce(data_ptr: int, n: int, precision: str = "fp64", device: int = 0, stream: int = 0) -> Dict[str, float]:
@@ -851,5 +866,5 @@ def cahn_hilliard_model(n: int, M: float = 1.0, kappa: float = 1.0, data=None) -
 CH_BULK_POLY = [0.0, 0.0, 1.0, -2.0, 1.0]   # f(c) = c^2 - 2 c^3 + c^4 = c^2 (1 - c)^2
 
 
-from .fileio import (CsvIO, FieldDB, Float64IO, LoadCSV, LoadFloat64, NewFloat64IO, NewUint8IO,  # noqa: E402
+from .fileio import (BatchFieldDB, CsvIO, FieldDB, Float64IO, LoadCSV, LoadFloat64, NewFloat64IO, NewUint8IO,  # noqa: E402
                      SaveCsv, SaveFloat64, Uint8IO, CreateXDMF, WriteXDMF)

@@ -36,6 +36,7 @@ public:
     virtual void propagate(int nsteps) = 0;
     virtual void synchronize() = 0;
     virtual int tuning(int* out, int maxn) = 0;
+    virtual void set_layout(const int* v, int n) = 0;
 };
 
 BatchBase* make_batch(const Model& m, int ncls, const int* shapes, const int* counts, double dt, const gpf_options& o);

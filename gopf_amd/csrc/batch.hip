@@ -12,6 +12,7 @@
 // and replayed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <map>
@@ -68,9 +69,11 @@ int pick_tw(int n, int variant) {
     return tw > 64 ? 64 : tw;
 }
 
+// a block of tw lines fits: threads, and LDS for the exchange lines, the row pass's copy of
+// the twiddle table and the monitor's partial sums (160 KiB per workgroup on gfx950)
 bool tw_ok(int n, int tw, size_t elem, int variant) {
-    return tw * fft_threads_per_line(n, variant) <= kMaxBlockThreads &&
-           (size_t)(n + n / 16 + 1) * (tw + 1) * elem <= 150 * 1024;
+    const size_t lds = ((size_t)(n + n / 16 + 1) * (tw + 1) + n) * elem + (size_t)kMaxBlockThreads * sizeof(double);
+    return tw * fft_threads_per_line(n, variant) <= kMaxBlockThreads && lds <= 160 * 1024;
 }
 
 }  // namespace
@@ -225,15 +228,51 @@ public:
 
     void synchronize() override { BCHK(hipStreamSynchronize(st_)); }
 
+    // per class: rows lines per block, update lines per block, rows schedule; then the
+    // threads per block of the grouped launches (0: one launch per class and pass)
     int tuning(int* out, int maxn) override {
         int k = 0;
         for (auto& c : cls_) {
-            if (k < maxn) out[k] = c->tw_rows;
-            ++k;
-            if (k < maxn) out[k] = c->tw_upd;
-            ++k;
+            for (int v : {c->tw_rows, c->tw_upd, c->rows_var}) {
+                if (k < maxn) out[k] = v;
+                ++k;
+            }
         }
+        if (k < maxn) out[k] = grouped_ ? gthreads_ : 0;
+        ++k;
         return k;
+    }
+
+    // a layout in tuning()'s format, applied instead of the tuner
+    void set_layout(const int* v, int n) override {
+        const int nc = (int)cls_.size();
+        if (n != 3 * nc + 1) throw BatchError(GPF_EINVAL, "a batch layout has 3 values per class and 1 more");
+        if (dirty_) compile();
+        for (int k = 0; k < nc; ++k) {
+            BatchClass<T>& c = *cls_[k];
+            const int tr = v[3 * k], tu = v[3 * k + 1], var = v[3 * k + 2];
+            auto pow2 = [](int x) { return x >= 1 && (x & (x - 1)) == 0; };
+            if (!pow2(tr) || !pow2(tu) || (var != 0 && var != 1) || !tw_ok(c.nx, tr, sizeof(cpx<T>), var) ||
+                !tw_ok(c.ny, tu, sizeof(cpx<T>), 1))
+                throw BatchError(GPF_EINVAL, "batch layout not admissible for class " + std::to_string(k));
+        }
+        const int th = v[3 * nc];
+        if (th != 0 && (!group_ok() || (th != 128 && th != 256 && th != 512)))
+            throw BatchError(GPF_EINVAL, "grouped launches need power-of-two sides 64..1024 and 128, 256 or 512 threads");
+        BCHK(hipStreamSynchronize(st_));
+        for (int k = 0; k < nc; ++k) {
+            cls_[k]->tw_rows = v[3 * k];
+            cls_[k]->tw_upd = v[3 * k + 1];
+            cls_[k]->rows_var = v[3 * k + 2];
+        }
+        grouped_ = false;
+        if (th != 0) {
+            if (!build_group(th, own_S(), own_W(), own_mean()))
+                throw BatchError(GPF_EINVAL, "grouped launches do not fit these classes at this block size");
+            grouped_ = true;
+        }
+        tuned_ = true;
+        drop_graphs();
     }
 
 private:
@@ -304,6 +343,10 @@ private:
         }
         drop_graphs();
         dirty_ = false;
+        // the coefficient tables were reallocated: the grouped launches' argument arrays point
+        // at the new ones
+        if (grouped_ && !build_group(gthreads_, own_S(), own_W(), own_mean()))
+            throw BatchError(GPF_EINVAL, "grouped launch layout no longer fits");
     }
 
     // y transform of one sim's region (main plane and x-Nyquist column): dst = F_y^dir(src),
@@ -381,11 +424,91 @@ private:
     }
 
     void enqueue_step() {
+        if (grouped_) {
+            if (shape_.has_src) BCHK(launch_rows_fused_group<T>(grows_, gthreads_, grows_smem_, st_));
+            BCHK(launch_pencil_update_group<T>(gupd_, gthreads_, gupd_smem_, st_));
+            return;
+        }
         for (auto& cp : cls_) {
             BatchClass<T>& c = *cp;
             if (shape_.has_src) BCHK(launch_rows_fused<T>(rows_args(c, c.W.p, c.mean.p, c.tw_rows, c.rows_var), st_));
             BCHK(launch_pencil_update<T>(upd_args(c, c.S.p, c.W.p, c.mean.p, c.tw_upd), c.ny, st_));
         }
+    }
+
+    // every class has a length the grouped launches are compiled for
+    bool group_ok() const {
+        if ((int)cls_.size() > kMaxGroup) return false;
+        for (auto& c : cls_)
+            if (!group_length(c->nx) || !group_length(c->ny)) return false;
+        return true;
+    }
+
+    // the grouped launches of one step for `threads` threads per block over the arrays S[c],
+    // W[c] and means mu[c] of each class; false when a class's lines per block would not fit
+    bool build_group(int threads, const std::vector<cpx<T>*>& S, const std::vector<cpx<T>*>& W,
+                     const std::vector<double*>& mu) {
+        const int nc = (int)cls_.size();
+        std::vector<RowFusedArgs<T>> ra(nc);
+        std::vector<PencilUpdateArgs<T>> ua(nc);
+        BatchGroupArgs<T> gr, gu;
+        gr.ncls = gu.ncls = nc;
+        size_t sr = 0, su = 0;
+        int fr = 0, fu = 0;
+        const size_t el = sizeof(cpx<T>);
+        for (int k = 0; k < nc; ++k) {
+            BatchClass<T>& c = *cls_[k];
+            const int tr = threads / fft_threads_per_line(c.nx, c.rows_var);
+            const int tu = threads / fft_threads_per_line(c.ny, 1);
+            if (tr < 1 || tu < 1 || tr * fft_threads_per_line(c.nx, c.rows_var) != threads ||
+                tu * fft_threads_per_line(c.ny, 1) != threads || !tw_ok(c.nx, tr, el, c.rows_var) || !tw_ok(c.ny, tu, el, 1))
+                return false;
+            ra[k] = rows_args(c, W[k], mu[k], tr, c.rows_var);
+            ua[k] = upd_args(c, S[k], W[k], mu[k], tu);
+            ua[k].p.n = c.ny;
+            ua[k].p.lpt = 1;
+            const long br = rows_fused_blocks(c.rows, tr);
+            const long bu = pencil_blocks(ua[k].p.seg, ua[k].p.nseg, tu, &ua[k].p.tiles0);
+            gr.first[k] = fr;
+            gr.blocks[k] = (int)br;
+            gu.first[k] = fu;
+            gu.blocks[k] = (int)bu;
+            fr += (int)((br + 7) / 8 * 8);
+            fu += (int)((bu + 7) / 8 * 8);
+            sr = std::max(sr, ((size_t)(c.nx + c.nx / 16 + 1) * (tr + 1) + c.nx) * el);
+            su = std::max(su, (size_t)(c.ny + c.ny / 16 + 1) * (tu + 1) * el);
+        }
+        gr.first[nc] = fr;
+        gu.first[nc] = fu;
+        growsbuf_.alloc(nc);
+        gupdbuf_.alloc(nc);
+        BCHK(hipMemcpyAsync(growsbuf_.p, ra.data(), nc * sizeof(RowFusedArgs<T>), hipMemcpyHostToDevice, st_));
+        BCHK(hipMemcpyAsync(gupdbuf_.p, ua.data(), nc * sizeof(PencilUpdateArgs<T>), hipMemcpyHostToDevice, st_));
+        BCHK(hipStreamSynchronize(st_));
+        gr.rows = growsbuf_.p;
+        gu.upd = gupdbuf_.p;
+        grows_ = gr;
+        gupd_ = gu;
+        grows_smem_ = sr;
+        gupd_smem_ = su;
+        gthreads_ = threads;
+        return true;
+    }
+
+    std::vector<cpx<T>*> own_S() const {
+        std::vector<cpx<T>*> v;
+        for (auto& c : cls_) v.push_back(c->S.p);
+        return v;
+    }
+    std::vector<cpx<T>*> own_W() const {
+        std::vector<cpx<T>*> v;
+        for (auto& c : cls_) v.push_back(c->W.p);
+        return v;
+    }
+    std::vector<double*> own_mean() const {
+        std::vector<double*> v;
+        for (auto& c : cls_) v.push_back(c->mean.p);
+        return v;
     }
 
     void capture() {
@@ -435,6 +558,63 @@ private:
             }
             BCHK(hipStreamSynchronize(st_));
         }
+        // packing: per-class launches (2 per class and step) against grouped launches (2 per
+        // step) of 128, 256 or 512 threads per block, timed over a whole step on scratch copies
+        grouped_ = false;
+        if (group_ok()) {
+            std::vector<DBuf<cpx<T>>> Sb(cls_.size()), Wb(cls_.size());
+            std::vector<DBuf<double>> mb(cls_.size());
+            std::vector<cpx<T>*> Sp, Wp;
+            std::vector<double*> mp;
+            for (size_t k = 0; k < cls_.size(); ++k) {
+                BatchClass<T>& c = *cls_[k];
+                Sb[k].alloc(c.nk);
+                Wb[k].alloc(c.nk);
+                mb[k].alloc(c.nsim);
+                BCHK(hipMemcpyAsync(Sb[k].p, c.S.p, c.nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+                BCHK(hipMemcpyAsync(Wb[k].p, c.W.p, c.nk * sizeof(cpx<T>), hipMemcpyDeviceToDevice, st_));
+                BCHK(hipMemcpyAsync(mb[k].p, c.mean.p, c.nsim * sizeof(double), hipMemcpyDeviceToDevice, st_));
+                Sp.push_back(Sb[k].p);
+                Wp.push_back(Wb[k].p);
+                mp.push_back(mb[k].p);
+            }
+            auto time_step = [&](bool grouped) {
+                auto one = [&] {
+                    if (grouped) {
+                        if (shape_.has_src) BCHK(launch_rows_fused_group<T>(grows_, gthreads_, grows_smem_, st_));
+                        BCHK(launch_pencil_update_group<T>(gupd_, gthreads_, gupd_smem_, st_));
+                    } else {
+                        for (size_t k = 0; k < cls_.size(); ++k) {
+                            BatchClass<T>& c = *cls_[k];
+                            if (shape_.has_src)
+                                BCHK(launch_rows_fused<T>(rows_args(c, Wp[k], mp[k], c.tw_rows, c.rows_var), st_));
+                            BCHK(launch_pencil_update<T>(upd_args(c, Sp[k], Wp[k], mp[k], c.tw_upd), c.ny, st_));
+                        }
+                    }
+                };
+                one();
+                BCHK(hipEventRecord(e0, st_));
+                for (int r = 0; r < 5; ++r) one();
+                BCHK(hipEventRecord(e1, st_));
+                BCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                BCHK(hipEventElapsedTime(&ms, e0, e1));
+                return ms;
+            };
+            float best = time_step(false);
+            int best_threads = 0;
+            for (int th : {128, 256, 512}) {
+                if (!build_group(th, Sp, Wp, mp)) continue;
+                const float t = time_step(true);
+                if (t < best) { best = t; best_threads = th; }
+            }
+            BCHK(hipStreamSynchronize(st_));
+            if (best_threads > 0) {
+                if (!build_group(best_threads, own_S(), own_W(), own_mean()))
+                    throw BatchError(GPF_EINVAL, "grouped launch layout no longer fits");
+                grouped_ = true;
+            }
+        }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         drop_graphs();
@@ -451,6 +631,13 @@ private:
     int dense_ = 0;  // dense level of the per-sim multiplier rows (make_dense)
     FusedShape shape_;
     hipGraphExec_t graph_exec_ = nullptr;
+    // grouped launches: argument arrays in device memory and the launches' shape
+    bool grouped_ = false;
+    int gthreads_ = 256;
+    DBuf<RowFusedArgs<T>> growsbuf_;
+    DBuf<PencilUpdateArgs<T>> gupdbuf_;
+    BatchGroupArgs<T> grows_, gupd_;
+    size_t grows_smem_ = 0, gupd_smem_ = 0;
 
     void drop_graphs() {
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);

@@ -262,3 +262,53 @@ class FieldDB:
         """LoadLast (database.go:425-436)."""
         (ts,) = self.DB.execute("SELECT MAX(timestep) FROM fields WHERE simID=?", (simID,)).fetchone()
         return self.Load(simID, ts if ts is not None else 0)
+
+
+class BatchFieldDB:
+    """Every sim of a ``Batch`` under its own FieldDB ``simID`` (database.go:67-80).
+
+    The reference runs a sweep as one ``pf.Solver`` per sim, each saving through its own
+    ``FieldDB`` and so under its own random simID.  A database keeps a single positions table
+    for one domain size (database.go:159-197, 344-371), so the sims of one grid shape share a
+    database file here, each with its own simID; ``path`` maps a shape (ny, nx) to that
+    class's file (or is a format string with ``{ny}`` and ``{nx}``).  ``SaveFields(epoch)``
+    stores every sim's field as ``FieldDB.SaveFields`` stores a solver's, in one transaction
+    per sim; ``Attrs(sim, attrs)`` records a sim's parameters as ``SetAttr`` does."""
+
+    def __init__(self, batch, path, field_name: str = "conc"):
+        self.batch = batch
+        self.field_name = field_name
+        self.conns: Dict[Tuple[int, ...], sqlite3.Connection] = {}
+        self.dbs: List[FieldDB] = []
+        for s in range(len(batch)):
+            shape = tuple(int(v) for v in batch.shapes[s])
+            if shape not in self.conns:
+                p = path(shape) if callable(path) else str(path).format(ny=shape[0], nx=shape[1])
+                self.conns[shape] = sqlite3.connect(p)
+            self.dbs.append(FieldDB(self.conns[shape], list(shape)))
+        used = set()
+        for db in self.dbs:
+            db.initialize()
+            # simIDs are random 31-bit integers; no two sims of one sweep share one
+            while db.simID in used:
+                with db.DB:
+                    db.DB.execute("DELETE FROM simIDs WHERE simID=?", (db.simID,))
+                db.simID = 0
+                db.initialize()
+            used.add(db.simID)
+
+    def simID(self, sim: int) -> int:
+        return self.dbs[sim].simID
+
+    def Attrs(self, sim: int, attrs: Dict[str, float]) -> None:
+        self.dbs[sim].SetAttr(attrs)
+
+    def SaveFields(self, epoch: int) -> None:
+        for s, db in enumerate(self.dbs):
+            if not db.positionTableIsPopulated():
+                db.populatePositionsTable()
+            db.insertRealPart(self.field_name, epoch, self.batch.Field(s))
+
+    def close(self) -> None:
+        for c in self.conns.values():
+            c.close()

@@ -245,3 +245,47 @@ def test_fielddb_load(tmp_path):
         assert np.allclose(f[0][1], -0.1 * (step + 1)) and np.allclose(f[1][1], 0.1 * (step + 1))
     last = db.LoadLast(db.simID)
     assert np.allclose(last[1][1], 1.0)
+
+
+class _FakeBatch:
+    """A Batch's host interface (shapes, len, Field) over fixed arrays, so that the FieldDB
+    mapping runs without a device."""
+
+    def __init__(self, classes):
+        self.shapes, self.data = [], []
+        rng = np.random.default_rng(0)
+        for shape, count in classes:
+            for _ in range(count):
+                self.shapes.append(tuple(shape))
+                self.data.append(rng.random(int(np.prod(shape))).astype(np.complex128))
+
+    def __len__(self):
+        return len(self.shapes)
+
+    def Field(self, s):
+        return self.data[s]
+
+
+def test_batch_fielddb_gives_every_sim_its_own_simid(tmp_path):
+    """A sweep stored as the reference stores one solver per sim (database.go:67-80,
+    230-244): every sim under its own simID, sims of one shape in one database (one positions
+    table per domain size), each sim's field loadable by its simID (Load, database.go:376-421)."""
+    b = _FakeBatch([((4, 6), 3), ((8, 4), 2)])
+    bdb = G.BatchFieldDB(b, str(tmp_path / "sweep_{ny}x{nx}.db"))
+    for s in range(len(b)):
+        bdb.Attrs(s, {"M": 1.0 + s, "kappa": 0.5 * s})
+    bdb.SaveFields(7)
+    ids = [bdb.simID(s) for s in range(len(b))]
+    assert len(set(ids)) == len(b) and all(0 < i < 2 ** 31 for i in ids)
+    for s in range(len(b)):
+        db = G.FieldDB(bdb.conns[b.shapes[s]], list(b.shapes[s]))
+        (name, field), = db.Load(ids[s], 7)
+        assert name == "conc" and np.array_equal(field.real, b.data[s].real)
+        (m,) = db.DB.execute("SELECT value FROM simAttributes WHERE simID=? AND key='M'", (ids[s],)).fetchone()
+        assert m == 1.0 + s
+    # one positions table per shape, each holding its grid
+    for shape, conn in bdb.conns.items():
+        (n,) = conn.execute("SELECT COUNT(*) FROM positions").fetchone()
+        assert n == int(np.prod(shape))
+    assert len(bdb.conns) == 2
+    bdb.close()

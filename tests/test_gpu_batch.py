@@ -90,3 +90,53 @@ def test_batch_fp32_close_to_fp64_and_conserves_mass():
         r = b64.Field(s).real
         assert np.linalg.norm(a - r) / np.linalg.norm(r) <= 1e-5, s
         assert abs(a.sum() - inits[s].sum()) <= 1e-9 * inits[s].sum(), s
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_grouped_launches_match_per_class_launches(prec):
+    """One grouped row launch and one grouped update launch per step (every class in the same
+    launch, 128 / 256 / 512 threads per block) against one launch per class and pass, with the
+    same transform schedules: the per-line arithmetic is the same, so the fields must agree
+    bit for bit after 25 steps (two replayed 10-step graphs and 5 direct steps)."""
+    classes = [((64, 64), 3), ((128, 256), 2), ((256, 128), 2)]
+    m = G.Model()
+    m.AddField("conc", n=16)
+    for k, v in W.ch_scalars().items():
+        m.AddScalar(k, v)
+    m.AddEquation(W.CH_EQUATION)
+    m.Init()
+    fields = {}
+    for th in (0, 128, 256, 512):
+        print(f"batch {prec} grouped threads {th}", flush=True)
+        b = G.Batch(m, classes, 0.1, precision=prec, autotune=False)
+        for s in range(len(b)):
+            M, kappa, c0 = W.c3_sim_inputs(s, int(np.prod(b.shapes[s])))
+            for k, v in W.ch_scalars(M, kappa).items():
+                b.SetScalar(s, k, v)
+            b.SetField(s, c0)
+        b.SetLayout({"classes": [{"rows_tw": 8, "update_tw": 8, "rows_schedule": 0}] * len(classes),
+                     "grouped_threads": th})
+        assert b.Tuning()["grouped_threads"] == th
+        b.Propagate(25)
+        fields[th] = [b.Field(s).real for s in range(len(b))]
+        del b
+    for th in (128, 256, 512):
+        for s in range(len(fields[0])):
+            assert np.array_equal(fields[th][s], fields[0][s]), (th, s)
+    # the step moved the fields
+    c0 = W.c3_sim_inputs(0, 64 * 64)[2]
+    assert np.abs(fields[0][0] - c0).max() > 1e-3
+
+
+def test_batch_layout_refusals():
+    m = G.Model()
+    m.AddField("conc", n=16)
+    for k, v in W.ch_scalars().items():
+        m.AddScalar(k, v)
+    m.AddEquation(W.CH_EQUATION)
+    m.Init()
+    b = G.Batch(m, [((96, 96), 2)], 0.1, precision="fp32", autotune=False)
+    with pytest.raises(G.GpfError):   # 96 has no grouped launch
+        b.SetLayout({"classes": [{"rows_tw": 8, "update_tw": 8, "rows_schedule": 0}], "grouped_threads": 256})
+    with pytest.raises(G.GpfError):   # lines per block must be a power of two
+        b.SetLayout({"classes": [{"rows_tw": 6, "update_tw": 8, "rows_schedule": 0}], "grouped_threads": 0})
