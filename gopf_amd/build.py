@@ -16,12 +16,13 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgopf_amd.so")
 # heaviest units first: the pool starts them before the light ones
-SOURCES = ["spectral_pencils_fwd.hip", "spectral_pencils_inv.hip", "spectral_sync.hip", "spectral_r2c.hip",
-           "spectral_c2r.hip", "spectral_update.hip", "spectral_rows.hip", "spectral_pencils_fwd64.hip",
-           "spectral_pencils_inv64.hip", "spectral_sync64.hip", "spectral_r2c64.hip", "spectral_c2r64.hip",
-           "spectral_update64.hip", "spectral_rows64.hip", "spectral_pencils.hip", "spectral_pencils64.hip",
-           "spectral.hip", "engine.hip", "pointwise.hip", "batch.hip", "krylov.hip", "io.hip", "reduce.hip",
-           "model.cpp"]
+SOURCES = ["spectral_ext_rows.hip", "spectral_ext_rows64.hip", "spectral_ext_fwd.hip", "spectral_ext_inv.hip",
+           "spectral_ext_fwd64.hip", "spectral_ext_inv64.hip", "spectral_pencils_fwd.hip", "spectral_pencils_inv.hip",
+           "spectral_sync.hip", "spectral_r2c.hip", "spectral_c2r.hip", "spectral_update.hip", "spectral_rows.hip",
+           "spectral_pencils_fwd64.hip", "spectral_pencils_inv64.hip", "spectral_sync64.hip", "spectral_r2c64.hip",
+           "spectral_c2r64.hip", "spectral_update64.hip", "spectral_rows64.hip", "spectral_group.hip",
+           "spectral_group64.hip", "spectral_pencils.hip", "spectral_pencils64.hip", "spectral.hip", "engine.hip",
+           "pointwise.hip", "batch.hip", "krylov.hip", "io.hip", "reduce.hip", "model.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = ["-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",

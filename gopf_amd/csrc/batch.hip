@@ -62,11 +62,15 @@ std::vector<cpx<T>> twiddles(int n) {
     return w;
 }
 
+// lines per block of an untuned launch: about 256 threads, and the exchange lines plus a
+// twiddle table within 80 KiB of LDS (FP64, 16-byte elements: the largest case)
 int pick_tw(int n, int variant) {
     const int T = fft_threads_per_line(n, variant);
     int tw = 1;
     while (tw * 2 * T <= 256) tw *= 2;
-    return tw > 64 ? 64 : tw;
+    if (tw > 64) tw = 64;
+    while (tw > 1 && ((size_t)(n + n / 16 + 1) * (tw + 1) + n) * 16 > 80 * 1024) tw /= 2;
+    return tw;
 }
 
 // a block of tw lines fits: threads, and LDS for the exchange lines, the row pass's copy of

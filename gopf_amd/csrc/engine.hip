@@ -586,6 +586,12 @@ private:
     // general-path storage
     std::vector<Buf<T>> F_, Dr_;
     std::vector<Buf<cpx<T>>> K_, Tsq_, Tel_, INIT_, FINAL_, KF_;
+    // RK4 stage factor tables per field: [3 i] gamma = dt / 2, [3 i + 1] gamma = dt,
+    // [3 i + 2] gamma = dt with the filter (empty when the equation has a term that needs the
+    // frequency vector); built once per plan, dropped when the filter changes
+    std::vector<Buf<T>> stab_;
+    bool stab_ok_ = false;
+    void build_stage_tables();
     std::vector<Buf<double>> Ktab_;  // K(k) of each elastic term, nk values
     std::vector<Buf<T>> lag_;
     // VolumeConservingLP: indicator brick, field, device state {Multiplier, CurrentIntegral,
@@ -1371,6 +1377,7 @@ void Engine<T>::set_filter(const double* table) {
     if (table && stepper_ == GPF_SDD) throw Error(GPF_EINVAL, "SDD: Does not support modal filters");
     HIPCHK(hipStreamSynchronize(st_));
     drop_graph();
+    stab_ok_ = false;
     if (!table) {
         has_filter_ = false;
         return;
@@ -1378,6 +1385,7 @@ void Engine<T>::set_filter(const double* table) {
     if (!filter_.get()) filter_.alloc(1000);
     HIPCHK(hipMemcpy(filter_.get(), table, 1000 * sizeof(double), hipMemcpyHostToDevice));
     has_filter_ = true;
+    stab_ok_ = false;
 }
 
 template <typename T>
@@ -1848,6 +1856,35 @@ void Engine<T>::sync_from_spectra() {
     if (g_.ndim == 3) pencils(out, {}, 2, -1);
 }
 
+// The factor 1 / (1 - gamma dt D(k)) of RK4's stages depends only on the plan and on gamma
+// (1/2 for stages 0-1, 1 for stages 2-3, the filter on the last): tabulated once in FP64 and
+// stored in T, so the stage kernels read it instead of forming it in every stage.
+template <typename T>
+void Engine<T>::build_stage_tables() {
+    stab_.clear();
+    for (int i = 0; i < nf_; ++i) {
+        bool ok = true;
+        for (const auto& t : eqs_[i].terms) ok &= (t.kind != kConservativeNoise);
+        for (int k = 0; k < 3; ++k) {
+            stab_.emplace_back();
+            if (!ok || (k == 2 && !has_filter_)) continue;
+            stab_.back().alloc(g_.nk);
+            StageDenomArgs<T> a;
+            a.denom = eqs_[i].denom;
+            a.sv = eqs_[i].sv;
+            a.hs = eqs_[i].hs;
+            a.gamma = (k == 0) ? 0.5 * dt_ : dt_;
+            a.filter = (k == 2) ? filter_.get() : nullptr;
+            a.g = g_.kg;
+            a.nmain = g_.nmain;
+            a.nk = g_.nk;
+            a.out = stab_.back().get();
+            HIPCHK(launch_stage_denoms<T>(a, st_));
+        }
+    }
+    stab_ok_ = true;
+}
+
 template <typename T>
 void Engine<T>::general_step_rk4() {
     // RK4.Step (rk4.go:29-127) with each stage's GetRHS, PrepareNextCorrection and
@@ -1855,6 +1892,7 @@ void Engine<T>::general_step_rk4() {
     // evaluated from the bricks inside the pass instead of being stored.  The corrected
     // spectra go to KF_ (free, as the right-hand side is never stored) and swap with K_
     // after all fields, so every field of a stage reads the same bricks.
+    if (!stab_ok_) build_stage_tables();
     ensure_k_fields();
     sync_derived();
     eval_elastic();  // the lagged field is constant within a step (homoLinElast.go:130-134)
@@ -1880,6 +1918,10 @@ void Engine<T>::general_step_rk4() {
             a.g = g_.kg;
             a.nmain = g_.nmain;
             a.nk = g_.nk;
+            const int kind = (stage < 2) ? 0 : (stage == 3 && has_filter_) ? 2 : 1;
+            a.stab = stab_[3 * i + kind].get();
+            a.need_l = 0;
+            for (int t = 0; t < a.nterm; ++t) a.need_l |= (a.pw[t] != 0);
             HIPCHK(launch_rk4_stage<T>(a, st_));
         }
         std::vector<const cpx<T>*> in;

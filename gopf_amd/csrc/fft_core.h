@@ -107,12 +107,16 @@ GPF_SCHED_V(2048, 1, 8, 8, 8, 8, 4)
 #undef GPF_SCHED
 #undef GPF_SCHED_V
 
-// every length with a compiled schedule, for dispatch tables: X(n) per length
-#define GPF_FAST_LENGTHS(X)                                                                        \
-    X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)                          \
-    X(12) X(24) X(48) X(96) X(192) X(384) X(768)                                                   \
-    X(80) X(100) X(160) X(200) X(250) X(320) X(400) X(500) X(640) X(800) X(1000) X(1280) X(1600)   \
-    X(2000) X(2560) X(112) X(224) X(448) X(896)
+// every length with a compiled schedule, for dispatch tables: X(n) per length.  The core
+// lengths (2^a <= 1024, 2^a 3 <= 768) and the extended ones (2048, radix 5 and 7) are
+// compiled into separate units, so that the code objects of the hot passes stay small
+#define GPF_CORE_LENGTHS(X)                                                                        \
+    X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)                                  \
+    X(12) X(24) X(48) X(96) X(192) X(384) X(768)
+#define GPF_EXT_LENGTHS(X)                                                                         \
+    X(2048) X(80) X(100) X(160) X(200) X(250) X(320) X(400) X(500) X(640) X(800) X(1000) X(1280)   \
+    X(1600) X(2000) X(2560) X(112) X(224) X(448) X(896)
+#define GPF_FAST_LENGTHS(X) GPF_CORE_LENGTHS(X) GPF_EXT_LENGTHS(X)
 
 // span before stage S
 template <int N, int V, int S>

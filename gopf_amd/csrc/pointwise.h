@@ -189,8 +189,28 @@ struct Rk4StageArgs {
     const double* filter = nullptr;
     KGrid g;
     long nmain = 0, nk = 0;
+    // optional table of this stage's factor s(k) = filt / (1 - gamma dt D(k)) (built once per
+    // plan by launch_stage_denoms, in FP64 and stored in T); the kernel then computes no
+    // denominator and, when no term has a LAP power (need_l = 0), no frequency at all
+    const T* stab = nullptr;
+    int need_l = 1;
 };
 template <typename T> hipError_t launch_rk4_stage(const Rk4StageArgs<T>& a, hipStream_t s);
+
+// the factor table of one RK4 stage kind: out[k] = filt(k) / (1 - gamma D(k)) with the
+// TensorialHessian rows averaged over k and -k, as k_rk4_stage forms it (filter optional)
+template <typename T>
+struct StageDenomArgs {
+    KPoly denom;
+    SvPoly sv;
+    HessPoly hs;
+    double gamma = 0.0;
+    const double* filter = nullptr;
+    KGrid g;
+    long nmain = 0, nk = 0;
+    T* out = nullptr;
+};
+template <typename T> hipError_t launch_stage_denoms(const StageDenomArgs<T>& a, hipStream_t s);
 
 // spectral gradient components: out_d = i 2 pi f_d' src   (f_d' = 0 at f_d = +0.5)
 template <typename T>

@@ -340,6 +340,120 @@ __global__ void __launch_bounds__(kStageThreads) k_rk4_stage(Rk4StageArgs<T> a) 
     }
 }
 
+// the per-plan factor table of one stage kind (see StageDenomArgs): the FP64 arithmetic of
+// k_rk4_stage's factor, done once per plan instead of in every stage
+template <typename T>
+__global__ void k_stage_denoms(StageDenomArgs<T> a) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
+        int kx, ky, kz;
+        kcoords(a.g, i, a.nmain, kx, ky, kz);
+        double f[3];
+        ref_freq(a.g, kx, ky, kz, f);
+        const double f2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+        const double L = -(4.0 * kPi * kPi) * f2;
+        const double D = denominator(a.denom, a.sv, a.hs, a.g.ndim, f, L, f2);
+        double s = 1.0 / (1.0 - a.gamma * D);
+        if (asym_row(a.hs, a.g.ndim, f)) {
+            double fcj[3];
+            conj_freq(a.g.ndim, f, fcj);
+            const double Dc = denominator(a.denom, a.sv, a.hs, a.g.ndim, fcj, L, f2);
+            s = 0.5 * (s + 1.0 / (1.0 - a.gamma * Dc));
+        }
+        if (a.filter) s *= vandeven(a.filter, sqrt(f2) * 2.0 / kPi);
+        a.out[i] = (T)s;
+    }
+}
+
+// k_rk4_stage with the stage factor read from its table (every term of tmode 0).  A thread
+// takes EPT elements kStageThreads apart and issues every load of all of them (the table,
+// the stage operands and the first kPre bricks) before any arithmetic, so the kernel keeps
+// many loads in flight; L(k) is formed only when some term has a LAP power (NEEDL).
+template <typename T, int EPT, bool NEEDL>
+__global__ void __launch_bounds__(kStageThreads) k_rk4_stage_tab(Rk4StageArgs<T> a) {
+    constexpr int kPre = 3;
+    const long i0 = (long)blockIdx.x * (kStageThreads * EPT) + threadIdx.x;
+    long iv[EPT];
+    bool ok[EPT];
+    T sf[EPT];
+    cpx<T> o1[EPT], o2[EPT], br[kPre][EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const long i = i0 + kStageThreads * e;
+        ok[e] = i < a.nk;
+        iv[e] = ok[e] ? i : 0;  // past the end: a valid element, never stored
+        sf[e] = a.stab[iv[e]];
+        o1[e] = (a.stage == 0) ? a.k[iv[e]] : a.final[iv[e]];
+        o2[e] = (a.stage == 1 || a.stage == 2) ? a.init[iv[e]] : mk<T>(T(0), T(0));
+#pragma unroll
+        for (int t = 0; t < kPre; ++t)
+            br[t][e] = (t < a.nterm && a.brick[t]) ? a.brick[t][iv[e]] : mk<T>(T(0), T(0));
+    }
+    double L[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        L[e] = 0.0;
+        if constexpr (NEEDL) {
+            int kx = 0, ky = 0, kz = 0;
+            kcoords(a.g, iv[e], a.nmain, kx, ky, kz);
+            double f[3];
+            ref_freq(a.g, kx, ky, kz, f);
+            L[e] = -(4.0 * kPi * kPi) * (f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        }
+    }
+    double rx[EPT], ry[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) rx[e] = ry[e] = 0.0;
+    for (int t = 0; t < a.nterm; ++t) {
+        const double c = a.coef[t];
+        const double* dmp = a.dmul[t];
+        const double dm = dmp ? dmp[0] : 1.0;
+        const int pw = a.pw[t];
+        const cpx<T>* bp = a.brick[t];
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            double me = NEEDL ? c * ipow(L[e], pw) : c;
+            if (dmp) me *= dm;
+            if (!bp) {
+                rx[e] += me;
+                continue;
+            }
+            cpx<T> b;
+            if (t == 0) b = br[0][e];
+            else if (t == 1) b = br[1][e];
+            else if (t == 2) b = br[2][e];
+            else b = bp[iv[e]];
+            rx[e] += me * (double)b.x;
+            ry[e] += me * (double)b.y;
+        }
+    }
+    const double bp = a.fp * a.dt;
+    const double bc = a.fc * a.dt;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        if (!ok[e]) continue;
+        const long i = iv[e];
+        // KF is stored in T between the stage kernels of RK4.Step; round it the same way
+        const double qx = (double)(T)rx[e];
+        const double qy = (double)(T)ry[e];
+        const double s = (double)sf[e];
+        if (a.stage == 0) {
+            const cpx<T> k = o1[e];
+            a.init[i] = k;
+            a.final[i] = mk<T>((T)((double)k.x + bp * qx), (T)((double)k.y + bp * qy));
+            a.kn[i] = mk<T>((T)(s * ((double)k.x + bc * qx)), (T)(s * ((double)k.y + bc * qy)));
+        } else if (a.stage < 3) {
+            const cpx<T> fin = o1[e], ini = o2[e];
+            a.final[i] = mk<T>((T)((double)fin.x + bp * qx), (T)((double)fin.y + bp * qy));
+            a.kn[i] = mk<T>((T)(s * ((double)ini.x + bc * qx)), (T)(s * ((double)ini.y + bc * qy)));
+        } else {
+            const cpx<T> fin = o1[e];
+            // final is stored in T before the division (rk4.go:57-67)
+            const T fx = (T)((double)fin.x + bp * qx), fy = (T)((double)fin.y + bp * qy);
+            a.kn[i] = mk<T>((T)(s * (double)fx), (T)(s * (double)fy));
+        }
+    }
+}
+
 // e^{D dt} and nonlinearIntegral (implicitEuler.go:151-162) for real D
 __device__ __forceinline__ void etd_update(double D, double dt, double rx, double ry, double px, double py, double& e,
                                            double& ix, double& iy) {
@@ -1003,7 +1117,25 @@ template <typename T>
 hipError_t launch_rk4_stage(const Rk4StageArgs<T>& a, hipStream_t s) {
     constexpr int EPT = kStageEpt;
     const long blocks = (a.nk + kStageThreads * EPT - 1) / (kStageThreads * EPT);
+    if (a.stab) {
+        for (int t = 0; t < a.nterm; ++t)
+            if (a.tmode[t] != 0) return hipErrorInvalidValue;  // needs the frequency vector
+        // FP64 operands take twice the registers: two elements per thread keep 4 waves per SIMD
+        constexpr int ET = sizeof(T) == 8 ? 2 : 4;
+        const long bt = (a.nk + kStageThreads * ET - 1) / (kStageThreads * ET);
+        if (a.need_l)
+            hipLaunchKernelGGL((k_rk4_stage_tab<T, ET, true>), dim3((unsigned)bt), dim3(kStageThreads), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_rk4_stage_tab<T, ET, false>), dim3((unsigned)bt), dim3(kStageThreads), 0, s, a);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL((k_rk4_stage<T, EPT>), dim3((unsigned)blocks), dim3(kStageThreads), 0, s, a);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_stage_denoms(const StageDenomArgs<T>& a, hipStream_t s) {
+    if (!a.out) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_stage_denoms<T>, dim3(grid_for(a.nk)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 template <typename T>
@@ -1183,6 +1315,7 @@ hipError_t launch_reduce2(const double* a, long na, double ca, const double* b, 
     template hipError_t launch_kupdate<T>(const KUpdateArgs<T>&, hipStream_t);                                \
     template hipError_t launch_etd_residual<T>(const EtdArgs<T>&, hipStream_t);                               \
     template hipError_t launch_rk4_stage<T>(const Rk4StageArgs<T>&, hipStream_t);                             \
+    template hipError_t launch_stage_denoms<T>(const StageDenomArgs<T>&, hipStream_t);                        \
     template hipError_t launch_volcons_update<T>(const double*, long, const T*, double, double*, hipStream_t); \
     template hipError_t launch_kgrad<T>(const KGradArgs<T>&, int, hipStream_t);                               \
     template hipError_t launch_kelastic<T>(const KElasticArgs<T>&, hipStream_t);                              \

@@ -1,4 +1,4 @@
-// spectral_c2r.hip -- FP32 instantiations of the half-spectrum-to-real row transforms k_rows_c2r (every length).
+// spectral_c2r.hip -- FP32 instantiations of the half-spectrum-to-real row transforms k_rows_c2r (the core lengths).
 #include "spectral_kernels.h"
 
 namespace gpf {

@@ -1,0 +1,9 @@
+// spectral_ext_fwd64.hip -- FP64 forward pencil transforms of the extended lengths (2048, radix 5
+// and 7), apart from the core lengths' units so that the hot passes' code objects stay small.
+#include "spectral_kernels.h"
+
+namespace gpf {
+
+template hipError_t launch_pencils_dir_set<double, -1, 1>(const PencilArgs<double>&, int, hipStream_t);
+
+}  // namespace gpf

@@ -1,0 +1,10 @@
+// spectral_group64.hip -- FP64 grouped row and update launches of a batch (every size class in one
+// launch), in a unit of their own, apart from the per-class kernels.
+#include "spectral_kernels.h"
+
+namespace gpf {
+
+template hipError_t launch_rows_fused_group<double>(const BatchGroupArgs<double>&, int, size_t, hipStream_t);
+template hipError_t launch_pencil_update_group<double>(const BatchGroupArgs<double>&, int, size_t, hipStream_t);
+
+}  // namespace gpf

@@ -437,9 +437,12 @@ __device__ __forceinline__ void update_multipliers(const PencilUpdateArgs<T>& u,
 
 // MON: the energy monitor's partial sums are compiled in (a separate instantiation, so
 // that the common path does not carry their registers)
-template <typename T, int N, bool MON>
-__device__ __forceinline__ void pencil_update_body(const PencilUpdateArgs<T>& u, int blk, int nblk,
-                                                   unsigned char* smem) {
+// UA is PencilUpdateArgs<T> for the plain kernel (its argument by value: read through a
+// reference, the kernel argument costs about ten more VGPRs, one wave per SIMD at 256) and
+// const PencilUpdateArgs<T>& for the grouped kernel (arguments read in place from global
+// memory: a copy would live in scratch)
+template <typename T, int N, bool MON, typename UA = PencilUpdateArgs<T>>
+__device__ __forceinline__ void pencil_update_body(UA u, int blk, int nblk, unsigned char* smem) {
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
     constexpr int V = 1;  // low-register schedule: two lines and the multipliers stay in registers
     constexpr int E = LineShape<N, V>::E;
@@ -563,76 +566,95 @@ __device__ __forceinline__ void split_store(const cpx<T>* v, cpx<T>* L, int n, i
     __syncthreads();
 }
 
-// split_store for the row pass with a compiled line shape: rows r1 (A) and r1 + 1 (B) of the
-// output, the k loop unrolled at compile time (nh a multiple of TTc), stores at immediate
-// offsets from per-row base pointers; the x-Nyquist pair is written by thread 0
+// 16-byte access needs 16-byte aligned row bases (every row of an array is, when the array
+// base is and nh is even)
+__device__ __forceinline__ bool al16(const void* p, const void* q) {
+    return ((reinterpret_cast<uintptr_t>(p) | reinterpret_cast<uintptr_t>(q)) & 15) == 0;
+}
+
+// split_store for a compiled line shape (N >= 16, E even, so nh = TTc E / 2): the line holds
+// Z = A + i B of two real rows; A's half spectrum goes to the row at amain (x-Nyquist value
+// at *anyq), B's to bmain / *bnyq when v2.  The k loop is unrolled at compile time, stores use
+// immediate offsets from the row bases, and the x-Nyquist pair is written by thread 0.
 template <typename T, int E, int TTc>
-__device__ __forceinline__ void split_store_rows(const cpx<T>* v, cpx<T>* L, int n, int t, long r1, bool v1, bool v2,
-                                                 cpx<T>* gmain, cpx<T>* gnyq) {
+__device__ __forceinline__ void split_store2(const cpx<T>* v, cpx<T>* L, int n, int t, cpx<T>* amain, cpx<T>* anyq,
+                                             cpx<T>* bmain, cpx<T>* bnyq, bool v1, bool v2) {
     const int nh = n / 2;
 #pragma unroll
     for (int e = 0; e < E; ++e) L[lds_pad(t + TTc * e)] = v[e];
     __syncthreads();
     constexpr int J = (E * TTc / 2) / TTc;  // = E / 2 k values per thread below nh
+    // mirror of k (k <= nh): n - k, and 0 for k = 0
+    auto mir = [n](int k) { return k == 0 ? 0 : n - k; };
+    bool wide = false;
+    if constexpr (sizeof(T) == 4 && J % 2 == 0) wide = al16(amain, v2 ? bmain : amain);
     if constexpr (sizeof(T) == 4 && J % 2 == 0) {
-        // FP32: lane t stores k = 2t, 2t + 1 (+ 2 TTc j) of each row as one 16-byte value
-        float4* fa = reinterpret_cast<float4*>(gmain + r1 * nh) + t;
-        float4* fb = reinterpret_cast<float4*>(gmain + (r1 + 1) * nh) + t;
-        float4 A4[J / 2], B4[J / 2];
-#pragma unroll
-        for (int j = 0; j < J / 2; ++j) {
-            const int k = 2 * t + 2 * TTc * j;
-            const cpx<T> z0 = L[lds_pad(k)], z1 = L[lds_pad(k + 1)];
-            const cpx<T> m0 = L[lds_pad((n - k) & (n - 1))], m1 = L[lds_pad((n - k - 1) & (n - 1))];
-            A4[j] = make_float4(0.5f * (z0.x + m0.x), 0.5f * (z0.y - m0.y), 0.5f * (z1.x + m1.x), 0.5f * (z1.y - m1.y));
-            B4[j] = make_float4(0.5f * (z0.y + m0.y), -0.5f * (z0.x - m0.x), 0.5f * (z1.y + m1.y), -0.5f * (z1.x - m1.x));
-        }
-        if (v1 && v2) {
+        if (wide) {
+            // FP32: lane t stores k = 2t, 2t + 1 (+ 2 TTc j) of each row as one 16-byte value
+            float4* fa = reinterpret_cast<float4*>(amain) + t;
+            float4* fb = reinterpret_cast<float4*>(bmain) + t;
+            float4 A4[J / 2], B4[J / 2];
 #pragma unroll
             for (int j = 0; j < J / 2; ++j) {
-                fa[TTc * j] = A4[j];
-                fb[TTc * j] = B4[j];
+                const int k = 2 * t + 2 * TTc * j;
+                const cpx<T> z0 = L[lds_pad(k)], z1 = L[lds_pad(k + 1)];
+                const cpx<T> m0 = L[lds_pad(mir(k))], m1 = L[lds_pad(n - k - 1)];
+                A4[j] = make_float4(0.5f * (z0.x + m0.x), 0.5f * (z0.y - m0.y), 0.5f * (z1.x + m1.x),
+                                    0.5f * (z1.y - m1.y));
+                B4[j] = make_float4(0.5f * (z0.y + m0.y), -0.5f * (z0.x - m0.x), 0.5f * (z1.y + m1.y),
+                                    -0.5f * (z1.x - m1.x));
+            }
+            if (v1 && v2) {
+#pragma unroll
+                for (int j = 0; j < J / 2; ++j) {
+                    fa[TTc * j] = A4[j];
+                    fb[TTc * j] = B4[j];
+                }
+            } else if (v1) {
+#pragma unroll
+                for (int j = 0; j < J / 2; ++j) fa[TTc * j] = A4[j];
+            }
+        }
+    }
+    if (!wide) {
+        cpx<T>* a1 = amain + t;
+        cpx<T>* b1 = bmain + t;
+        cpx<T> A[J], B[J];
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int k = t + TTc * j;
+            const cpx<T> z = L[lds_pad(k)];
+            const cpx<T> zm = L[lds_pad(mir(k))];
+            // A = (z + conj zm) / 2,  B = (z - conj zm) / (2 i)
+            A[j] = mk<T>(T(0.5) * (z.x + zm.x), T(0.5) * (z.y - zm.y));
+            B[j] = mk<T>(T(0.5) * (z.y + zm.y), T(-0.5) * (z.x - zm.x));
+        }
+        if (v1 && v2) {
+            // every line but possibly the last: one branch for all stores
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                a1[TTc * j] = A[j];
+                b1[TTc * j] = B[j];
             }
         } else if (v1) {
 #pragma unroll
-            for (int j = 0; j < J / 2; ++j) fa[TTc * j] = A4[j];
+            for (int j = 0; j < J; ++j) a1[TTc * j] = A[j];
         }
-        if (t == 0) {
-            const cpx<T> z = L[lds_pad(nh)];
-            if (v1) gnyq[r1] = mk<T>(z.x, T(0));
-            if (v2) gnyq[r1 + 1] = mk<T>(z.y, T(0));
-        }
-        __syncthreads();
-        return;
-    }
-    cpx<T>* a1 = gmain + r1 * nh + t;
-    cpx<T> A[J], B[J];
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const int k = t + TTc * j;
-        const cpx<T> z = L[lds_pad(k)];
-        const cpx<T> zm = L[lds_pad((n - k) & (n - 1))];
-        // A = (z + conj zm) / 2,  B = (z - conj zm) / (2 i)
-        A[j] = mk<T>(T(0.5) * (z.x + zm.x), T(0.5) * (z.y - zm.y));
-        B[j] = mk<T>(T(0.5) * (z.y + zm.y), T(-0.5) * (z.x - zm.x));
-    }
-    if (v1 && v2) {
-        // every line but possibly the last: one branch for all stores
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-            a1[TTc * j] = A[j];
-            a1[TTc * j + nh] = B[j];
-        }
-    } else if (v1) {
-#pragma unroll
-        for (int j = 0; j < J; ++j) a1[TTc * j] = A[j];
     }
     if (t == 0) {
         const cpx<T> z = L[lds_pad(nh)];
-        if (v1) gnyq[r1] = mk<T>(z.x, T(0));
-        if (v2) gnyq[r1 + 1] = mk<T>(z.y, T(0));
+        if (v1) *anyq = mk<T>(z.x, T(0));
+        if (v2) *bnyq = mk<T>(z.y, T(0));
     }
     __syncthreads();
+}
+
+// split_store2 for the fused row pass: rows r1 (A) and r1 + 1 (B) of one array
+template <typename T, int E, int TTc>
+__device__ __forceinline__ void split_store_rows(const cpx<T>* v, cpx<T>* L, int n, int t, long r1, bool v1, bool v2,
+                                                 cpx<T>* gmain, cpx<T>* gnyq) {
+    const int nh = n / 2;
+    split_store2<T, E, TTc>(v, L, n, t, gmain + r1 * nh, gnyq + r1, gmain + (r1 + 1) * nh, gnyq + r1 + 1, v1, v2);
 }
 
 // Hermitian extension of a half-spectrum row at canonical positions
@@ -691,64 +713,64 @@ __device__ __forceinline__ void load_herm_pair(cpx<T>* v, const cpx<T>* m, const
     }
 }
 
-// As load_herm_pair, with each half-spectrum value read from HBM once: the direct
-// positions are loaded and staged in the line's LDS (row r1 at [0, nh], row r1 + 1 at
-// [nh + 1, n + 1]), and the mirrored positions are read back from LDS.  The line's LDS
-// region (lds_line(n) >= n + 2 elements for n >= 16) is free again on return.
-// Every lane issues all of its loads before it uses any of them: a lane whose position is
-// mirrored, or whose row is past the end, loads from a valid address of a valid row and
-// discards the value, so no load sits behind a branch or waits for the one before it.
+// Two half-spectrum rows (A: main row at pa, x-Nyquist value at *qa; B: pb, *qb),
+// Hermitian-extended and packed as A + i B, so that one inverse transform yields both real
+// rows.  Each half-spectrum value is read from HBM once: the direct positions are loaded and
+// staged in the line's LDS (row A at [0, nh], row B at [nh + 1, n + 1]), and the mirrored
+// positions are read back from LDS.  The line's LDS region (lds_line(n) >= n + 2 elements
+// for n >= 16) is free again on return.  Every lane issues all of its loads before it uses
+// any of them; the row pointers must be valid addresses even when v1 / v2 is false (the
+// values are then discarded), so no load sits behind a branch.
 template <typename T, int E, int TTc>
-__device__ __forceinline__ void load_herm_pair_lds(cpx<T>* v, const cpx<T>* m, const cpx<T>* q, long r1, int n, int t,
-                                                   bool v1, bool v2, cpx<T>* L) {
+__device__ __forceinline__ void load_herm2_lds(cpx<T>* v, const cpx<T>* pa, const cpx<T>* qa, const cpx<T>* pb,
+                                               const cpx<T>* qb, int n, int t, bool v1, bool v2, cpx<T>* L) {
     const int nh = n / 2;
-    const long ra = v1 ? r1 : 0, rb = v2 ? r1 + 1 : ra;
     if constexpr (sizeof(T) == 4 && (TTc * E) % (4 * TTc) == 0) {
-        // FP32: lane t loads positions 2t, 2t + 1 (+ 2 TTc e) of each row as one 16-byte
-        // value; every canonical element then comes from LDS
-        constexpr int NL = (TTc * E / 2) / (2 * TTc);  // 16-byte loads per row below nh
-        const float4* fa = reinterpret_cast<const float4*>(m + ra * nh) + t;
-        const float4* fb = reinterpret_cast<const float4*>(m + rb * nh) + t;
-        float4 A4[NL], B4[NL];
+        if (al16(pa, pb)) {
+            // FP32: lane t loads positions 2t, 2t + 1 (+ 2 TTc e) of each row as one 16-byte
+            // value; every canonical element then comes from LDS
+            constexpr int NL = (TTc * E / 2) / (2 * TTc);  // 16-byte loads per row below nh
+            const float4* fa = reinterpret_cast<const float4*>(pa) + t;
+            const float4* fb = reinterpret_cast<const float4*>(pb) + t;
+            float4 A4[NL], B4[NL];
 #pragma unroll
-        for (int e = 0; e < NL; ++e) {
-            A4[e] = fa[TTc * e];
-            B4[e] = fb[TTc * e];
-        }
-        const cpx<T> qa = q[ra], qb = q[rb];
-        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-        for (int e = 0; e < NL; ++e) {
-            const int p = 2 * t + 2 * TTc * e;
-            const float4 a4 = v1 ? A4[e] : z4, b4 = v2 ? B4[e] : z4;
-            L[p] = mk<T>(a4.x, a4.y);
-            L[p + 1] = mk<T>(a4.z, a4.w);
-            L[nh + 1 + p] = mk<T>(b4.x, b4.y);
-            L[nh + 2 + p] = mk<T>(b4.z, b4.w);
-        }
-        if (t == 0) {
-            L[nh] = v1 ? qa : mk<T>(T(0), T(0));
-            L[n + 1] = v2 ? qb : mk<T>(T(0), T(0));
-        }
-        __syncthreads();
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int p = t + TTc * e;
-            cpx<T> a, b;
-            if (p <= nh) {
-                a = L[p];
-                b = L[nh + 1 + p];
-            } else {
-                a = conjg(L[n - p]);
-                b = conjg(L[nh + 1 + n - p]);
+            for (int e = 0; e < NL; ++e) {
+                A4[e] = fa[TTc * e];
+                B4[e] = fb[TTc * e];
             }
-            v[e] = mk<T>(a.x - b.y, a.y + b.x);
+            const cpx<T> qav = *qa, qbv = *qb;
+            const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int e = 0; e < NL; ++e) {
+                const int p = 2 * t + 2 * TTc * e;
+                const float4 a4 = v1 ? A4[e] : z4, b4 = v2 ? B4[e] : z4;
+                L[p] = mk<T>(a4.x, a4.y);
+                L[p + 1] = mk<T>(a4.z, a4.w);
+                L[nh + 1 + p] = mk<T>(b4.x, b4.y);
+                L[nh + 2 + p] = mk<T>(b4.z, b4.w);
+            }
+            if (t == 0) {
+                L[nh] = v1 ? qav : mk<T>(T(0), T(0));
+                L[n + 1] = v2 ? qbv : mk<T>(T(0), T(0));
+            }
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int p = t + TTc * e;
+                cpx<T> a, b;
+                if (p <= nh) {
+                    a = L[p];
+                    b = L[nh + 1 + p];
+                } else {
+                    a = conjg(L[n - p]);
+                    b = conjg(L[nh + 1 + n - p]);
+                }
+                v[e] = mk<T>(a.x - b.y, a.y + b.x);
+            }
+            __syncthreads();
+            return;
         }
-        __syncthreads();
-        return;
     }
-    const cpx<T>* pa = m + ra * nh;
-    const cpx<T>* pb = m + rb * nh;
     const cpx<T> zero = mk<T>(T(0), T(0));
     cpx<T> A[E], B[E];
 #pragma unroll
@@ -761,8 +783,8 @@ __device__ __forceinline__ void load_herm_pair_lds(cpx<T>* v, const cpx<T>* m, c
             B[e] = pb[p];
         } else {
             // the element that straddles nh: direct below it, the x-Nyquist value at it
-            const cpx<T>* sa = (p < nh) ? pa + p : (p == nh ? q + ra : pa);
-            const cpx<T>* sb = (p < nh) ? pb + p : (p == nh ? q + rb : pb);
+            const cpx<T>* sa = (p < nh) ? pa + p : (p == nh ? qa : pa);
+            const cpx<T>* sb = (p < nh) ? pb + p : (p == nh ? qb : pb);
             A[e] = *sa;
             B[e] = *sb;
         }
@@ -793,6 +815,16 @@ __device__ __forceinline__ void load_herm_pair_lds(cpx<T>* v, const cpx<T>* m, c
     __syncthreads();
 }
 
+// load_herm2_lds for the fused row pass: rows r1 (A) and r1 + 1 (B) of one array; a row
+// past the end reads a valid row and is discarded
+template <typename T, int E, int TTc>
+__device__ __forceinline__ void load_herm_pair_lds(cpx<T>* v, const cpx<T>* m, const cpx<T>* q, long r1, int n, int t,
+                                                   bool v1, bool v2, cpx<T>* L) {
+    const int nh = n / 2;
+    const long ra = v1 ? r1 : 0, rb = v2 ? r1 + 1 : ra;
+    load_herm2_lds<T, E, TTc>(v, m + ra * nh, q + ra, m + rb * nh, q + rb, n, t, v1, v2, L);
+}
+
 // sum_ij dc[i][j] x0^i x1^j in T, Horner in x0 of polynomials in x1 (the k_derived formula)
 template <typename T>
 __device__ __forceinline__ T dense_poly(const double (&dc)[4][4], T x0, T x1) {
@@ -813,39 +845,90 @@ __device__ __forceinline__ T row_poly(const RowR2CArgs<T>& a, int q, long off) {
     return dense_poly<T>(a.dc[q], x0, x1);
 }
 
+// Row kernels of the general path.  Compiled lengths n >= 16 (kStaged) stage each line's
+// Hermitian loads and split stores through its LDS region (every half-spectrum value read
+// and written once, 16-byte FP32 accesses) and keep the block's copy of the twiddle table in
+// LDS after the exchange lines; rows past the end read row 0 and store nothing.
+template <int N, int E>
+constexpr bool row_staged() {
+    return N >= 16 && E % 2 == 0;
+}
+
+// the block's copy of the n-point twiddle table, after the exchange lines of TW lines
+template <typename T>
+__device__ __forceinline__ const cpx<T>* stage_twiddles(cpx<T>* lds, const cpx<T>* tw, int n, int TW) {
+    cpx<T>* twl = lds + lds_elems(n, TW);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) twl[i] = tw[i];
+    __syncthreads();
+    return twl;
+}
+
 template <typename T, int N>
 __global__ void __launch_bounds__(kMaxBlock) k_rows_r2c(RowR2CArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
     constexpr int V = plain_variant<N>();
     constexpr int E = LineShape<N, V>::E;
+    constexpr bool kStaged = row_staged<N, E>();
     const int n = (N > 0) ? N : a.nx;
+    const int nh = n / 2;
     const int TT = (N > 0) ? LineShape<N, V>::T : n;
     const LineMap lm = line_map(true, a.TW, TT, n);
     const long row = (long)blockIdx.x * a.TW + lm.l;
     const bool valid = row < a.rows;
+    const long rv = valid ? row : 0;
+    const cpx<T>* twp = a.tw;
+    if constexpr (kStaged) twp = stage_twiddles<T>(lds, a.tw, n, a.TW);
 #pragma unroll
     for (int q = 0; q < kMaxArr; q += 2) {
         if (q >= a.narr) break;
         const T* ia = a.in[q];
         const T* ib = (q + 1 < a.narr) ? a.in[q + 1] : nullptr;
+        const bool two = q + 1 < a.narr;
         cpx<T> v[E];
+        // the source rows of the pair: (f0, f1) of a polynomial or the plain input; when the
+        // second array reads the same rows as the first (two polynomials of the same fields),
+        // each row is loaded once, and every load is issued before any arithmetic
+        const T* s0 = a.poly[q] ? a.pf0[q] : ia;
+        const T* s1 = a.poly[q] ? a.pf1[q] : nullptr;
+        const T* t0 = two ? (a.poly[q + 1] ? a.pf0[q + 1] : ib) : nullptr;
+        const T* t1 = (two && a.poly[q + 1]) ? a.pf1[q + 1] : nullptr;
+        if (!two || (t0 == s0 && t1 == s1)) {
+            T x0[E], x1[E];
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const long off = row * n + lm.t + TT * e;
-            T x = T(0), y = T(0);
-            if (valid) {
+            for (int e = 0; e < E; ++e) {
+                const long off = rv * n + lm.t + TT * e;
+                x0[e] = s0 ? s0[off] : T(0);
+                x1[e] = s1 ? s1[off] : T(0);
+            }
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const T x = a.poly[q] ? dense_poly<T>(a.dc[q], x0[e], x1[e]) : x0[e];
+                const T y = !two ? T(0) : a.poly[q + 1] ? dense_poly<T>(a.dc[q + 1], x0[e], x1[e]) : x0[e];
+                v[e] = valid ? mk<T>(x * a.in_scale, y * a.in_scale) : mk<T>(T(0), T(0));
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const long off = rv * n + lm.t + TT * e;
+                T x = T(0), y = T(0);
                 if (a.poly[q]) x = row_poly(a, q, off) * a.in_scale;
                 else if (ia) x = ia[off] * a.in_scale;
-                if (q + 1 < a.narr && a.poly[q + 1]) y = row_poly(a, q + 1, off) * a.in_scale;
+                if (two && a.poly[q + 1]) y = row_poly(a, q + 1, off) * a.in_scale;
                 else if (ib) y = ib[off] * a.in_scale;
+                v[e] = valid ? mk<T>(x, y) : mk<T>(T(0), T(0));
             }
-            v[e] = mk<T>(x, y);
         }
-        line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
-        const bool two = q + 1 < a.narr;
-        split_store<T, E>(v, lds + lm.off, n, lm.t, TT, row, valid, a.main[q], a.nyq[q],
-                          two ? a.main[q + 1] : nullptr, two ? a.nyq[q + 1] : nullptr);
+        line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, twp);
+        if constexpr (kStaged) {
+            constexpr int TTc = LineShape<N, V>::T;
+            cpx<T>* am = a.main[q] + rv * nh;
+            split_store2<T, E, TTc>(v, lds + lm.off, n, lm.t, am, a.nyq[q] + rv, two ? a.main[q + 1] + rv * nh : am,
+                                    two ? a.nyq[q + 1] + rv : a.nyq[q] + rv, valid, valid && two);
+        } else {
+            split_store<T, E>(v, lds + lm.off, n, lm.t, TT, row, valid, a.main[q], a.nyq[q],
+                              two ? a.main[q + 1] : nullptr, two ? a.nyq[q + 1] : nullptr);
+        }
     }
 }
 
@@ -855,27 +938,41 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_c2r(RowC2RArgs<T> a) {
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
     constexpr int V = plain_variant<N>();
     constexpr int E = LineShape<N, V>::E;
+    constexpr bool kStaged = row_staged<N, E>();
     const int n = (N > 0) ? N : a.nx;
+    const int nh = n / 2;
     const int TT = (N > 0) ? LineShape<N, V>::T : n;
     const LineMap lm = line_map(true, a.TW, TT, n);
     const long row = (long)blockIdx.x * a.TW + lm.l;
     const bool valid = row < a.rows;
+    const long rv = valid ? row : 0;
     const double add0 = a.add_mean ? *a.add_mean : 0.0;
+    const cpx<T>* twp = a.tw;
+    if constexpr (kStaged) twp = stage_twiddles<T>(lds, a.tw, n, a.TW);
 #pragma unroll
     for (int q = 0; q < kMaxArr; q += 2) {
         if (q >= a.narr) break;
         const bool two = q + 1 < a.narr;
-        cpx<T> A[E], v[E];
-        load_herm<T, E>(A, a.main[q], a.nyq[q], row, n, lm.t, TT, valid);
-        if (two) {
-            load_herm<T, E>(v, a.main[q + 1], a.nyq[q + 1], row, n, lm.t, TT, valid);
-#pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = mk<T>(A[e].x - v[e].y, A[e].y + v[e].x);  // A + i B
+        cpx<T> v[E];
+        if constexpr (kStaged) {
+            constexpr int TTc = LineShape<N, V>::T;
+            const cpx<T>* pa = a.main[q] + rv * nh;
+            load_herm2_lds<T, E, TTc>(v, pa, a.nyq[q] + rv, two ? a.main[q + 1] + rv * nh : pa,
+                                      two ? a.nyq[q + 1] + rv : a.nyq[q] + rv, n, lm.t, valid, valid && two,
+                                      lds + lm.off);
         } else {
+            cpx<T> A[E];
+            load_herm<T, E>(A, a.main[q], a.nyq[q], row, n, lm.t, TT, valid);
+            if (two) {
+                load_herm<T, E>(v, a.main[q + 1], a.nyq[q + 1], row, n, lm.t, TT, valid);
 #pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = A[e];
+                for (int e = 0; e < E; ++e) v[e] = mk<T>(A[e].x - v[e].y, A[e].y + v[e].x);  // A + i B
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; ++e) v[e] = A[e];
+            }
         }
-        line_fft<T, N, +1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
+        line_fft<T, N, +1, V>(v, lds + lm.off, 1, n, lm.t, twp);
         if (valid) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
@@ -888,20 +985,43 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_c2r(RowC2RArgs<T> a) {
     }
 }
 
-template <typename T, int N>
+// MODE: 0 = at most two derived arrays (one transform pair, the derived rows formed in place
+// of the real rows), 1 = the elastic source, 2 = the adjoint's lambda, 3 = more than two
+// derived arrays; separate instantiations keep the common mode 0 at its own register count
+template <typename T, int N, int MODE>
 __global__ void __launch_bounds__(kMaxBlock) k_rows_sync(RowSyncArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
     constexpr int V = plain_variant<N>();
     constexpr int E = LineShape<N, V>::E;
+    constexpr bool kStaged = row_staged<N, E>();
+    constexpr int TTc = (N > 0) ? LineShape<N, V>::T : 0;
     const int n = (N > 0) ? N : a.nx;
+    const int nh = n / 2;
     const int TT = (N > 0) ? LineShape<N, V>::T : n;
     const LineMap lm = line_map(true, a.TW, TT, n);
     const long row = (long)blockIdx.x * a.TW + lm.l;
     const bool valid = row < a.rows;
+    const long rv = valid ? row : 0;
     const bool two = a.fmain[1] != nullptr;
+    const cpx<T>* twp = a.tw;
+    if constexpr (kStaged) twp = stage_twiddles<T>(lds, a.tw, n, a.TW);
+    // one derived row (A only) or a pair (A, B) of the forward transform in v to its arrays
+    auto store_pair = [&](const cpx<T>* vv, cpx<T>* am, cpx<T>* aq, cpx<T>* bm, cpx<T>* bq) {
+        if constexpr (kStaged) {
+            cpx<T>* a0 = am + rv * nh;
+            split_store2<T, E, TTc>(vv, lds + lm.off, n, lm.t, a0, aq + rv, bm ? bm + rv * nh : a0,
+                                    bq ? bq + rv : aq + rv, valid, valid && bm != nullptr);
+        } else {
+            split_store<T, E>(vv, lds + lm.off, n, lm.t, TT, row, valid, am, aq, bm, bq);
+        }
+    };
     cpx<T> v[E];
-    {
+    if constexpr (kStaged) {
+        const cpx<T>* pa = a.fmain[0] + rv * nh;
+        load_herm2_lds<T, E, TTc>(v, pa, a.fnyq[0] + rv, two ? a.fmain[1] + rv * nh : pa,
+                                  two ? a.fnyq[1] + rv : a.fnyq[0] + rv, n, lm.t, valid, valid && two, lds + lm.off);
+    } else {
         cpx<T> A[E];
         load_herm<T, E>(A, a.fmain[0], a.fnyq[0], row, n, lm.t, TT, valid);
         if (two) {
@@ -913,20 +1033,41 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_sync(RowSyncArgs<T> a) {
             for (int e = 0; e < E; ++e) v[e] = A[e];
         }
     }
-    line_fft<T, N, +1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
+    line_fft<T, N, +1, V>(v, lds + lm.off, 1, n, lm.t, twp);
+    if constexpr (MODE == 0) {
+        // the real rows (rounded to T as the inverse pass stores them) and, in place, the
+        // derived rows of the (one or two) dense polynomials
+        const bool dtwo = a.nd > 1;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const T x0 = v[e].x * a.scale;
+            const T x1 = two ? v[e].y * a.scale : T(0);
+            if (valid) {
+                const long off = row * n + lm.t + TT * e;
+                if (a.fout[0]) a.fout[0][off] = x0;
+                if (two && a.fout[1]) a.fout[1][off] = x1;
+            }
+            const T d0 = valid ? dense_poly<T>(a.dc[0], x0, x1) : T(0);
+            const T d1 = (valid && dtwo) ? dense_poly<T>(a.dc[1], x0, x1) : T(0);
+            v[e] = mk<T>(d0, d1);
+        }
+        line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, twp);
+        store_pair(v, a.dmain[0], a.dnyq[0], dtwo ? a.dmain[1] : nullptr, dtwo ? a.dnyq[1] : nullptr);
+        return;
+    }
     // the real rows, rounded to T as the inverse pass stores them
     T x0[E], x1[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         x0[e] = v[e].x * a.scale;
         x1[e] = two ? v[e].y * a.scale : T(0);
-        if (valid && a.mode == 0) {
+        if (valid && MODE == 3) {
             const long off = row * n + lm.t + TT * e;
             if (a.fout[0]) a.fout[0][off] = x0[e];
             if (two && a.fout[1]) a.fout[1][off] = x1[e];
         }
     }
-    if (a.mode == 2) {
+    if constexpr (MODE == 2) {
         // the k_adj_lambda formula on the inverse rows
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -943,41 +1084,39 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_sync(RowSyncArgs<T> a) {
             }
             v[e] = mk<T>(lam, T(0));
         }
-        line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
-        split_store<T, E>(v, lds + lm.off, n, lm.t, TT, row, valid, a.dmain[0], a.dnyq[0], nullptr, nullptr);
+        line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, twp);
+        store_pair(v, a.dmain[0], a.dnyq[0], nullptr, nullptr);
         return;
     }
-    if (a.mode == 1) {
+    if constexpr (MODE == 1) {
         // P = H'(phi) (S - 2 e* H(phi)) in T on the inverse row, H = 3 phi^2 - 2 phi^3
         // (homoLinElast.go:10-12, 91-127)
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            T P = T(0);
-            if (valid) {
-                const T x = a.phi[row * n + lm.t + TT * e];
-                const T H = T(3) * x * x - T(2) * x * x * x;
-                const T dH = T(6) * x - T(6) * x * x;
-                P = dH * (x0[e] - (T)a.two_estar * H);
-            }
+            const T x = a.phi[rv * n + lm.t + TT * e];
+            const T H = T(3) * x * x - T(2) * x * x * x;
+            const T dH = T(6) * x - T(6) * x * x;
+            const T P = valid ? dH * (x0[e] - (T)a.two_estar * H) : T(0);
             v[e] = mk<T>(P, T(0));
         }
-        line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
-        split_store<T, E>(v, lds + lm.off, n, lm.t, TT, row, valid, a.dmain[0], a.dnyq[0], nullptr, nullptr);
+        line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, twp);
+        store_pair(v, a.dmain[0], a.dnyq[0], nullptr, nullptr);
         return;
     }
+    if constexpr (MODE == 3) {
 #pragma unroll
-    for (int q = 0; q < kMaxArr; q += 2) {
-        if (q >= a.nd) break;
-        const bool dtwo = q + 1 < a.nd;
+        for (int q = 0; q < kMaxArr; q += 2) {
+            if (q >= a.nd) break;
+            const bool dtwo = q + 1 < a.nd;
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const T d0 = valid ? dense_poly<T>(a.dc[q], x0[e], x1[e]) : T(0);
-            const T d1 = (valid && dtwo) ? dense_poly<T>(a.dc[q + 1], x0[e], x1[e]) : T(0);
-            v[e] = mk<T>(d0, d1);
+            for (int e = 0; e < E; ++e) {
+                const T d0 = valid ? dense_poly<T>(a.dc[q], x0[e], x1[e]) : T(0);
+                const T d1 = (valid && dtwo) ? dense_poly<T>(a.dc[q + 1], x0[e], x1[e]) : T(0);
+                v[e] = mk<T>(d0, d1);
+            }
+            line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, twp);
+            store_pair(v, a.dmain[q], a.dnyq[q], dtwo ? a.dmain[q + 1] : nullptr, dtwo ? a.dnyq[q + 1] : nullptr);
         }
-        line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, a.tw);
-        split_store<T, E>(v, lds + lm.off, n, lm.t, TT, row, valid, a.dmain[q], a.dnyq[q],
-                          dtwo ? a.dmain[q + 1] : nullptr, dtwo ? a.dnyq[q + 1] : nullptr);
     }
 }
 
@@ -1021,7 +1160,7 @@ __device__ __forceinline__ void rows_pointwise(cpx<T>* v, const T* gp, const Row
 
 // V: transform schedule (0 plain, 1 low-register); MON: monitor partial sums compiled in
 template <typename T, int N, int V, bool MON>
-__device__ __forceinline__ void rows_fused_body(const RowFusedArgs<T>& a, int blk, unsigned char* smem) {
+__device__ __forceinline__ void rows_fused_body(RowFusedArgs<T> a, int blk, unsigned char* smem) {
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
     constexpr int E = LineShape<N, V>::E;
     const int n = (N > 0) ? N : a.nx;
@@ -1074,7 +1213,7 @@ __device__ __forceinline__ void rows_fused_body(const RowFusedArgs<T>& a, int bl
         }
     }
     line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, twl);
-    if constexpr (N >= 16 && (N & (N - 1)) == 0 && E % 2 == 0)
+    if constexpr (row_staged<N, E>())
         split_store_rows<T, E, LineShape<N, V>::T>(v, lds + lm.off, n, lm.t, r1, v1, v2, a.g_main, a.g_nyq);
     else
         split_store<T, E>(v, lds + lm.off, n, lm.t, TT, r1, v1, a.g_main, a.g_nyq, v2 ? a.g_main + n / 2 : nullptr,
@@ -1142,7 +1281,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_pencil_update_group(BatchGroupArg
     const PencilUpdateArgs<T>& u = g.upd[c];
     switch (u.p.n) {
 #define GPF_G(NN) \
-    case NN: pencil_update_body<T, NN, false>(u, blk, cnt, smem); break;
+    case NN: pencil_update_body<T, NN, false, const PencilUpdateArgs<T>&>(u, blk, cnt, smem); break;
         GPF_GROUP_LENGTHS(GPF_G)
 #undef GPF_G
         default: break;
@@ -1159,15 +1298,40 @@ int threads_per_line(int n, int v = 0) {
     }
 }
 
-template <typename F>
-hipError_t dispatch_size(int n, F&& f) {
+// Length sets of the launch functions: 0 = the core lengths and the direct DFT (n <= 64),
+// 1 = the extended lengths.  Each set's kernels are compiled in units of their own: a code
+// object holding every length made each dispatch of the 256^3 step about 1 us slower (2-4 %
+// of the step), with the kernels themselves unchanged.
+__host__ __device__ constexpr bool ext_length(int n) {
     switch (n) {
-#define GPF_D(NN) case NN: return f(std::integral_constant<int, NN>{});
-        GPF_FAST_LENGTHS(GPF_D)
-#undef GPF_D
+#define GPF_C(NN) case NN:
+        GPF_EXT_LENGTHS(GPF_C)
+#undef GPF_C
+        return true;
         default:
-            if (n >= 1 && n <= 64) return f(std::integral_constant<int, 0>{});
-            return hipErrorInvalidValue;
+            return false;
+    }
+}
+
+template <int SET, typename F>
+hipError_t dispatch_size(int n, F&& f) {
+    if constexpr (SET == 0) {
+        switch (n) {
+#define GPF_D(NN) case NN: return f(std::integral_constant<int, NN>{});
+            GPF_CORE_LENGTHS(GPF_D)
+#undef GPF_D
+            default:
+                if (n >= 1 && n <= 64) return f(std::integral_constant<int, 0>{});
+                return hipErrorInvalidValue;
+        }
+    } else {
+        switch (n) {
+#define GPF_D(NN) case NN: return f(std::integral_constant<int, NN>{});
+            GPF_EXT_LENGTHS(GPF_D)
+#undef GPF_D
+            default:
+                return hipErrorInvalidValue;
+        }
     }
 }
 
@@ -1185,9 +1349,10 @@ hipError_t launch_k(K kernel, dim3 grid, dim3 block, size_t smem, hipStream_t s,
 
 }  // namespace
 
-// the pencils of one direction (the two directions are instantiated in separate units)
-template <typename T, int DIR>
-hipError_t launch_pencils_dir(const PencilArgs<T>& a0, int n, hipStream_t s) {
+// the pencils of one direction (the two directions are instantiated in separate units), for
+// the lengths of set SET
+template <typename T, int DIR, int SET>
+hipError_t launch_pencils_dir_set(const PencilArgs<T>& a0, int n, hipStream_t s) {
     constexpr int dir = DIR;
     PencilArgs<T> a = a0;
     a.n = n;
@@ -1196,7 +1361,7 @@ hipError_t launch_pencils_dir(const PencilArgs<T>& a0, int n, hipStream_t s) {
     const long blocks = pencil_blocks(a.seg, a.nseg, a.TW * a.lpt, &a.tiles0);
     const int thr = a.TW * threads_per_line(n, a.variant);
     const size_t smem = lds_elems(n, a.TW) * sizeof(cpx<T>);
-    return dispatch_size(n, [&](auto NC) -> hipError_t {
+    return dispatch_size<SET>(n, [&](auto NC) -> hipError_t {
         constexpr int NN = decltype(NC)::value;
         auto go = [&](auto VC) -> hipError_t {
             constexpr int VV = decltype(VC)::value;
@@ -1252,8 +1417,8 @@ hipError_t launch_bluestein(const BluesteinArgs<T>& b0, int n, hipStream_t s) {
     }
 }
 
-template <typename T>
-hipError_t launch_pencil_update(const PencilUpdateArgs<T>& u0, int n, hipStream_t s) {
+template <typename T, int SET>
+hipError_t launch_pencil_update_set(const PencilUpdateArgs<T>& u0, int n, hipStream_t s) {
     PencilUpdateArgs<T> u = u0;
     u.p.n = n;
     u.p.lpt = 1;
@@ -1261,58 +1426,67 @@ hipError_t launch_pencil_update(const PencilUpdateArgs<T>& u0, int n, hipStream_
     const int thr = u.p.TW * threads_per_line(n, 1);
     size_t smem = lds_elems(n, u.p.TW) * sizeof(cpx<T>);
     if (u.egrad_partial) smem += (size_t)thr * sizeof(double);
-    return dispatch_size(n, [&](auto NC) -> hipError_t {
+    return dispatch_size<SET>(n, [&](auto NC) -> hipError_t {
         constexpr int NN = decltype(NC)::value;
         if (u.egrad_partial) return launch_k(k_pencil_update<T, NN, true>, dim3(blocks), dim3(thr), smem, s, u);
         return launch_k(k_pencil_update<T, NN, false>, dim3(blocks), dim3(thr), smem, s, u);
     });
 }
 
-template <typename T>
-hipError_t launch_rows_r2c(const RowR2CArgs<T>& a, hipStream_t s) {
+template <typename T, int SET>
+hipError_t launch_rows_r2c_set(const RowR2CArgs<T>& a, hipStream_t s) {
     const long blocks = rows_blocks(a.rows, a.TW);
     const int thr = a.TW * threads_per_line(a.nx);
-    const size_t smem = lds_elems(a.nx, a.TW) * sizeof(cpx<T>);
-    return dispatch_size(a.nx, [&](auto NC) -> hipError_t {
+    // exchange lines and the block's copy of the twiddle table
+    const size_t smem = (lds_elems(a.nx, a.TW) + a.nx) * sizeof(cpx<T>);
+    return dispatch_size<SET>(a.nx, [&](auto NC) -> hipError_t {
         constexpr int NN = decltype(NC)::value;
         return launch_k(k_rows_r2c<T, NN>, dim3(blocks), dim3(thr), smem, s, a);
     });
 }
 
-template <typename T>
-hipError_t launch_rows_c2r(const RowC2RArgs<T>& a, hipStream_t s) {
+template <typename T, int SET>
+hipError_t launch_rows_c2r_set(const RowC2RArgs<T>& a, hipStream_t s) {
     const long blocks = rows_blocks(a.rows, a.TW);
     const int thr = a.TW * threads_per_line(a.nx);
-    const size_t smem = lds_elems(a.nx, a.TW) * sizeof(cpx<T>);
-    return dispatch_size(a.nx, [&](auto NC) -> hipError_t {
+    // exchange lines and the block's copy of the twiddle table
+    const size_t smem = (lds_elems(a.nx, a.TW) + a.nx) * sizeof(cpx<T>);
+    return dispatch_size<SET>(a.nx, [&](auto NC) -> hipError_t {
         constexpr int NN = decltype(NC)::value;
         return launch_k(k_rows_c2r<T, NN>, dim3(blocks), dim3(thr), smem, s, a);
     });
 }
 
-template <typename T>
-hipError_t launch_rows_sync(const RowSyncArgs<T>& a, hipStream_t s) {
+template <typename T, int SET>
+hipError_t launch_rows_sync_set(const RowSyncArgs<T>& a, hipStream_t s) {
     if (a.nd < 1 || a.nd > kMaxArr || !a.fmain[0]) return hipErrorInvalidValue;
     if (a.mode == 1 && (!a.phi || a.nd != 1 || a.fmain[1])) return hipErrorInvalidValue;
     if (a.mode == 2 && (!a.fmain[1] || !a.fout[0] || a.nd != 1 || a.npoly < 0 || a.npoly > kMaxPoly))
         return hipErrorInvalidValue;
     const long blocks = rows_blocks(a.rows, a.TW);
     const int thr = a.TW * threads_per_line(a.nx);
-    const size_t smem = lds_elems(a.nx, a.TW) * sizeof(cpx<T>);
-    return dispatch_size(a.nx, [&](auto NC) -> hipError_t {
+    // exchange lines and the block's copy of the twiddle table
+    const size_t smem = (lds_elems(a.nx, a.TW) + a.nx) * sizeof(cpx<T>);
+    const int mode = (a.mode == 0) ? (a.nd <= 2 ? 0 : 3) : a.mode;
+    return dispatch_size<SET>(a.nx, [&](auto NC) -> hipError_t {
         constexpr int NN = decltype(NC)::value;
-        return launch_k(k_rows_sync<T, NN>, dim3(blocks), dim3(thr), smem, s, a);
+        switch (mode) {
+            case 0: return launch_k(k_rows_sync<T, NN, 0>, dim3(blocks), dim3(thr), smem, s, a);
+            case 1: return launch_k(k_rows_sync<T, NN, 1>, dim3(blocks), dim3(thr), smem, s, a);
+            case 2: return launch_k(k_rows_sync<T, NN, 2>, dim3(blocks), dim3(thr), smem, s, a);
+            default: return launch_k(k_rows_sync<T, NN, 3>, dim3(blocks), dim3(thr), smem, s, a);
+        }
     });
 }
 
-template <typename T>
-hipError_t launch_rows_fused(const RowFusedArgs<T>& a, hipStream_t s) {
+template <typename T, int SET>
+hipError_t launch_rows_fused_set(const RowFusedArgs<T>& a, hipStream_t s) {
     const long blocks = rows_fused_blocks(a.rows, a.TW);
     const int thr = a.TW * threads_per_line(a.nx, a.variant);
     // exchange lines, the block's copy of the twiddle table, the monitor's partial sums
     size_t smem = (lds_elems(a.nx, a.TW) + a.nx) * sizeof(cpx<T>);
     if (a.bulk_partial) smem += (size_t)thr * sizeof(double);
-    return dispatch_size(a.nx, [&](auto NC) -> hipError_t {
+    return dispatch_size<SET>(a.nx, [&](auto NC) -> hipError_t {
         constexpr int NN = decltype(NC)::value;
         if constexpr (plain_variant<NN>() == 0) {
             if (!a.variant) {
@@ -1323,6 +1497,50 @@ hipError_t launch_rows_fused(const RowFusedArgs<T>& a, hipStream_t s) {
         if (a.bulk_partial) return launch_k(k_rows_fused<T, NN, 1, true>, dim3(blocks), dim3(thr), smem, s, a);
         return launch_k(k_rows_fused<T, NN, 1, false>, dim3(blocks), dim3(thr), smem, s, a);
     });
+}
+
+// the public launch functions: extended lengths go to the set-1 instantiations, which live
+// in units of their own (spectral_ext*.hip)
+#define GPF_EXT_DECL(T)                                                                                  \
+    extern template hipError_t launch_pencils_dir_set<T, -1, 1>(const PencilArgs<T>&, int, hipStream_t);  \
+    extern template hipError_t launch_pencils_dir_set<T, 1, 1>(const PencilArgs<T>&, int, hipStream_t);   \
+    extern template hipError_t launch_pencil_update_set<T, 1>(const PencilUpdateArgs<T>&, int, hipStream_t); \
+    extern template hipError_t launch_rows_r2c_set<T, 1>(const RowR2CArgs<T>&, hipStream_t);             \
+    extern template hipError_t launch_rows_c2r_set<T, 1>(const RowC2RArgs<T>&, hipStream_t);             \
+    extern template hipError_t launch_rows_sync_set<T, 1>(const RowSyncArgs<T>&, hipStream_t);           \
+    extern template hipError_t launch_rows_fused_set<T, 1>(const RowFusedArgs<T>&, hipStream_t);
+GPF_EXT_DECL(float)
+GPF_EXT_DECL(double)
+#undef GPF_EXT_DECL
+
+template <typename T, int DIR>
+hipError_t launch_pencils_dir(const PencilArgs<T>& a, int n, hipStream_t s) {
+    return ext_length(n) ? launch_pencils_dir_set<T, DIR, 1>(a, n, s) : launch_pencils_dir_set<T, DIR, 0>(a, n, s);
+}
+
+template <typename T>
+hipError_t launch_pencil_update(const PencilUpdateArgs<T>& u, int n, hipStream_t s) {
+    return ext_length(n) ? launch_pencil_update_set<T, 1>(u, n, s) : launch_pencil_update_set<T, 0>(u, n, s);
+}
+
+template <typename T>
+hipError_t launch_rows_r2c(const RowR2CArgs<T>& a, hipStream_t s) {
+    return ext_length(a.nx) ? launch_rows_r2c_set<T, 1>(a, s) : launch_rows_r2c_set<T, 0>(a, s);
+}
+
+template <typename T>
+hipError_t launch_rows_c2r(const RowC2RArgs<T>& a, hipStream_t s) {
+    return ext_length(a.nx) ? launch_rows_c2r_set<T, 1>(a, s) : launch_rows_c2r_set<T, 0>(a, s);
+}
+
+template <typename T>
+hipError_t launch_rows_sync(const RowSyncArgs<T>& a, hipStream_t s) {
+    return ext_length(a.nx) ? launch_rows_sync_set<T, 1>(a, s) : launch_rows_sync_set<T, 0>(a, s);
+}
+
+template <typename T>
+hipError_t launch_rows_fused(const RowFusedArgs<T>& a, hipStream_t s) {
+    return ext_length(a.nx) ? launch_rows_fused_set<T, 1>(a, s) : launch_rows_fused_set<T, 0>(a, s);
 }
 
 }  // namespace gpf

@@ -1,5 +1,5 @@
-// spectral_pencils_fwd64.hip -- FP64 instantiations of the forward pencil transforms k_pencils (every
-// length, schedule); a unit of its own so that the families compile in parallel.
+// spectral_pencils_fwd64.hip -- FP64 instantiations of the forward pencil transforms k_pencils (the core
+// lengths, every schedule); a unit of its own so that the families compile in parallel.
 #include "spectral_kernels.h"
 
 namespace gpf {

@@ -1,4 +1,4 @@
-// spectral_r2c.hip -- FP32 instantiations of the real-to-half-spectrum row transforms k_rows_r2c (every length).
+// spectral_r2c.hip -- FP32 instantiations of the real-to-half-spectrum row transforms k_rows_r2c (the core lengths).
 #include "spectral_kernels.h"
 
 namespace gpf {

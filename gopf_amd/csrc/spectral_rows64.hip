@@ -5,8 +5,7 @@
 namespace gpf {
 
 #define GPF_INST(T) \
-    template hipError_t launch_rows_fused<T>(const RowFusedArgs<T>&, hipStream_t); \
-    template hipError_t launch_rows_fused_group<T>(const BatchGroupArgs<T>&, int, size_t, hipStream_t);
+    template hipError_t launch_rows_fused<T>(const RowFusedArgs<T>&, hipStream_t);
 GPF_INST(double)
 #undef GPF_INST
 

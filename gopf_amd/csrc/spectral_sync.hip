@@ -1,4 +1,4 @@
-// spectral_sync.hip -- FP32 instantiations of the derived-field row pass k_rows_sync (every length).
+// spectral_sync.hip -- FP32 instantiations of the derived-field row pass k_rows_sync (the core lengths).
 #include "spectral_kernels.h"
 
 namespace gpf {

@@ -1,11 +1,10 @@
-// spectral_update.hip -- FP32 instantiations of k_pencil_update (every length, with and without the monitor).
+// spectral_update.hip -- FP32 instantiations of k_pencil_update (the core lengths, with and without the monitor).
 #include "spectral_kernels.h"
 
 namespace gpf {
 
 #define GPF_INST(T) \
-    template hipError_t launch_pencil_update<T>(const PencilUpdateArgs<T>&, int, hipStream_t); \
-    template hipError_t launch_pencil_update_group<T>(const BatchGroupArgs<T>&, int, size_t, hipStream_t);
+    template hipError_t launch_pencil_update<T>(const PencilUpdateArgs<T>&, int, hipStream_t);
 GPF_INST(float)
 #undef GPF_INST
 

@@ -5,8 +5,7 @@
 namespace gpf {
 
 #define GPF_INST(T) \
-    template hipError_t launch_pencil_update<T>(const PencilUpdateArgs<T>&, int, hipStream_t); \
-    template hipError_t launch_pencil_update_group<T>(const BatchGroupArgs<T>&, int, size_t, hipStream_t);
+    template hipError_t launch_pencil_update<T>(const PencilUpdateArgs<T>&, int, hipStream_t);
 GPF_INST(double)
 #undef GPF_INST
 
