@@ -321,18 +321,19 @@ def cpu_baseline_c4(W, N: int, steps: int):
 def c4_traffic(N: int, s: int) -> dict:
     """Algorithmic HBM bytes of one step of the two-field precipitate RK4 step, every array
     counted once per pass that reads or writes it.  R = a real array (N^3 s bytes), H = a
-    half spectrum ((N/2+1) N^2 complex values), K = the elastic multiplier table ((N/2+1) N^2
-    doubles).  Per step: the derived fields' transform (rows with the polynomials inline,
-    y, z) 2R+4H read / 6H written; the elastic term (rows of phi with the indicator inline,
-    y, z; the K(k) multiply; z, y inverse; the fused row pass of H'(phi)(S - 2 e* H(phi)),
-    which reads phi; y, z) 2R+8H+K / 9H; eight RK4 stage passes (conc 13H / 8H, phase
-    17H / 8H); three between-stage syncs (z, y inverse of both fields, the fused row pass,
-    y, z forward of both derived fields) 30H / 30H; the last inverse (z, y, rows) 6H /
-    4H+2R; the lagged-field copy R / R."""
+    half spectrum ((N/2+1) N^2 complex values), K = a per-k table in the field precision
+    ((N/2+1) N^2 values of s bytes: the elastic multiplier, and each RK4 stage's factor).
+    Per step: the derived fields' transform (rows with the polynomials inline, y, z) 2R+4H
+    read / 6H written; the elastic term (rows of phi with the indicator inline, y, z; the
+    K(k) multiply; z, y inverse; the fused row pass of H'(phi)(S - 2 e* H(phi)), which reads
+    phi; y, z) 2R+8H+K / 9H; eight RK4 stage passes (conc 13H / 8H, phase 17H / 8H, and the
+    stage factor table, one K per pass); three between-stage syncs (z, y inverse of both
+    fields, the fused row pass, y, z forward of both derived fields) 30H / 30H; the last
+    inverse (z, y, rows) 6H / 4H+2R; the lagged-field copy R / R."""
     R = N ** 3 * s
     H = (N // 2 + 1) * N * N * 2 * s
-    K = (N // 2 + 1) * N * N * 8
-    rd = 5 * R + 78 * H + K
+    K = (N // 2 + 1) * N * N * s
+    rd = 5 * R + 78 * H + 9 * K
     wr = 3 * R + 65 * H
     return {"read": rd, "write": wr, "total": rd + wr, "per_voxel_step": (rd + wr) / N ** 3}
 

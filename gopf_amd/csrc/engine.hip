@@ -248,10 +248,10 @@ static bool tw_fits(int n, int tw, size_t elem, int variant = 0) {
     return tw * T <= kMaxBlockThreads && lds <= 160 * 1024;
 }
 
-static int choose_tw(int n, int narr, size_t elem, int min_tw, int variant = 0) {
+static int choose_tw(int n, int narr, size_t elem, int min_tw, int variant = 0, int threads = 256) {
     const int T = fft_threads_per_line(n, variant);
     int tw = 1;
-    while (tw * 2 * T <= 256) tw *= 2;  // lines per block: a power of two
+    while (tw * 2 * T <= threads) tw *= 2;  // lines per block: a power of two
     if (tw > 64) tw = 64;
     while (tw < min_tw && tw * 2 * T <= 256) tw *= 2;
     while (tw * T > kMaxBlockThreads && tw > 1) tw /= 2;
@@ -592,7 +592,7 @@ private:
     std::vector<Buf<T>> stab_;
     bool stab_ok_ = false;
     void build_stage_tables();
-    std::vector<Buf<double>> Ktab_;  // K(k) of each elastic term, nk values
+    std::vector<Buf<T>> Ktab_;  // K(k) of each elastic term, nk values (built in FP64, stored in T)
     std::vector<Buf<T>> lag_;
     // VolumeConservingLP: indicator brick, field, device state {Multiplier, CurrentIntegral,
     // IsFirstUpdate} per occurrence, and the partial sums of its field integral
@@ -916,7 +916,9 @@ void Engine<T>::pencils(const std::vector<cpx<T>*>& dst, const std::vector<const
         a.seg[0] = (axis == 1) ? seg_y_main(g_) : seg_z_main(g_);
         a.seg[1] = (axis == 1) ? seg_y_nyq(g_) : seg_z_nyq(g_);
         a.nseg = 2;
-        a.TW = choose_tw(n, a.narr, sizeof(cpx<T>), 8);
+        // about 128 threads per block: at 192^3 (C4) 8 lines per block beat 16 by 1 % (FP32)
+        // and 3 % (FP64) per step
+        a.TW = choose_tw(n, a.narr, sizeof(cpx<T>), 8, 0, 128);
         for (int q = 0; q < a.narr; ++q) {
             a.arr[q][0] = dst[q0 + q];
             a.arr[q][1] = dst[q0 + q] + g_.nmain;

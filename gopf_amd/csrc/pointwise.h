@@ -234,7 +234,7 @@ struct KElasticArgs {
     ElasticParams ep;
     KGrid g;
     long nmain = 0, nk = 0;
-    const double* ktab = nullptr;  // K(k), filled once by launch_kelastic_table
+    const T* ktab = nullptr;       // K(k), filled once by launch_kelastic_table
 };
 
 // adjoint of the fused semi-implicit step (single field)
@@ -324,7 +324,7 @@ hipError_t launch_sdd_scalars(int mode, const double* part, double* sc, double* 
 template <typename T> hipError_t launch_etd_residual(const EtdArgs<T>& a, hipStream_t s);
 template <typename T> hipError_t launch_kgrad(const KGradArgs<T>& a, int ndim, hipStream_t s);
 template <typename T> hipError_t launch_kelastic(const KElasticArgs<T>& a, hipStream_t s);
-template <typename T> hipError_t launch_kelastic_table(const KElasticArgs<T>& a, double* ktab, hipStream_t s);
+template <typename T> hipError_t launch_kelastic_table(const KElasticArgs<T>& a, T* ktab, hipStream_t s);
 // VolumeConservingLP.OnStepFinished (volumeConserving.go:31-50) on device state
 // st = {Multiplier, CurrentIntegral, IsFirstUpdate}: the field integral is the sum of the
 // partials px[0..nb), the indicator integral *ind (the zero mode of a k-space derived brick,

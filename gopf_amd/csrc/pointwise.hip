@@ -649,22 +649,23 @@ __device__ double elastic_kernel(const ElasticParams& ep, const double f[3]) {
     return K;
 }
 
-// K(k) depends only on the frequencies and the material, so it is tabulated once
+// K(k) depends only on the frequencies and the material, so it is tabulated once (in FP64,
+// stored in T)
 template <typename T>
-__global__ void k_kelastic_table(KElasticArgs<T> a, double* __restrict__ ktab) {
+__global__ void k_kelastic_table(KElasticArgs<T> a, T* __restrict__ ktab) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
         int kx, ky, kz;
         kcoords(a.g, i, a.nmain, kx, ky, kz);
         double f[3];
         ref_freq(a.g, kx, ky, kz, f);
-        ktab[i] = elastic_kernel(a.ep, f);
+        ktab[i] = (T)elastic_kernel(a.ep, f);
     }
 }
 
 template <typename T>
 __global__ void k_kelastic(KElasticArgs<T> a) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.nk; i += (long)gridDim.x * blockDim.x) {
-        const double K = a.ktab[i];
+        const double K = (double)a.ktab[i];
         const cpx<T> s = a.src[i];
         a.dst[i] = mk<T>((T)(K * (double)s.x), (T)(K * (double)s.y));
     }
@@ -1154,7 +1155,7 @@ hipError_t launch_kgrad(const KGradArgs<T>& a, int ndim, hipStream_t s) {
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_kelastic_table(const KElasticArgs<T>& a, double* ktab, hipStream_t s) {
+hipError_t launch_kelastic_table(const KElasticArgs<T>& a, T* ktab, hipStream_t s) {
     hipLaunchKernelGGL(k_kelastic_table<T>, dim3(grid_for(a.nk)), dim3(256), 0, s, a, ktab);
     return hipGetLastError();
 }
@@ -1319,7 +1320,7 @@ hipError_t launch_reduce2(const double* a, long na, double ca, const double* b, 
     template hipError_t launch_volcons_update<T>(const double*, long, const T*, double, double*, hipStream_t); \
     template hipError_t launch_kgrad<T>(const KGradArgs<T>&, int, hipStream_t);                               \
     template hipError_t launch_kelastic<T>(const KElasticArgs<T>&, hipStream_t);                              \
-    template hipError_t launch_kelastic_table<T>(const KElasticArgs<T>&, double*, hipStream_t);               \
+    template hipError_t launch_kelastic_table<T>(const KElasticArgs<T>&, T*, hipStream_t);                    \
     template hipError_t launch_sqsum<T>(const T* const*, int, T, T*, long, hipStream_t);                      \
     template hipError_t launch_copy<T>(const T*, T*, long, hipStream_t);                                      \
     template hipError_t launch_real_sums<T>(const T*, long, double, int, const double*, double*, double*,     \
