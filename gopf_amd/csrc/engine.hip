@@ -208,7 +208,10 @@ struct Geometry {
             ny = d[1];
             nx = d[2];
         }
-        if (nx % 2) throw Error(GPF_EINVAL, "the fastest extent must be even (real-to-half-spectrum layout)");
+        // the half spectrum keeps columns kx = 0 .. nx / 2; for odd nx the last one is an
+        // ordinary frequency (its mirror is -kx), handled by the direct-DFT row kernels
+        if (nx % 2 && nx > 64)
+            throw Error(GPF_EINVAL, "an odd fastest extent must be at most 64 (no compiled odd-length schedule)");
         for (int v : {nx, ny, nz})
             if (v > 1 && !fft_size_supported(v))
                 throw Error(GPF_EINVAL, "unsupported transform length " + std::to_string(v));

@@ -669,6 +669,7 @@ __device__ __forceinline__ void load_herm(cpx<T>* v, const cpx<T>* __restrict__ 
         if (valid) {
             if (p < nh) x = m[row * nh + p];
             else if (p == nh) x = q[row];
+            else if (n - p == nh) x = conjg(q[row]);  // odd n: the mirror is the stored column kx = nh
             else x = conjg(m[row * nh + (n - p)]);
         }
         v[e] = x;
@@ -705,6 +706,10 @@ __device__ __forceinline__ void load_herm_pair(cpx<T>* v, const cpx<T>* m, const
         } else if (p == nh) {
             if (v1) A = q[r1];
             if (v2) B = q[r1 + 1];
+        } else if (n - p == nh) {
+            // odd n: the mirror of p is the stored column kx = nh, a frequency of its own
+            if (v1) A = conjg(q[r1]);
+            if (v2) B = conjg(q[r1 + 1]);
         } else {
             if (v1) A = conjg(b1[-lo]);
             if (v2) B = conjg(b1[nh - lo]);

@@ -452,17 +452,60 @@ def test_ch_radix5_radix7_and_2048_lengths_match_oracle(oracle_lib, dims):
     assert err <= 1e-10, (dims, err)
 
 
-def test_odd_fastest_extent_is_refused():
-    """The device stores real fields as half spectra of the fastest axis, packed two rows per
-    transform; an odd fastest extent (a 9 x 15 grid) is refused with a message."""
-    import gopf_amd as G
-    from gopf_amd import workloads as W
+@pytest.mark.parametrize("dims,precision", [([9, 15], "fp64"), ([15, 9], "fp64"), ([24, 33], "fp64"),
+                                            ([15, 15, 15], "fp64"), ([9, 15], "fp32")])
+def test_odd_fastest_extent_matches_oracle(oracle_lib, dims, precision):
+    """An odd fastest extent, as FFTW plans it (pfutil/fftWrap.go:16-23): the half spectrum
+    keeps columns kx = 0 .. nx/2, whose last one is then an ordinary frequency with a mirror
+    at -kx.  The fused Cahn-Hilliard step (direct-DFT rows for lengths <= 64), 20 Euler
+    steps against the oracle: <= 1e-10 FP64, <= 1e-5 relative L2 FP32."""
+    from oracle.engine import OracleSolver
+    print("odd fastest extent", dims, precision, flush=True)
+    n = int(np.prod(dims))
+    c0 = ch_init(n, 5)
+    pm, om = pair_models({"conc": c0}, CH_SCALARS, [CH_EQ], n)
+    s = G.Solver(pm, dims, 0.05, precision=precision)
+    assert s.fused
+    s.StepperPropagate(20)
+    o = OracleSolver(om, dims, {"conc": c0}, 0.05)
+    o.stepper_propagate(20)
+    got, ref = s.Field(0).real, o.field("conc").real
+    if precision == "fp64":
+        assert maxrel(got, ref) <= 1e-10
+    else:
+        assert relL2(got, ref) <= 1e-5
+    assert maxrel(ref, c0) > 1e-3
+
+
+def test_odd_fastest_extent_two_fields_rk4_matches_oracle(oracle_lib):
+    """The general path (RK4, two fields, dense derived fields in the stages' row pass) on a
+    12 x 15 grid against the oracle, 20 steps, <= 1e-10."""
+    from oracle.engine import OracleSolver
+    dims, n = [12, 15], 12 * 15
+    rng = np.random.default_rng(23)
+    fields = {"phi": 0.5 + 0.05 * rng.standard_normal(n), "psi": 0.3 + 0.05 * rng.standard_normal(n)}
+    eqs = ["dphi/dt = ka*LAP phi - ka*phi^2 + kb*LAP psi^2", "dpsi/dt = kb*LAP psi + phi*psi - psi^2"]
+    scal = {"ka": 0.5, "kb": 1.0}
+    pm, om = pair_models(fields, scal, eqs, n)
+    s = G.Solver(pm, dims, 0.05, stepper="rk4")
+    assert not s.fused
+    s.StepperPropagate(20)
+    o = OracleSolver(om, dims, fields, 0.05, "rk4")
+    o.stepper_propagate(20)
+    for f in fields:
+        assert maxrel(s.Field(f).real, o.field(f).real) <= 1e-10, f
+
+
+def test_odd_fastest_extent_beyond_64_is_refused():
+    """Odd lengths above 64 have no compiled schedule (the direct DFT covers <= 64)."""
     m = G.Model()
-    m.AddField("conc", W.ch_initial(3, 135))
-    for k, v in W.ch_scalars().items():
+    m.AddField("conc", ch_init(16 * 65, 3))
+    for k, v in CH_SCALARS.items():
         m.AddScalar(k, v)
-    m.AddEquation(W.CH_EQUATION)
+    m.AddEquation(CH_EQ)
     m.Init()
     with pytest.raises(G.GpfError) as ei:
-        G.Solver(m, [9, 15], 0.1)
-    assert ei.value.code == G.GPF_EINVAL and "even" in str(ei.value)
+        G.Solver(m, [16, 65], 0.1)
+    assert ei.value.code == G.GPF_EINVAL and "odd" in str(ei.value)
+
+
