@@ -329,12 +329,16 @@ def c4_traffic(N: int, s: int) -> dict:
     phi; y, z) 2R+8H+K / 9H; eight RK4 stage passes (conc 13H / 8H, phase 17H / 8H, and the
     stage factor table, one K per pass); three between-stage syncs (z, y inverse of both
     fields, the fused row pass, y, z forward of both derived fields) 30H / 30H; the last
-    inverse (z, y, rows) 6H / 4H+2R; the lagged-field copy R / R."""
+    inverse (z, y, rows) 6H / 4H+2R; the lagged-field copy R / R.
+    FP64 (s = 8) does each stage's update in the fields' first inverse pencil pass: the
+    corrected spectra are neither read back by that pass (8H) nor, before the last stage,
+    written (6H)."""
     R = N ** 3 * s
     H = (N // 2 + 1) * N * N * 2 * s
     K = (N // 2 + 1) * N * N * s
-    rd = 5 * R + 78 * H + 9 * K
-    wr = 3 * R + 65 * H
+    fused = s == 8
+    rd = 5 * R + (70 if fused else 78) * H + 9 * K
+    wr = 3 * R + (59 if fused else 65) * H
     return {"read": rd, "write": wr, "total": rd + wr, "per_voxel_step": (rd + wr) / N ** 3}
 
 

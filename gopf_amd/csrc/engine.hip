@@ -487,13 +487,16 @@ public:
 private:
     // transforms
     const cpx<T>* tw(int n);
-    void pencils(const std::vector<cpx<T>*>& dst, const std::vector<const cpx<T>*>& src, int axis, int dir);
+    // stage (optional, inverse passes): the fields' RK4 stage update done as they are loaded
+    // (launch_pencils_stage); src is then unused
+    void pencils(const std::vector<cpx<T>*>& dst, const std::vector<const cpx<T>*>& src, int axis, int dir,
+                 const PencilStageArgs<T>* stage = nullptr);
     // prog (optional): per input, a dense derived-field program evaluated while loading
     // instead of reading the input array
     void r2c(const std::vector<const T*>& in, const std::vector<cpx<T>*>& out, T in_scale,
              const std::vector<const DerivedArgs<T>*>* prog = nullptr);
     void c2r(const std::vector<const cpx<T>*>& in, const std::vector<T*>& out, T scale,
-             const double* add_mean = nullptr);
+             const double* add_mean = nullptr, const PencilStageArgs<T>* stage = nullptr);
     cpx<T>* kscratch(int i);
     T* rscratch(int i);
     // timing
@@ -529,7 +532,8 @@ private:
     // general path
     void general_step_euler();
     void general_step_rk4();
-    void sync_from_spectra();
+    void sync_from_spectra(const PencilStageArgs<T>* stage = nullptr);
+    bool fused_stage_args(int stage, PencilStageArgs<T>* out);
     void ensure_k_fields();
     void sync_derived();
     void eval_user_terms_for(int eq);
@@ -910,8 +914,11 @@ int Engine<T>::pass_times(int maxpass, double* ms, long* launches, char* names, 
 }
 
 template <typename T>
-void Engine<T>::pencils(const std::vector<cpx<T>*>& dst, const std::vector<const cpx<T>*>& src, int axis, int dir) {
+void Engine<T>::pencils(const std::vector<cpx<T>*>& dst, const std::vector<const cpx<T>*>& src, int axis, int dir,
+                        const PencilStageArgs<T>* stage) {
     const int n = (axis == 1) ? g_.ny : g_.nz;
+    if (stage && (dir != 1 || dst.size() > (size_t)kMaxArr))
+        throw Error(GPF_EINVAL, "fused stage pencils: one inverse pass of at most 4 fields");
     for (size_t q0 = 0; q0 < dst.size(); q0 += kMaxArr) {
         PencilArgs<T> a;
         a.tw = tw(n);
@@ -928,6 +935,14 @@ void Engine<T>::pencils(const std::vector<cpx<T>*>& dst, const std::vector<const
             if (!src.empty() && src[q0 + q]) {
                 a.src[q][0] = src[q0 + q];
                 a.src[q][1] = src[q0 + q] + g_.nmain;
+            }
+        }
+        if (stage) {
+            if constexpr (sizeof(T) == 8) {
+                PencilStageArgs<T> sa = *stage;
+                sa.p = a;
+                HIPCHK(launch_pencils_stage<T>(sa, n, st_));
+                continue;
             }
         }
         HIPCHK(launch_pencils<T>(a, n, dir, st_));
@@ -966,7 +981,8 @@ void Engine<T>::r2c(const std::vector<const T*>& in, const std::vector<cpx<T>*>&
 
 template <typename T>
 void Engine<T>::c2r(const std::vector<const cpx<T>*>& in, const std::vector<T*>& out, T scale,
-                    const double* add_mean) {
+                    const double* add_mean, const PencilStageArgs<T>* stage) {
+    if (stage && in.size() > (size_t)kMaxArr) throw Error(GPF_EINVAL, "fused stage pencils: at most 4 fields");
     for (size_t q0 = 0; q0 < in.size(); q0 += kMaxArr) {
         const int na = (int)std::min(in.size() - q0, (size_t)kMaxArr);
         std::vector<cpx<T>*> scr;
@@ -976,10 +992,10 @@ void Engine<T>::c2r(const std::vector<const cpx<T>*>& in, const std::vector<T*>&
             src.push_back(in[q0 + q]);
         }
         if (g_.ndim == 3) {
-            pencils(scr, src, 2, +1);
+            pencils(scr, src, 2, +1, stage);
             pencils(scr, {}, 1, +1);
         } else {
-            pencils(scr, src, 1, +1);
+            pencils(scr, src, 1, +1, stage);
         }
         RowC2RArgs<T> a;
         a.tw = tw(g_.nx);
@@ -1824,7 +1840,7 @@ void Engine<T>::general_step_euler() {
 // (the last stage writes them).  Used when every derived field is a dense polynomial of
 // the (at most two) fields.
 template <typename T>
-void Engine<T>::sync_from_spectra() {
+void Engine<T>::sync_from_spectra(const PencilStageArgs<T>* stage) {
     std::vector<cpx<T>*> scr;
     std::vector<const cpx<T>*> src;
     for (int i = 0; i < nf_; ++i) {
@@ -1832,10 +1848,10 @@ void Engine<T>::sync_from_spectra() {
         src.push_back(K_[i].get());
     }
     if (g_.ndim == 3) {
-        pencils(scr, src, 2, +1);
+        pencils(scr, src, 2, +1, stage);
         pencils(scr, {}, 1, +1);
     } else {
-        pencils(scr, src, 1, +1);
+        pencils(scr, src, 1, +1, stage);
     }
     RowSyncArgs<T> a;
     a.tw = tw(g_.nx);
@@ -1890,13 +1906,66 @@ void Engine<T>::build_stage_tables() {
     stab_ok_ = true;
 }
 
+// A stage's update done by the fields' first inverse pencil pass (launch_pencils_stage)
+// instead of its own pass: when every field's equation has at most kStageTerms terms of
+// tmode 0, none formed per stage (SquaredGradient), a stage factor table, and the first
+// inverse axis has a core length.  The pass reads the operands the stage kernel reads and
+// does its arithmetic, so the result is the same; the corrected spectrum is stored only
+// after the last stage (the next step's state) or when an equation reads a field's
+// spectrum as a brick.
+template <typename T>
+bool Engine<T>::fused_stage_args(int stage, PencilStageArgs<T>* out) {
+    if (sizeof(T) != 8 || !stab_ok_ || nf_ > kMaxArr) return false;  // FP64 only (spectral_stage.hip)
+    if (!stage_pencil_length(g_.ndim == 3 ? g_.nz : g_.ny)) return false;
+    const double fac_prep[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    const double fac_corr[3] = {0.5, 0.5, 1.0};
+    PencilStageArgs<T> a;
+    a.stage = stage;
+    a.dt = dt_;
+    a.fp = fac_prep[stage];
+    a.fc = stage < 3 ? fac_corr[stage] : 0.0;
+    a.g = g_.kg;
+    a.nmain = g_.nmain;
+    a.nk = g_.nk;
+    bool field_brick = false;
+    for (int i = 0; i < nf_; ++i) {
+        for (const auto& t : eqs_[i].terms)
+            if (t.kind == kSquaredGradient || t.kind == kConservativeNoise) return false;
+        Rk4StageArgs<T> r;
+        terms_into(i, r);
+        if (r.nterm > kStageTerms) return false;
+        const int kind = (stage < 2) ? 0 : (stage == 3 && has_filter_) ? 2 : 1;
+        StageLine<T>& f = a.f[i];
+        f.stab = stab_[3 * i + kind].get();
+        if (!f.stab) return false;
+        f.k = K_[i].get();
+        f.init = INIT_[i].get();
+        f.final = FINAL_[i].get();
+        f.kn = KF_[i].get();
+        f.nterm = r.nterm;
+        for (int q = 0; q < r.nterm; ++q) {
+            if (r.tmode[q] != 0) return false;
+            f.brick[q] = r.brick[q];
+            f.coef[q] = r.coef[q];
+            f.pw[q] = r.pw[q];
+            f.dmul[q] = r.dmul[q];
+            a.need_l |= (r.pw[q] != 0) ? 1 : 0;
+            for (int j = 0; j < nf_; ++j) field_brick |= (r.brick[q] == K_[j].get());
+        }
+    }
+    a.store_kn = (stage == 3 || field_brick) ? 1 : 0;
+    *out = a;
+    return true;
+}
+
 template <typename T>
 void Engine<T>::general_step_rk4() {
     // RK4.Step (rk4.go:29-127) with each stage's GetRHS, PrepareNextCorrection and
     // correction fused into one k-space pass per field: the stage's right-hand side is
     // evaluated from the bricks inside the pass instead of being stored.  The corrected
     // spectra go to KF_ (free, as the right-hand side is never stored) and swap with K_
-    // after all fields, so every field of a stage reads the same bricks.
+    // after all fields, so every field of a stage reads the same bricks.  Where possible
+    // (fused_stage_args) that pass is the fields' first inverse pencil pass itself.
     if (!stab_ok_) build_stage_tables();
     ensure_k_fields();
     sync_derived();
@@ -1904,7 +1973,9 @@ void Engine<T>::general_step_rk4() {
     const double fac_prep[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
     const double fac_corr[3] = {0.5, 0.5, 1.0};
     for (int stage = 0; stage < 4; ++stage) {
-        for (int i = 0; i < nf_; ++i) {
+        PencilStageArgs<T> fs;
+        const bool fused = fused_stage_args(stage, &fs);
+        for (int i = 0; i < nf_ && !fused; ++i) {
             eval_user_terms_for(i);
             Rk4StageArgs<T> a;
             terms_into(i, a);
@@ -1937,10 +2008,10 @@ void Engine<T>::general_step_rk4() {
             out.push_back(F_[i].get());
         }
         if (stage < 3 && row_sync_) {
-            sync_from_spectra();
+            sync_from_spectra(fused ? &fs : nullptr);
             continue;
         }
-        c2r(in, out, T(1) / (T)g_.n);
+        c2r(in, out, T(1) / (T)g_.n, nullptr, fused ? &fs : nullptr);
         if (stage < 3) sync_derived();
     }
 }

@@ -242,6 +242,38 @@ struct RowFusedArgs {
     const double* gpoly_sim = nullptr;
 };
 
+// One field's RK4 stage (rk4.go:35-127) done as the field's values are loaded by its first
+// inverse pencil pass: the stage's right-hand side from at most kStageTerms bricks of tmode 0,
+// the stage factor from its table, FINAL / INIT updated in place, and the corrected spectrum
+// transformed (and stored to kn when store_kn) instead of being written and read back.
+constexpr int kStageTerms = 4;
+template <typename T>
+struct StageLine {
+    const cpx<T>* k = nullptr;           // stage 0: the field spectrum (copied to init)
+    cpx<T>* init = nullptr;
+    cpx<T>* final = nullptr;
+    cpx<T>* kn = nullptr;                // the corrected spectrum
+    const T* stab = nullptr;             // the stage factor 1 / (1 - gamma dt D(k))
+    int nterm = 0;
+    const cpx<T>* brick[kStageTerms] = {};  // null: a constant term
+    double coef[kStageTerms] = {};
+    int pw[kStageTerms] = {};
+    const double* dmul[kStageTerms] = {};
+};
+template <typename T>
+struct PencilStageArgs {
+    PencilArgs<T> p;                     // the inverse pencil pass; arr[q] = its outputs, narr fields
+    StageLine<T> f[kMaxArr];
+    int stage = 0;
+    double dt = 0.0, fp = 0.0, fc = 0.0;
+    int need_l = 0;                      // some term has a LAP power
+    int store_kn = 1;
+    KGrid g;
+    long nmain = 0, nk = 0;
+};
+bool stage_pencil_length(int n);  // lengths the fused stage pencils are compiled for
+template <typename T> hipError_t launch_pencils_stage(const PencilStageArgs<T>& a, int n, hipStream_t s);
+
 // launchers (implemented in spectral.hip); n is the line length.
 // Grouped launches of a batch of size classes: class c owns blocks [first[c], first[c] +
 // blocks[c]) of the launch (first[c] a multiple of 8, first[ncls] the launch's block count);

@@ -61,11 +61,6 @@ __device__ __forceinline__ void line_fft(cpx<T>* v, cpx<T>* lds, int es, int n, 
     }
 }
 
-__device__ __forceinline__ double wrapf(int k, int n) {
-    double f = (double)k / (double)n;
-    return (f > 0.5) ? f - 1.0 : f;
-}
-
 // 1/x in FP64: correctly rounded division for the plain FP64 path; hardware
 // reciprocal refined by two Newton steps (relative error ~1e-16) for FP32-bulk
 template <typename T>
