@@ -1,0 +1,49 @@
+// blockred.h -- block-wide FP64 sums and compensated accumulation, shared by the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace gpf {
+
+// Block-wide FP64 sum, returned to every thread; red must hold blockDim.x doubles.  Blocks of
+// whole wavefronts sum each wavefront with cross-lane exchanges and then one value per
+// wavefront through LDS (two barriers); others use a tree in LDS.  Every thread adds the
+// same values in the same order, so the result is deterministic.
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    if ((nt & 63) == 0) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((tid & 63) == 0) red[tid >> 6] = v;
+        __syncthreads();
+        double r = 0.0;
+        for (int w = 0; w < (nt >> 6); ++w) r += red[w];
+        __syncthreads();
+        return r;
+    }
+    red[tid] = v;
+    __syncthreads();
+    int p2 = 1;
+    while (p2 < nt) p2 <<= 1;
+    for (int s = p2 / 2; s > 0; s >>= 1) {
+        if (tid < s && tid + s < nt) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// Neumaier compensated accumulation
+struct KSum {
+    double s = 0.0, c = 0.0;
+    __device__ void add(double v) {
+        double t = s + v;
+        c += (fabs(s) >= fabs(v)) ? ((s - t) + v) : ((v - t) + s);
+        s = t;
+    }
+    __device__ double get() const { return s + c; }
+};
+
+}  // namespace gpf

@@ -11,6 +11,7 @@
 //                 displacement solve (linearElasticity.go:33-63), strains
 //                 (linearElasticity.go:67-83) and the sigma*:eps sum (homoLinElast.go:72-89)
 //                 collapsed by linearity into one multiplier of FFT(H)
+#include "blockred.h"
 #include "fft_core.h"
 #include "pointwise.h"
 #include "kspace.h"
@@ -659,28 +660,7 @@ __global__ void __launch_bounds__(256) k_copy4(const float4* __restrict__ src, f
     for (; i < n4; i += stride) dst[i] = src[i];
 }
 
-__device__ double block_sum(double v, double* red) {
-    const int tid = threadIdx.x;
-    red[tid] = v;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (tid < s) red[tid] += red[tid + s];
-        __syncthreads();
-    }
-    const double r = red[0];
-    __syncthreads();
-    return r;
-}
 
-struct KSum {
-    double s = 0.0, c = 0.0;
-    __device__ void add(double v) {
-        double t = s + v;
-        c += (fabs(s) >= fabs(v)) ? ((s - t) + v) : ((v - t) + s);
-        s = t;
-    }
-    __device__ double get() const { return s + c; }
-};
 
 constexpr int kRedThreads = 256;
 

@@ -29,6 +29,7 @@
 
 #include <type_traits>
 
+#include "blockred.h"
 #include "fft_core.h"
 #include "spectral.h"
 
@@ -97,33 +98,7 @@ __device__ __forceinline__ double vandeven(const double* __restrict__ tab, doubl
     return tab[idx] + (x - x0) * dy / dx;
 }
 
-// block-wide FP64 sum; red must hold blockDim.x doubles; result valid in thread 0
-__device__ double block_sum(double v, double* red) {
-    const int tid = threadIdx.x;
-    const int nt = blockDim.x;
-    red[tid] = v;
-    __syncthreads();
-    int p2 = 1;
-    while (p2 < nt) p2 <<= 1;
-    for (int s = p2 / 2; s > 0; s >>= 1) {
-        if (tid < s && tid + s < nt) red[tid] += red[tid + s];
-        __syncthreads();
-    }
-    double r = red[0];
-    __syncthreads();
-    return r;
-}
 
-// Neumaier compensated accumulation
-struct KSum {
-    double s = 0.0, c = 0.0;
-    __device__ void add(double v) {
-        double t = s + v;
-        c += (fabs(s) >= fabs(v)) ? ((s - t) + v) : ((v - t) + s);
-        s = t;
-    }
-    __device__ double get() const { return s + c; }
-};
 
 // ---------------------------------------------------------------------------
 // tile decode shared by the pencil kernels
