@@ -46,4 +46,34 @@ struct KSum {
     __device__ double get() const { return s + c; }
 };
 
+// Sum of n partial sums by the block's first wavefront (blockDim.x >= 64): lane k adds the
+// partials k, k + 64, ... with compensation, then the lanes are combined by cross-lane
+// exchanges.  The order depends only on n, not on the block size, so every kernel that
+// forms a sum of the same partials gets the same value.  Valid in lanes 0 .. 63.
+__device__ __forceinline__ double partials_sum(const double* p, long n) {
+    KSum k;
+    if (threadIdx.x < 64)
+        for (long i = threadIdx.x; i < n; i += 64) k.add(p[i]);
+    double v = k.get();
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// One slot of the energy series at the slot a device counter names (replayable from a
+// graph): series[2 i] = sum bulk + ceg sum eg, series[2 i + 1] = sum mass, i = *counter,
+// then *counter = i + 1.  Called by a whole block of at least 64 threads.
+__device__ __forceinline__ void series_slot(const double* bulk, const double* mass, long nb, const double* eg, long ne,
+                                            double ceg, double* series, long* counter) {
+    const double b = partials_sum(bulk, nb);
+    const double m = partials_sum(mass, nb);
+    const double e = partials_sum(eg, ne);
+    if (threadIdx.x == 0) {
+        const long i = *counter;
+        series[2 * i] = b + ceg * e;
+        series[2 * i + 1] = m;
+        *counter = i + 1;
+    }
+}
+
 }  // namespace gpf

@@ -505,7 +505,8 @@ private:
     // fused path
     void fused_step();
     void prepare_fused(long upcoming);  // allocations ahead of launches (never during graph capture)
-    void enqueue_fused();               // the launches of one step (capturable)
+    // the launches of step pos of count steps enqueued back to back (capturable)
+    void enqueue_fused(int pos = 0, int count = 1);
     void capture_graph();               // instantiates graph_: kGraphSteps steps of enqueue_fused
     void drop_graph();
     hipGraphExec_t graph_ = nullptr;
@@ -1468,8 +1469,9 @@ void Engine<T>::prepare_fused(long upcoming) {
     PencilUpdateArgs<T> u = update_args(S_.get(), W_.get(), mean_.get(), tw_pu_);
     int tiles0 = 0;
     const long nblk_c = pencil_blocks(u.p.seg, u.p.nseg, u.p.TW, &tiles0);
-    if (nblk_a_ != nblk_a) { pa_bulk_.alloc(nblk_a); pa_mass_.alloc(nblk_a); nblk_a_ = nblk_a; drop_graph(); }
-    if (nblk_c_ != nblk_c) { pc_eg_.alloc(nblk_c); nblk_c_ = nblk_c; drop_graph(); }
+    // two sets of partial sums: consecutive steps alternate (enqueue_fused)
+    if (nblk_a_ != nblk_a) { pa_bulk_.alloc(2 * nblk_a); pa_mass_.alloc(2 * nblk_a); nblk_a_ = nblk_a; drop_graph(); }
+    if (nblk_c_ != nblk_c) { pc_eg_.alloc(2 * nblk_c); nblk_c_ = nblk_c; drop_graph(); }
     if (series_n_ + upcoming > series_cap_) {
         const long cap = std::max({1024L, 2 * series_cap_, series_n_ + upcoming});
         Buf<double> nb(2 * cap);
@@ -1482,17 +1484,38 @@ void Engine<T>::prepare_fused(long upcoming) {
     }
 }
 
+// With the monitor, step pos writes its partial sums to set pos mod 2, and its energy-series
+// slot is written by block 0 of the next step's row launch (the row blocks having at least
+// 64 threads); the last step of the sequence writes its own slot with a launch of its own.
 template <typename T>
-void Engine<T>::enqueue_fused() {
+void Engine<T>::enqueue_fused(int pos, int count) {
     RowFusedArgs<T> ra = rows_args(W_.get(), mean_.get(), tw_rows_, rows_var_);
     const int nu = axis_len(upd_axis());
     PencilUpdateArgs<T> u = update_args(S_.get(), W_.get(), mean_.get(), tw_pu_);
+    const double ceg = 0.5 * mon_kappa_ / (double)g_.n;
+    const long cur = pos & 1, prev = (pos + 1) & 1;
+    const bool rows_pass = fp_.has_src || mon_;
+    const bool fold = mon_ && rows_pass && pos > 0 && ra.TW * fft_threads_per_line(g_.nx, ra.variant) >= 64;
     if (mon_) {
         ra.nbulk = (int)mon_poly_.size();
         for (int q = 0; q < ra.nbulk; ++q) ra.bulk[q] = mon_poly_[q];
-        ra.bulk_partial = pa_bulk_.get();
-        ra.mass_partial = pa_mass_.get();
-        u.egrad_partial = pc_eg_.get();
+        ra.bulk_partial = pa_bulk_.get() + cur * nblk_a_;
+        ra.mass_partial = pa_mass_.get() + cur * nblk_a_;
+        u.egrad_partial = pc_eg_.get() + cur * nblk_c_;
+    }
+    if (fold) {
+        ra.prev_bulk = pa_bulk_.get() + prev * nblk_a_;
+        ra.prev_mass = pa_mass_.get() + prev * nblk_a_;
+        ra.prev_eg = pc_eg_.get() + prev * nblk_c_;
+        ra.prev_nb = nblk_a_;
+        ra.prev_ne = nblk_c_;
+        ra.prev_ceg = ceg;
+        ra.series = series_.get();
+        ra.counter = series_count_.get();
+    } else if (mon_ && pos > 0) {
+        // the previous step's slot was not folded into this row launch
+        HIPCHK(launch_series_append(pa_bulk_.get() + prev * nblk_a_, nblk_a_, pc_eg_.get() + prev * nblk_c_, nblk_c_,
+                                    ceg, pa_mass_.get() + prev * nblk_a_, series_.get(), series_count_.get(), st_));
     }
     // W holds S inverse-transformed along the update axis; the real-space pass is needed
     // for the explicit source and for the monitor's bulk sums
@@ -1514,9 +1537,9 @@ void Engine<T>::enqueue_fused() {
     tbegin(2);
     HIPCHK(launch_pencil_update<T>(u, nu, st_));
     tend(2);
-    if (mon_)
-        HIPCHK(launch_series_append(pa_bulk_.get(), nblk_a_, pc_eg_.get(), nblk_c_, 0.5 * mon_kappa_ / (double)g_.n,
-                                    pa_mass_.get(), series_.get(), series_count_.get(), st_));
+    if (mon_ && pos == count - 1)
+        HIPCHK(launch_series_append(pa_bulk_.get() + cur * nblk_a_, nblk_a_, pc_eg_.get() + cur * nblk_c_, nblk_c_,
+                                    ceg, pa_mass_.get() + cur * nblk_a_, series_.get(), series_count_.get(), st_));
 }
 
 template <typename T>
@@ -1537,7 +1560,7 @@ void Engine<T>::capture_graph() {
     graph_mon_ = mon_;
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < kGraphSteps; ++k) enqueue_fused();
+    for (int k = 0; k < kGraphSteps; ++k) enqueue_fused(k, kGraphSteps);
     HIPCHK(hipStreamEndCapture(st_, &g));
     const hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);

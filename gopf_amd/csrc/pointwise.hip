@@ -1218,22 +1218,7 @@ hipError_t launch_accumulate(const double* partial, int nscal, long nblocks, dou
 __global__ void __launch_bounds__(kRedThreads) k_series_append(const double* bulk, long nbulk, const double* eg,
                                                                  long neg, double ceg, const double* mass,
                                                                  double* series, long* counter) {
-    __shared__ double red[kRedThreads];
-    KSum sb, se, sm;
-    for (long i = threadIdx.x; i < nbulk; i += blockDim.x) {
-        sb.add(bulk[i]);
-        sm.add(mass[i]);
-    }
-    for (long i = threadIdx.x; i < neg; i += blockDim.x) se.add(eg[i]);
-    const double b = block_sum(sb.get(), red);
-    const double e = block_sum(se.get(), red);
-    const double m = block_sum(sm.get(), red);
-    if (threadIdx.x == 0) {
-        const long i = *counter;
-        series[2 * i] = b + ceg * e;
-        series[2 * i + 1] = m;
-        *counter = i + 1;
-    }
+    series_slot(bulk, mass, nbulk, eg, neg, ceg, series, counter);
 }
 
 hipError_t launch_series_append(const double* bulk, long nbulk, const double* eg, long neg, double ceg,

@@ -237,6 +237,16 @@ struct RowFusedArgs {
     double bulk[kMaxPoly];
     double* bulk_partial = nullptr;      // per-block FP64 sum of f(c)
     double* mass_partial = nullptr;      // per-block FP64 sum of c
+    // optional (monitor, blocks of >= 64 threads): block 0 also writes the previous step's
+    // energy-series slot from that step's partial sums (series_slot), so that a sequence of
+    // steps needs no launch of its own per slot; the previous partials are other buffers
+    const double* prev_bulk = nullptr;
+    const double* prev_mass = nullptr;
+    const double* prev_eg = nullptr;
+    long prev_nb = 0, prev_ne = 0;
+    double prev_ceg = 0.0;
+    double* series = nullptr;
+    long* counter = nullptr;
     // batched sims: rows_per_sim > 0 (even) selects gpoly_sim[sim][kMaxPoly] and mean[sim], sim = row / rows_per_sim
     long rows_per_sim = 0;
     const double* gpoly_sim = nullptr;

@@ -1186,6 +1186,8 @@ __device__ __forceinline__ void rows_fused_body(RowFusedArgs<T> a, int blk, unsi
             a.bulk_partial[blk] = b;
             a.mass_partial[blk] = ms;
         }
+        if (a.series && blk == 0)
+            series_slot(a.prev_bulk, a.prev_mass, a.prev_nb, a.prev_eg, a.prev_ne, a.prev_ceg, a.series, a.counter);
     }
     line_fft<T, N, -1, V>(v, lds + lm.off, 1, n, lm.t, twl);
     if constexpr (row_staged<N, E>())
