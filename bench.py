@@ -215,17 +215,21 @@ def run_c1(G, W, args):
     m.AddEquation(W.CH_EQUATION)
     m.Init()
     s = G.Solver(m, dims, dt, precision="fp64", autotune=True)
+    s.SetPersistent({"auto": -1, "on": 1, "off": 0}[args.persistent])
     s.StepperPropagate(args.warmup)   # tunes the launch configuration
     s.SetField(0, c0)
     e0, m0 = s.Energy(0, W.CH_BULK_POLY, 1.0)
     s.MonitorEnergy(W.CH_BULK_POLY, 1.0)
+    p0 = s.PersistentSteps()
     el = time_steps(s, steps)
+    persistent = s.PersistentSteps() - p0 == steps
     e, mass = s.EnergySeries()
     series = np.concatenate([[e0], e])
     out = {"metric": "steps/s, 128^2 CH FP64 with the per-step free-energy series", "value": steps / el,
            "unit": "steps/s", "ms_per_step": 1e3 * el / steps, "voxel_steps_per_s": N * N * steps / el,
            "dtype": "f64", "config": {"workload": "cahn_hilliard_2d_128", "grid": dims, "dt": dt, "steps": steps,
-                                      "init": "0.5 + U(-0.05, 0.05), SplitMix64 seed 0"},
+                                      "init": "0.5 + U(-0.05, 0.05), SplitMix64 seed 0",
+                                      "launch": "persistent" if persistent else "per-pass graphs"},
            "energy": {"values": int(series.size), "first": float(series[0]), "last": float(series[-1]),
                       "max_increase": float(np.max(np.diff(series))), "mass_drift": float(abs(mass[-1] - m0) / m0)}}
     if not args.no_cpu:
@@ -572,6 +576,8 @@ def main() -> int:
                     help="run the autotuner instead of applying the stored launch layout")
     ap.add_argument("--save-tuning", default=None,
                     help="write the layouts this run used to this JSON file (stored-layout format)")
+    ap.add_argument("--persistent", default="auto", choices=["auto", "on", "off"],
+                    help="c1: the persistent step (every step in one cooperative launch) or the per-pass launches")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "ie", "io"],
                     help="c2 (default): 256^3 CH headline; c1 128^2 CH FP64 with energy series; c3 batched "
                          "sweep; c4 192^3 elastic RK4; c5 512^2 adjoint; ie 128^2 CH ImplicitEuler; io snapshots")

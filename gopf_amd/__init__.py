@@ -118,6 +118,8 @@ def _declare(L) -> None:
         "gpf_solver_enable_timing": (C.c_int, [_vp, C.c_int]),
         "gpf_solver_pass_times": (C.c_int, [_vp, C.c_int, _dp, _lp, C.c_char_p, C.c_int]),
         "gpf_solver_stream": (_vp, [_vp]),
+        "gpf_solver_set_persistent": (C.c_int, [_vp, C.c_int, C.c_int]),
+        "gpf_solver_persistent_steps": (C.c_int, [_vp, _lp]),
         "gpf_solver_tuning": (C.c_int, [_vp, _ip, C.c_int]),
         "gpf_solver_set_tuning": (C.c_int, [_vp, _ip, C.c_int]),
         "gpf_solver_set_filter": (C.c_int, [_vp, _dp, C.c_long]),
@@ -583,6 +585,16 @@ class Solver:
 
     def Stream(self) -> int:
         return int(lib().gpf_solver_stream(self._h) or 0)
+
+    def SetPersistent(self, mode: int = -1, threads: int = 0) -> None:
+        """Persistent step of square 2D grids (side 32-256): -1 auto (sides up to 128), 0 off, 1 on;
+        threads per block (0: 64)."""
+        _check(lib().gpf_solver_set_persistent(self._h, int(mode), int(threads)))
+
+    def PersistentSteps(self) -> int:
+        n = C.c_long(0)
+        _check(lib().gpf_solver_persistent_steps(self._h, C.byref(n)))
+        return int(n.value)
 
     def SetFilter(self, filt) -> None:
         """TimeStepper.SetFilter (pf/euler.go:62-64) with a Vandeven filter (its table), or None."""

@@ -21,7 +21,7 @@ SOURCES = ["spectral_ext_rows.hip", "spectral_ext_rows64.hip", "spectral_ext_fwd
            "spectral_sync.hip", "spectral_r2c.hip", "spectral_c2r.hip", "spectral_update.hip", "spectral_rows.hip",
            "spectral_pencils_fwd64.hip", "spectral_pencils_inv64.hip", "spectral_sync64.hip", "spectral_r2c64.hip",
            "spectral_c2r64.hip", "spectral_update64.hip", "spectral_rows64.hip", "spectral_group.hip",
-           "spectral_group64.hip", "spectral_stage.hip", "spectral_pencils.hip", "spectral_pencils64.hip", "spectral.hip", "engine.hip",
+           "spectral_group64.hip", "spectral_stage.hip", "spectral_persist.hip", "spectral_persist64.hip", "spectral_pencils.hip", "spectral_pencils64.hip", "spectral.hip", "engine.hip",
            "pointwise.hip", "batch.hip", "krylov.hip", "io.hip", "reduce.hip", "model.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -50,6 +50,31 @@ struct KSum {
 // partials k, k + 64, ... with compensation, then the lanes are combined by cross-lane
 // exchanges.  The order depends only on n, not on the block size, so every kernel that
 // forms a sum of the same partials gets the same value.  Valid in lanes 0 .. 63.
+// partials_load / partials_reduce split it for n <= kPartialsFast: every lane's loads are
+// issued first (one round trip to memory for several sums), then added in the same order.
+constexpr int kPartialsFast = 256;
+struct Partials {
+    double v[kPartialsFast / 64];
+};
+__device__ __forceinline__ Partials partials_load(const double* p, long n) {
+    Partials r;
+#pragma unroll
+    for (int j = 0; j < kPartialsFast / 64; ++j) {
+        const long i = threadIdx.x + 64L * j;
+        r.v[j] = (threadIdx.x < 64 && i < n) ? p[i] : 0.0;
+    }
+    return r;
+}
+__device__ __forceinline__ double partials_reduce(const Partials& r, long n) {
+    KSum k;
+#pragma unroll
+    for (int j = 0; j < kPartialsFast / 64; ++j)
+        if (threadIdx.x < 64 && threadIdx.x + 64L * j < n) k.add(r.v[j]);
+    double v = k.get();
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
 __device__ __forceinline__ double partials_sum(const double* p, long n) {
     KSum k;
     if (threadIdx.x < 64)
@@ -62,14 +87,27 @@ __device__ __forceinline__ double partials_sum(const double* p, long n) {
 
 // One slot of the energy series at the slot a device counter names (replayable from a
 // graph): series[2 i] = sum bulk + ceg sum eg, series[2 i + 1] = sum mass, i = *counter,
-// then *counter = i + 1.  Called by a whole block of at least 64 threads.
+// then *counter = i + 1.  Called by a whole block of at least 64 threads.  With at most
+// kPartialsFast partials per sum, the three sums' loads and the counter's are issued together.
 __device__ __forceinline__ void series_slot(const double* bulk, const double* mass, long nb, const double* eg, long ne,
                                             double ceg, double* series, long* counter) {
-    const double b = partials_sum(bulk, nb);
-    const double m = partials_sum(mass, nb);
-    const double e = partials_sum(eg, ne);
+    double b, m, e;
+    long i = 0;
+    if (nb <= kPartialsFast && ne <= kPartialsFast) {
+        const Partials pb = partials_load(bulk, nb);
+        const Partials pm = partials_load(mass, nb);
+        const Partials pe = partials_load(eg, ne);
+        if (threadIdx.x == 0) i = *counter;
+        b = partials_reduce(pb, nb);
+        m = partials_reduce(pm, nb);
+        e = partials_reduce(pe, ne);
+    } else {
+        b = partials_sum(bulk, nb);
+        m = partials_sum(mass, nb);
+        e = partials_sum(eg, ne);
+        if (threadIdx.x == 0) i = *counter;
+    }
     if (threadIdx.x == 0) {
-        const long i = *counter;
         series[2 * i] = b + ceg * e;
         series[2 * i + 1] = m;
         *counter = i + 1;

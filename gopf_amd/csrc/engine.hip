@@ -444,6 +444,12 @@ public:
     virtual double sdd_dimer_length(double t) const = 0;
     virtual void flush_io() = 0;
     virtual void save_csv(const char* prefix, int epoch) = 0;
+    // the persistent 2D step (k_fused_persist): mode -1 where it is measured faster, 0 never,
+    // 1 wherever admissible; threads per block 0 (64) or 64, 128, 256, 512
+    virtual void set_persistent(int mode, int threads) = 0;
+    virtual long persistent_steps() const = 0;
+    // errors a finished asynchronous launch left behind (called after a stream synchronisation)
+    virtual void after_sync() = 0;
 };
 
 template <typename T>
@@ -483,6 +489,9 @@ public:
     double sdd_dimer_length(double t) const override;
     void flush_io() override;
     void save_csv(const char* prefix, int epoch) override;
+    void set_persistent(int mode, int threads) override;
+    long persistent_steps() const override { return persist_steps_; }
+    void after_sync() override { check_persist(); }
 
 private:
     // transforms
@@ -512,6 +521,27 @@ private:
     hipGraphExec_t graph_ = nullptr;
     bool graph_mon_ = false;  // the monitor setting the graph was captured with
     static constexpr int kGraphSteps = 10;
+    // largest side taking the persistent step by default: faster than the replayed per-pass
+    // graphs at 32^2-128^2 (FP64 128^2 with the energy series 16.1 against 17.6 us per step),
+    // slower at 256^2 (20.7 against 18.8 us)
+    static constexpr int kPersistAutoMax = 128;
+    // persistent 2D step: a run of steps in one cooperative launch (spectral_persist.h); its
+    // plan (the passes' arguments) is rebuilt whenever the graph is dropped
+    bool persist_eligible();
+    void build_persist();
+    long run_persist(long n);  // returns the steps left to the per-pass launches
+    void check_persist();
+    int persist_mode_ = -1, persist_threads_ = 0;
+    int coop_ = -1;  // the device's cooperative-launch attribute (-1 not yet read)
+    int local_ = -1;  // participants on one XCD: -1 not yet seen, 1 seen, 0 not the device's placement
+    long persist_steps_ = 0;
+    bool persist_ok_ = false;
+    long pnr_ = 0, pnu_ = 0;  // block counts of the plan's row and update passes
+    Buf<unsigned> pbar_;
+    Buf<double> ppa_bulk_, ppa_mass_, ppc_eg_;
+    int* pfail_ = nullptr;      // mapped host flag set by a failed device-wide barrier
+    int* pfail_dev_ = nullptr;
+    PersistLaunch<T> pl_;
     RowFusedArgs<T> rows_args(cpx<T>* W, double* mean, int tw, int var);
     PencilUpdateArgs<T> update_args(cpx<T>* S, cpx<T>* W, double* mean, int tw);
     void prime();
@@ -844,6 +874,7 @@ Engine<T>::~Engine() {
         }
         (void)hipStreamDestroy(st_);
     }
+    if (pfail_) (void)hipHostFree(pfail_);
 }
 
 template <typename T>
@@ -1057,6 +1088,7 @@ void Engine<T>::get_field(int idx, double* out) {
         HIPCHK(hipMemcpyAsync(h.data(), F_[idx].get(), g_.n * sizeof(T), hipMemcpyDeviceToHost, st_));
     }
     HIPCHK(hipStreamSynchronize(st_));
+    check_persist();
     const double mean = host_mean();
     for (long j = 0; j < g_.n; ++j) {
         out[2 * j] = (double)h[j] + mean;
@@ -1209,6 +1241,7 @@ void Engine<T>::monitor(int npoly, const double* poly, double kappa) {
 template <typename T>
 long Engine<T>::series(double* e, double* m, long cap) {
     HIPCHK(hipStreamSynchronize(st_));
+    check_persist();
     const long n = std::min(cap, series_n_);
     if (n > 0) {
         std::vector<double> h(2 * series_n_);
@@ -1460,6 +1493,7 @@ template <typename T>
 void Engine<T>::drop_graph() {
     if (graph_) (void)hipGraphExecDestroy(graph_);
     graph_ = nullptr;
+    persist_ok_ = false;
 }
 
 template <typename T>
@@ -1611,7 +1645,11 @@ double Engine<T>::prepare(long upcoming) {
         }
         if (!warmed_) warm(30.0);
         prepare_fused(std::max(upcoming, 0L));
-        if (!timing_ && !c_out_) capture_graph();
+        if (persist_eligible()) {
+            if (!persist_ok_) build_persist();
+        } else if (!timing_ && !c_out_) {
+            capture_graph();
+        }
         HIPCHK(hipStreamSynchronize(st_));
     }
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1629,6 +1667,11 @@ void Engine<T>::steps(long n) {
         fused_step();
         --n;
     }
+    if (n >= 2 && persist_eligible()) {
+        prepare_fused(n);
+        n = run_persist(n);
+        if (n == 0) return;
+    }
     if (timing_ || c_out_ || n < kGraphSteps) {
         for (long i = 0; i < n; ++i) fused_step();
         return;
@@ -1641,6 +1684,160 @@ void Engine<T>::steps(long n) {
         if (mon_) series_n_ += kGraphSteps;
     }
     for (; n > 0; --n) fused_step();
+}
+
+// The persistent step runs on square 2D grids of side 32 to 256 (the compiled lengths), with
+// the device's cooperative launch, outside per-pass timing and real-space output.  In the
+// default mode it is taken where it was measured faster than the replayed per-pass graphs.
+template <typename T>
+bool Engine<T>::persist_eligible() {
+    if (!fp_.ok || persist_mode_ == 0 || timing_ || c_out_ || g_.ndim != 2 || g_.nx != g_.ny || !persist_side(g_.nx))
+        return false;
+    // by default only with the participants on one XCD (the agent-fenced barrier is slower
+    // than the per-pass launches)
+    if (persist_mode_ < 0 && (g_.nx > kPersistAutoMax || local_ == 0)) return false;
+    if (coop_ < 0) {
+        int dev = 0, v = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeCooperativeLaunch, dev));
+        coop_ = v ? 1 : 0;
+    }
+    return coop_ == 1;
+}
+
+template <typename T>
+void Engine<T>::set_persistent(int mode, int threads) {
+    if (mode < -1 || mode > 1) throw Error(GPF_EINVAL, "persistent mode must be -1 (auto), 0 (off) or 1 (on)");
+    if (threads != 0 && threads != 64 && threads != 128 && threads != 256 && threads != 512)
+        throw Error(GPF_EINVAL, "threads per block of the persistent step: 0 (64), 64, 128, 256 or 512");
+    if (threads > 0 && g_.ndim == 2 && persist_side(g_.nx) && g_.nx == g_.ny) {
+        const int t1 = fft_threads_per_line(g_.nx, 1);
+        if (threads < t1 || !tw_fits(g_.nx, threads / t1, sizeof(cpx<T>), 1))
+            throw Error(GPF_EINVAL, "threads per block not admissible for this grid's persistent step");
+    }
+    HIPCHK(hipStreamSynchronize(st_));
+    persist_mode_ = mode;
+    persist_threads_ = threads;
+    persist_ok_ = false;
+}
+
+// the plan: the row pass in schedule 1 and the update pass with the same threads per block,
+// their partial sums in two sets (consecutive steps alternate)
+template <typename T>
+void Engine<T>::build_persist() {
+    const int n = g_.nx;
+    const int t1 = fft_threads_per_line(n, 1);
+    const int thr = persist_threads_ > 0 ? persist_threads_ : std::max(64, t1);
+    const int tw = thr / t1;
+    if (tw < 1 || tw * t1 != thr || !tw_fits(n, tw, sizeof(cpx<T>), 1))
+        throw Error(GPF_EINVAL, "persistent step: no admissible block for this grid");
+    PersistPlan<T> P;
+    RowFusedArgs<T> ra = rows_args(W_.get(), mean_.get(), tw, 1);
+    PencilUpdateArgs<T> u = update_args(S_.get(), W_.get(), mean_.get(), tw);
+    u.p.n = n;
+    u.p.lpt = 1;
+    int tiles0 = 0;
+    const long nr = rows_fused_blocks(g_.rows, tw);
+    const long nu = pencil_blocks(u.p.seg, u.p.nseg, tw, &tiles0);
+    u.p.tiles0 = tiles0;
+    P.rows_pass = (fp_.has_src || mon_) ? 1 : 0;
+    P.nblk_rows = (int)nr;
+    P.nblk_upd = (int)nu;
+    for (int k = 0; k < 3; ++k) P.ra[k] = ra;
+    P.u[0] = P.u[1] = u;
+    if (mon_) {
+        if ((long)ppa_bulk_.size() != 2 * nr) { ppa_bulk_.alloc(2 * nr); ppa_mass_.alloc(2 * nr); }
+        if ((long)ppc_eg_.size() != 2 * nu) ppc_eg_.alloc(2 * nu);
+        const double ceg = 0.5 * mon_kappa_ / (double)g_.n;
+        for (int k = 0; k < 3; ++k) {
+            RowFusedArgs<T>& a = P.ra[k];
+            const long cur = (k == 1) ? 1 : 0, prev = 1 - cur;
+            a.nbulk = (int)mon_poly_.size();
+            for (int q = 0; q < a.nbulk; ++q) a.bulk[q] = mon_poly_[q];
+            a.bulk_partial = ppa_bulk_.get() + cur * nr;
+            a.mass_partial = ppa_mass_.get() + cur * nr;
+            if (k > 0) {
+                a.prev_bulk = ppa_bulk_.get() + prev * nr;
+                a.prev_mass = ppa_mass_.get() + prev * nr;
+                a.prev_eg = ppc_eg_.get() + prev * nu;
+                a.prev_nb = nr;
+                a.prev_ne = nu;
+                a.prev_ceg = ceg;
+                a.series = series_.get();
+                a.counter = series_count_.get();
+            }
+        }
+        P.u[0].egrad_partial = ppc_eg_.get();
+        P.u[1].egrad_partial = ppc_eg_.get() + nu;
+    }
+    if (!pbar_.get()) pbar_.alloc(4);
+    if (!pfail_) {
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&pfail_), sizeof(int), hipHostMallocMapped));
+        *pfail_ = 0;
+        HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&pfail_dev_), pfail_, 0));
+    }
+    pl_ = PersistLaunch<T>{};
+    pl_.plan = P;
+    pl_.n = n;
+    pl_.mon = mon_ ? 1 : 0;
+    persist_geometry<T>(pl_, tw);
+    pl_.blocks = (int)(P.rows_pass ? std::max(nr, nu) : nu);
+    pl_.bar = pbar_.get();
+    pl_.spin_limit = 1u << 20;
+    pl_.local = local_ != 0 ? 1 : 0;
+    pl_.host_fail = pfail_dev_;
+    pnr_ = nr;
+    pnu_ = nu;
+    persist_ok_ = true;
+}
+
+// n steps in launches of at most 2^20 steps; with the monitor, the slot of a launch's last
+// step is written by a launch of its own after it.  The first launch with the participants on
+// one XCD is waited for: when the device placed them otherwise (no step run), the launch
+// layout falls back to the agent-fenced barrier (mode 1) or to the per-pass launches.
+template <typename T>
+long Engine<T>::run_persist(long n) {
+    check_persist();
+    if (!persist_ok_) build_persist();
+    const double ceg = 0.5 * mon_kappa_ / (double)g_.n;
+    while (n > 0) {
+        const int k = (int)std::min(n, 1L << 20);
+        HIPCHK(hipMemsetAsync(pbar_.get(), 0, 4 * sizeof(unsigned), st_));
+        pl_.steps = k;
+        HIPCHK(launch_fused_persist<T>(pl_, st_));
+        if (pl_.local && local_ < 0) {
+            HIPCHK(hipStreamSynchronize(st_));
+            if (*reinterpret_cast<volatile int*>(pfail_) == 2) {
+                *pfail_ = 0;
+                local_ = 0;
+                pl_.local = 0;
+                if (persist_mode_ < 0) return n;
+                continue;
+            }
+            local_ = 1;
+        }
+        if (mon_) {
+            const long cur = (k - 1) & 1;
+            HIPCHK(launch_series_append(ppa_bulk_.get() + cur * pnr_, pnr_, ppc_eg_.get() + cur * pnu_, pnu_, ceg,
+                                        ppa_mass_.get() + cur * pnr_, series_.get(), series_count_.get(), st_));
+            series_n_ += k;
+        }
+        persist_steps_ += k;
+        n -= k;
+    }
+    return 0;
+}
+
+template <typename T>
+void Engine<T>::check_persist() {
+    const int f = pfail_ ? *reinterpret_cast<volatile int*>(pfail_) : 0;
+    if (f == 0) return;
+    *pfail_ = 0;
+    if (f == 2)
+        throw Error(GPF_EDEVICE, "persistent step: the device placed the workgroups on more than one XCD; the steps "
+                                 "of that launch were not run");
+    throw Error(GPF_EDEVICE, "persistent step: a device-wide barrier wait exceeded its limit; the fields of that run "
+                             "are not valid");
 }
 
 // ---------------------------------------------------------------------------
@@ -3389,6 +3586,22 @@ int gpf_solver_synchronize(gpf_solver* s) {
         if (!s) throw Error(GPF_EINVAL, "bad argument");
         set_device(s->device);
         HIPCHK(hipStreamSynchronize(s->eng->stream()));
+        s->eng->after_sync();
+    });
+}
+
+int gpf_solver_set_persistent(gpf_solver* s, int mode, int threads) {
+    return guarded([&] {
+        if (!s) throw Error(GPF_EINVAL, "bad argument");
+        set_device(s->device);
+        s->eng->set_persistent(mode, threads);
+    });
+}
+
+int gpf_solver_persistent_steps(gpf_solver* s, long* n) {
+    return guarded([&] {
+        if (!s || !n) throw Error(GPF_EINVAL, "bad argument");
+        *n = s->eng->persistent_steps();
     });
 }
 

@@ -284,6 +284,35 @@ struct PencilStageArgs {
 bool stage_pencil_length(int n);  // lengths the fused stage pencils are compiled for
 template <typename T> hipError_t launch_pencils_stage(const PencilStageArgs<T>& a, int n, hipStream_t s);
 
+// A run of fused 2D steps in one cooperative launch (k_fused_persist): the row pass of a
+// launch's first step, of odd steps and of even steps after the first (partial-sum sets and
+// series slots alternate), the update pass of even and odd steps, and the passes' block counts
+// (the row pass, schedule 1, and the update pass with the same threads per block).
+template <typename T>
+struct PersistPlan {
+    RowFusedArgs<T> ra[3];
+    PencilUpdateArgs<T> u[2];
+    int nblk_rows = 0, nblk_upd = 0;
+    int rows_pass = 0;
+};
+template <typename T>
+struct PersistLaunch {
+    PersistPlan<T> plan;                   // passed by value (2.4 KB of the 4 KB of kernel arguments)
+    int n = 0;                             // the square grid's side: 32, 64, 128 or 256
+    int mon = 0;                           // the energy monitor's partial sums
+    int steps = 0;
+    int local = 0;                         // participants on one XCD: the grid is 8 blocks per participant
+    int blocks = 0, threads = 0;           // participating blocks
+    size_t smem = 0;
+    unsigned* bar = nullptr;               // 4 words, zero at the launch's start
+    unsigned spin_limit = 0;               // polls of a barrier's generation before it is declared failed
+    int* host_fail = nullptr;              // mapped host flag: 1 a failed barrier, 2 (local) participants
+                                           // on more than one XCD, the steps not run
+};
+bool persist_side(int n);  // the square sides with a persistent step
+template <typename T> void persist_geometry(PersistLaunch<T>& L, int tw);  // threads and smem of L.n, L.mon
+template <typename T> hipError_t launch_fused_persist(const PersistLaunch<T>& L, hipStream_t s);
+
 // launchers (implemented in spectral.hip); n is the line length.
 // Grouped launches of a batch of size classes: class c owns blocks [first[c], first[c] +
 // blocks[c]) of the launch (first[c] a multiple of 8, first[ncls] the launch's block count);

@@ -410,8 +410,10 @@ __device__ __forceinline__ void update_multipliers(const PencilUpdateArgs<T>& u,
 // UA is PencilUpdateArgs<T> for the plain kernel (its argument by value: read through a
 // reference, the kernel argument costs about ten more VGPRs, one wave per SIMD at 256) and
 // const PencilUpdateArgs<T>& for the grouped kernel (arguments read in place from global
-// memory: a copy would live in scratch)
-template <typename T, int N, bool MON, typename UA = PencilUpdateArgs<T>>
+// memory: a copy would live in scratch).  EARLY: the line's loads are issued before the
+// multipliers are computed (their latency hidden behind that arithmetic, for two more lines
+// of registers), for launches whose passes are latency-bound
+template <typename T, int N, bool MON, typename UA = PencilUpdateArgs<T>, bool EARLY = false>
 __device__ __forceinline__ void pencil_update_body(UA u, int blk, int nblk, unsigned char* smem) {
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
     constexpr int V = 1;  // low-register schedule: two lines and the multipliers stay in registers
@@ -452,6 +454,11 @@ __device__ __forceinline__ void pencil_update_body(UA u, int blk, int nblk, unsi
                         (vary == 2 || k2 == 0);
     T s1a[E], s2a[E];
     double m1_0 = 1.0, m2_0 = 0.0;
+    cpx<T> gs[E], c[E];
+    if constexpr (EARLY) {
+        if (u.has_src) load_line<T, E>(gs, warr, base, ti.es, lm.t, TT, valid);
+        load_line<T, E>(c, sarr, base, ti.es, lm.t, TT, valid);
+    }
     const MultLine ml{lm.t, TT, nv, invv, fixed2};
     // the multipliers and their denominators 1 - dt D(k) are built in FP64 at both
     // precisions; FP32-bulk rounds only the finished s1 = m1 - 1 and s2 to FP32
@@ -459,10 +466,9 @@ __device__ __forceinline__ void pencil_update_body(UA u, int blk, int nblk, unsi
     else if (u.dense == 4) update_multipliers<T, E, 4>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
     else update_multipliers<T, E, 0>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
     // 2) explicit source: forward transform along this axis, scale by its multiplier
-    cpx<T> gs[E];
     double g0 = 0.0;
     if (u.has_src) {
-        load_line<T, E>(gs, warr, base, ti.es, lm.t, TT, valid);
+        if constexpr (!EARLY) load_line<T, E>(gs, warr, base, ti.es, lm.t, TT, valid);
         line_fft<T, N, -1, V>(gs, lds + lm.off, lm.es, n, lm.t, a.tw);
         g0 = (double)gs[0].x;
 #pragma unroll
@@ -472,8 +478,7 @@ __device__ __forceinline__ void pencil_update_body(UA u, int blk, int nblk, unsi
         for (int e = 0; e < E; ++e) gs[e] = mk<T>(T(0), T(0));
     }
     // 3) semi-implicit update of the state spectrum, which never leaves k-space
-    cpx<T> c[E];
-    load_line<T, E>(c, sarr, base, ti.es, lm.t, TT, valid);
+    if constexpr (!EARLY) load_line<T, E>(c, sarr, base, ti.es, lm.t, TT, valid);
     double eg = 0.0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -1097,10 +1102,13 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_sync(RowSyncArgs<T> a) {
 
 // real space of the row pass (row r1 in x, row r1 + 1 in y): g from the shifted
 // coefficients with DEG of them (those beyond npoly are zero), optional real-space
-// output and the monitor's FP64 partial sums
+// output c_out and the monitor's FP64 partial sums of the energy density (nbulk
+// coefficients fb, held in registers: read from the arguments inside this loop, they would
+// be read again after every store where the arguments are in global memory)
 template <typename T, int E, int DEG, bool MON>
-__device__ __forceinline__ void rows_pointwise(cpx<T>* v, const T* gp, const RowFusedArgs<T>& a, long r1, int n,
-                                               int t, int TT, bool v1, bool v2, double mu, KSum& sb, KSum& sm) {
+__device__ __forceinline__ void rows_pointwise(cpx<T>* v, const T* gp, T* c_out, const double* fb, int nbulk, long r1,
+                                               int n, int t, int TT, bool v1, bool v2, double mu, KSum& sb,
+                                               KSum& sm) {
     const T muT = (T)mu;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -1113,9 +1121,9 @@ __device__ __forceinline__ void rows_pointwise(cpx<T>* v, const T* gp, const Row
         }
         v[e] = mk<T>(g1, g2);
         const long off = r1 * n + t + TT * e;
-        if (a.c_out) {
-            if (v1) a.c_out[off] = d1 + muT;
-            if (v2) a.c_out[off + n] = d2 + muT;
+        if (c_out) {
+            if (v1) c_out[off] = d1 + muT;
+            if (v2) c_out[off + n] = d2 + muT;
         }
         if (MON) {
 #pragma unroll
@@ -1125,7 +1133,7 @@ __device__ __forceinline__ void rows_pointwise(cpx<T>* v, const T* gp, const Row
                 double f = 0.0;
 #pragma unroll
                 for (int q = kMaxPoly - 1; q >= 0; --q)
-                    if (q < a.nbulk) f = f * cd + a.bulk[q];
+                    if (q < nbulk) f = f * cd + fb[q];
                 sb.add(f);
                 sm.add(cd);
             }
@@ -1133,9 +1141,11 @@ __device__ __forceinline__ void rows_pointwise(cpx<T>* v, const T* gp, const Row
     }
 }
 
-// V: transform schedule (0 plain, 1 low-register); MON: monitor partial sums compiled in
-template <typename T, int N, int V, bool MON>
-__device__ __forceinline__ void rows_fused_body(RowFusedArgs<T> a, int blk, unsigned char* smem) {
+// V: transform schedule (0 plain, 1 low-register); MON: monitor partial sums compiled in.
+// RA is RowFusedArgs<T> (by value) for the plain kernel and const RowFusedArgs<T>& where the
+// arguments are read in place from global memory (the persistent step)
+template <typename T, int N, int V, bool MON, typename RA = RowFusedArgs<T>>
+__device__ __forceinline__ void rows_fused_body(RA a, int blk, unsigned char* smem) {
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
     constexpr int E = LineShape<N, V>::E;
     const int n = (N > 0) ? N : a.nx;
@@ -1176,8 +1186,13 @@ __device__ __forceinline__ void rows_fused_body(RowFusedArgs<T> a, int blk, unsi
         load_herm_pair<T, E, (N > 0 ? LineShape<N, V>::T : 0)>(v, a.s_main, a.s_nyq, r1, n, lm.t, TT, v1, v2);
     line_fft<T, N, +1, V>(v, lds + lm.off, 1, n, lm.t, twl);
     KSum sb, sm;
-    if (a.npoly <= 4) rows_pointwise<T, E, 4, MON>(v, gp, a, r1, n, lm.t, TT, v1, v2, mu, sb, sm);
-    else rows_pointwise<T, E, kMaxPoly, MON>(v, gp, a, r1, n, lm.t, TT, v1, v2, mu, sb, sm);
+    double fb[kMaxPoly];
+    const int nbulk = MON ? a.nbulk : 0;
+#pragma unroll
+    for (int q = 0; q < kMaxPoly; ++q) fb[q] = (MON && q < nbulk) ? a.bulk[q] : 0.0;
+    T* const c_out = a.c_out;
+    if (a.npoly <= 4) rows_pointwise<T, E, 4, MON>(v, gp, c_out, fb, nbulk, r1, n, lm.t, TT, v1, v2, mu, sb, sm);
+    else rows_pointwise<T, E, kMaxPoly, MON>(v, gp, c_out, fb, nbulk, r1, n, lm.t, TT, v1, v2, mu, sb, sm);
     if (MON) {
         double* red = reinterpret_cast<double*>(lds + lds_elems(n, a.TW) + n);
         const double b = block_sum(sb.get(), red);

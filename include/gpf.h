@@ -217,6 +217,17 @@ int gpf_solver_tuning(gpf_solver* s, int* tws, int maxn);
  * run of the same grid and precision) instead of running the autotuner; GPF_EINVAL when
  * the layout is not admissible for this grid and precision */
 int gpf_solver_set_tuning(gpf_solver* s, const int* tws, int n);
+/* The persistent step of square 2D fused grids of side 32, 64, 128 or 256 (no counterpart in
+ * the reference: a launch layout of Solver.Propagate's steps, pf/solver.go:106-120): a run
+ * of steps is one cooperative launch whose workgroups meet at device-wide barriers between
+ * the passes.  mode -1 (default): taken for sides up to 128 when the device places the
+ * launch's workgroups on one XCD; 0: never (per-pass launches and replayed graphs); 1:
+ * wherever admissible.  threads per block: 0 (the default, 64) or 64,
+ * 128, 256, 512; GPF_EINVAL when not admissible for the grid.  A barrier wait that exceeds
+ * its limit ends the launch and the next synchronising call fails with GPF_EDEVICE. */
+int gpf_solver_set_persistent(gpf_solver* s, int mode, int threads);
+/* the number of steps the persistent step has run */
+int gpf_solver_persistent_steps(gpf_solver* s, long* n);
 /* the HIP stream the solver launches on (hipStream_t) */
 void* gpf_solver_stream(gpf_solver* s);
 /* TimeStepper.SetFilter (pf/euler.go:62-64, pf/rk4.go:138-140) with the filter as the
