@@ -1769,6 +1769,12 @@ void Engine<T>::build_persist() {
         }
         P.u[0].egrad_partial = ppc_eg_.get();
         P.u[1].egrad_partial = ppc_eg_.get() + nu;
+        if (P.rows_pass && nu > nr) {
+            // the participant beyond the row blocks writes the slots
+            for (int k = 1; k < 3; ++k) P.ra[k].series = nullptr;
+            P.slot_series = series_.get();
+            P.slot_counter = series_count_.get();
+        }
     }
     if (!pbar_.get()) pbar_.alloc(4);
     if (!pfail_) {

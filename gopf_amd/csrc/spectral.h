@@ -294,6 +294,10 @@ struct PersistPlan {
     PencilUpdateArgs<T> u[2];
     int nblk_rows = 0, nblk_upd = 0;
     int rows_pass = 0;
+    // with more participants than row blocks, the last participant (idle in the row pass)
+    // writes the previous step's series slot from ra[].prev_* (whose series is then null)
+    double* slot_series = nullptr;
+    long* slot_counter = nullptr;
 };
 template <typename T>
 struct PersistLaunch {

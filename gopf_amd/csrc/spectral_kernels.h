@@ -1107,8 +1107,8 @@ __global__ void __launch_bounds__(kMaxBlock) k_rows_sync(RowSyncArgs<T> a) {
 // be read again after every store where the arguments are in global memory)
 template <typename T, int E, int DEG, bool MON>
 __device__ __forceinline__ void rows_pointwise(cpx<T>* v, const T* gp, T* c_out, const double* fb, int nbulk, long r1,
-                                               int n, int t, int TT, bool v1, bool v2, double mu, KSum& sb,
-                                               KSum& sm) {
+                                               int n, int t, int TT, bool v1, bool v2, double mu, double& sb,
+                                               double& sm) {
     const T muT = (T)mu;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -1134,8 +1134,8 @@ __device__ __forceinline__ void rows_pointwise(cpx<T>* v, const T* gp, T* c_out,
 #pragma unroll
                 for (int q = kMaxPoly - 1; q >= 0; --q)
                     if (q < nbulk) f = f * cd + fb[q];
-                sb.add(f);
-                sm.add(cd);
+                sb += f;
+                sm += cd;
             }
         }
     }
@@ -1185,7 +1185,9 @@ __device__ __forceinline__ void rows_fused_body(RA a, int blk, unsigned char* sm
     else
         load_herm_pair<T, E, (N > 0 ? LineShape<N, V>::T : 0)>(v, a.s_main, a.s_nyq, r1, n, lm.t, TT, v1, v2);
     line_fft<T, N, +1, V>(v, lds + lm.off, 1, n, lm.t, twl);
-    KSum sb, sm;
+    // per-thread partials of at most 2 E values: plain FP64 sums (the blocks' partials are
+    // added with compensation)
+    double sb = 0.0, sm = 0.0;
     double fb[kMaxPoly];
     const int nbulk = MON ? a.nbulk : 0;
 #pragma unroll
@@ -1195,8 +1197,8 @@ __device__ __forceinline__ void rows_fused_body(RA a, int blk, unsigned char* sm
     else rows_pointwise<T, E, kMaxPoly, MON>(v, gp, c_out, fb, nbulk, r1, n, lm.t, TT, v1, v2, mu, sb, sm);
     if (MON) {
         double* red = reinterpret_cast<double*>(lds + lds_elems(n, a.TW) + n);
-        const double b = block_sum(sb.get(), red);
-        const double ms = block_sum(sm.get(), red);
+        const double b = block_sum(sb, red);
+        const double ms = block_sum(sm, red);
         if (threadIdx.x == 0) {
             a.bulk_partial[blk] = b;
             a.mass_partial[blk] = ms;

@@ -6,7 +6,8 @@
 // steps: each runs its share of the row pass, all meet at a device-wide barrier, run their
 // share of the update pass, and meet again.  The passes are the bodies of the per-pass
 // kernels (same arithmetic, same partial sums per block of the same layout), with the
-// energy-series slot of step s written by block 0 of step s + 1's row pass.
+// energy-series slot of step s written during step s + 1's row pass (by the participant the
+// row pass leaves idle, else by block 0 of that pass).
 //
 // The barrier is a counter and a generation word in device memory.  Its wait is bounded: a
 // workgroup that waits longer than the
@@ -109,6 +110,9 @@ __global__ void __launch_bounds__(kMaxBlock)
                 rows_fused_body<T, N, 1, MON, const RowFusedArgs<T>&>(ra, b, smem);
                 __syncthreads();
             }
+            if (MON && P.slot_series && s > 0 && pb == (int)np - 1)
+                series_slot(ra.prev_bulk, ra.prev_mass, ra.prev_nb, ra.prev_eg, ra.prev_ne, ra.prev_ceg, P.slot_series,
+                            P.slot_counter);
             if (!persist_barrier<LOCAL>(bar, gen, np, spin_limit, host_fail)) return;
         }
         const PencilUpdateArgs<T>& u = P.u[s & 1];
