@@ -522,8 +522,9 @@ private:
     bool graph_mon_ = false;  // the monitor setting the graph was captured with
     static constexpr int kGraphSteps = 10;
     // largest side taking the persistent step by default: faster than the replayed per-pass
-    // graphs at 32^2-128^2 (FP64 128^2 with the energy series 16.1 against 17.6 us per step),
-    // slower at 256^2 (20.7 against 18.8 us)
+    // graphs at 32^2-128^2 (FP64 128^2 with the energy series 12.9 against 17.0 us per step,
+    // 32^2 10.7 against 14.6), not reliably at 256^2 (FP64 17.0 against 18.1 us with 128
+    // threads per block, FP32 19.7 against 17.4)
     static constexpr int kPersistAutoMax = 128;
     // persistent 2D step: a run of steps in one cooperative launch (spectral_persist.h); its
     // plan (the passes' arguments) is rebuilt whenever the graph is dropped
@@ -539,6 +540,8 @@ private:
     long pnr_ = 0, pnu_ = 0;  // block counts of the plan's row and update passes
     Buf<unsigned> pbar_;
     Buf<double> ppa_bulk_, ppa_mass_, ppc_eg_;
+    Buf<cpx<T>> ptab_;  // the update pass's multipliers of every k-point
+    Buf<double> pm0_;   // m1(0), m2(0)
     int* pfail_ = nullptr;      // mapped host flag set by a failed device-wide barrier
     int* pfail_dev_ = nullptr;
     PersistLaunch<T> pl_;
@@ -1740,6 +1743,13 @@ void Engine<T>::build_persist() {
     const long nr = rows_fused_blocks(g_.rows, tw);
     const long nu = pencil_blocks(u.p.seg, u.p.nseg, tw, &tiles0);
     u.p.tiles0 = tiles0;
+    // the multipliers are the same every step: tabulated once, read with the lines
+    if (!ptab_.get()) ptab_.alloc(g_.nk);
+    if (!pm0_.get()) pm0_.alloc(2);
+    u.mtab[0] = ptab_.get();
+    u.mtab[1] = ptab_.get() + g_.nmain;
+    u.m0 = pm0_.get();
+    HIPCHK(launch_update_mult_table<T>(u, st_));
     P.rows_pass = (fp_.has_src || mon_) ? 1 : 0;
     P.nblk_rows = (int)nr;
     P.nblk_upd = (int)nu;

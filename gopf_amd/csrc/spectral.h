@@ -130,6 +130,10 @@ struct PencilUpdateArgs {
     // rows of kMaxKTerms: indexed by power when dense, else by term (powers from denom / lin)
     const double* denom_sim = nullptr;
     const double* lin_sim = nullptr;
+    // tabulated multipliers (the persistent step): s1 = m1 - 1 and s2 = m2 / N of every
+    // k-point, in the layout of the state spectrum (main, nyq), and m1(0), m2(0) in FP64
+    cpx<T>* mtab[2] = {};
+    double* m0 = nullptr;
 };
 
 // dense form of the update polynomials: the smallest of 2 and 4 that covers every LAP
@@ -314,6 +318,8 @@ struct PersistLaunch {
                                            // on more than one XCD, the steps not run
 };
 bool persist_side(int n);  // the square sides with a persistent step
+// fills u.mtab and u.m0 with the update pass's multipliers (u.p set as for the pass, n = u.p.n)
+template <typename T> hipError_t launch_update_mult_table(const PencilUpdateArgs<T>& u, hipStream_t s);
 template <typename T> void persist_geometry(PersistLaunch<T>& L, int tw);  // threads and smem of L.n, L.mon
 template <typename T> hipError_t launch_fused_persist(const PersistLaunch<T>& L, hipStream_t s);
 

@@ -412,8 +412,10 @@ __device__ __forceinline__ void update_multipliers(const PencilUpdateArgs<T>& u,
 // const PencilUpdateArgs<T>& for the grouped kernel (arguments read in place from global
 // memory: a copy would live in scratch).  EARLY: the line's loads are issued before the
 // multipliers are computed (their latency hidden behind that arithmetic, for two more lines
-// of registers), for launches whose passes are latency-bound
-template <typename T, int N, bool MON, typename UA = PencilUpdateArgs<T>, bool EARLY = false>
+// of registers), for launches whose passes are latency-bound.  MT: 0 the multipliers are
+// computed; 1 they are read from u.mtab / u.m0 with the line's loads; 2 they are computed,
+// written to u.mtab / u.m0, and the pass ends there (the table's own launch)
+template <typename T, int N, bool MON, typename UA = PencilUpdateArgs<T>, bool EARLY = false, int MT = 0>
 __device__ __forceinline__ void pencil_update_body(UA u, int blk, int nblk, unsigned char* smem) {
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
     constexpr int V = 1;  // low-register schedule: two lines and the multipliers stay in registers
@@ -459,12 +461,37 @@ __device__ __forceinline__ void pencil_update_body(UA u, int blk, int nblk, unsi
         if (u.has_src) load_line<T, E>(gs, warr, base, ti.es, lm.t, TT, valid);
         load_line<T, E>(c, sarr, base, ti.es, lm.t, TT, valid);
     }
-    const MultLine ml{lm.t, TT, nv, invv, fixed2};
-    // the multipliers and their denominators 1 - dt D(k) are built in FP64 at both
-    // precisions; FP32-bulk rounds only the finished s1 = m1 - 1 and s2 to FP32
-    if (u.dense == 2) update_multipliers<T, E, 2>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
-    else if (u.dense == 4) update_multipliers<T, E, 4>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
-    else update_multipliers<T, E, 0>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
+    if constexpr (MT == 1) {
+        cpx<T> mt[E];
+        load_line<T, E>(mt, pick(u.mtab[0], u.mtab[1], ti.s), base, ti.es, lm.t, TT, valid);
+        if (has_k0) {
+            m1_0 = u.m0[0];
+            m2_0 = u.m0[1];
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            s1a[e] = mt[e].x;
+            s2a[e] = mt[e].y;
+        }
+    } else {
+        const MultLine ml{lm.t, TT, nv, invv, fixed2};
+        // the multipliers and their denominators 1 - dt D(k) are built in FP64 at both
+        // precisions; FP32-bulk rounds only the finished s1 = m1 - 1 and s2 to FP32
+        if (u.dense == 2) update_multipliers<T, E, 2>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
+        else if (u.dense == 4) update_multipliers<T, E, 4>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
+        else update_multipliers<T, E, 0>(u, ml, dsim, lsim, s1a, s2a, m1_0, m2_0);
+    }
+    if constexpr (MT == 2) {
+        cpx<T> mt[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) mt[e] = mk<T>(s1a[e], s2a[e]);
+        store_line<T, E>(mt, pick(u.mtab[0], u.mtab[1], ti.s), base, ti.es, lm.t, TT, valid);
+        if (has_k0) {
+            u.m0[0] = m1_0;
+            u.m0[1] = m2_0;
+        }
+        return;
+    }
     // 2) explicit source: forward transform along this axis, scale by its multiplier
     double g0 = 0.0;
     if (u.has_src) {

@@ -117,10 +117,38 @@ __global__ void __launch_bounds__(kMaxBlock)
         }
         const PencilUpdateArgs<T>& u = P.u[s & 1];
         for (int b = pb; b < P.nblk_upd; b += (int)np) {
-            pencil_update_body<T, N, MON, const PencilUpdateArgs<T>&, true>(u, b, P.nblk_upd, smem);
+            pencil_update_body<T, N, MON, const PencilUpdateArgs<T>&, true, 1>(u, b, P.nblk_upd, smem);
             __syncthreads();
         }
         if (s + 1 < steps && !persist_barrier<LOCAL>(bar, gen, np, spin_limit, host_fail)) return;
+    }
+}
+
+// the update pass's multipliers of every k-point (the pass in its table-writing form)
+template <typename T, int N>
+__global__ void __launch_bounds__(kMaxBlock) k_update_mult_table(PencilUpdateArgs<T> u) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    pencil_update_body<T, N, false, PencilUpdateArgs<T>, false, 2>(u, blockIdx.x, gridDim.x, smem);
+}
+
+template <typename T>
+hipError_t launch_update_mult_table(const PencilUpdateArgs<T>& u0, hipStream_t s) {
+    PencilUpdateArgs<T> u = u0;
+    u.p.lpt = 1;
+    const long blocks = pencil_blocks(u.p.seg, u.p.nseg, u.p.TW, &u.p.tiles0);
+    const int thr = u.p.TW * threads_per_line(u.p.n, 1);
+    auto go = [&](auto NC) -> hipError_t {
+        constexpr int NN = decltype(NC)::value;
+        auto kern = k_update_mult_table<T, NN>;
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(thr), 0, s, u);
+        return hipGetLastError();
+    };
+    switch (u.p.n) {
+        case 32: return go(std::integral_constant<int, 32>{});
+        case 64: return go(std::integral_constant<int, 64>{});
+        case 128: return go(std::integral_constant<int, 128>{});
+        case 256: return go(std::integral_constant<int, 256>{});
+        default: return hipErrorInvalidValue;
     }
 }
 

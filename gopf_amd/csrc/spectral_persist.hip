@@ -8,5 +8,6 @@ bool persist_side(int n) { return n == 32 || n == 64 || n == 128 || n == 256; }
 
 template hipError_t launch_fused_persist<float>(const PersistLaunch<float>&, hipStream_t);
 template void persist_geometry<float>(PersistLaunch<float>&, int);
+template hipError_t launch_update_mult_table<float>(const PencilUpdateArgs<float>&, hipStream_t);
 
 }  // namespace gpf

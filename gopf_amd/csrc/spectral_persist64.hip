@@ -6,5 +6,6 @@ namespace gpf {
 
 template hipError_t launch_fused_persist<double>(const PersistLaunch<double>&, hipStream_t);
 template void persist_geometry<double>(PersistLaunch<double>&, int);
+template hipError_t launch_update_mult_table<double>(const PencilUpdateArgs<double>&, hipStream_t);
 
 }  // namespace gpf
