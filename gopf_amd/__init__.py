@@ -587,8 +587,8 @@ class Solver:
         return int(lib().gpf_solver_stream(self._h) or 0)
 
     def SetPersistent(self, mode: int = -1, threads: int = 0) -> None:
-        """Persistent step of square 2D grids (side 32-256): -1 auto (sides up to 128), 0 off, 1 on;
-        threads per block (0: 64)."""
+        """Persistent step of square 2D grids (side 32-256): -1 auto (sides up to 128), 0 off, 1 on,
+        2 on with the agent-fenced barrier; threads per block (0: 64)."""
         _check(lib().gpf_solver_set_persistent(self._h, int(mode), int(threads)))
 
     def PersistentSteps(self) -> int:

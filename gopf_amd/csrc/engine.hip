@@ -527,7 +527,9 @@ private:
     // threads per block, FP32 19.7 against 17.4)
     static constexpr int kPersistAutoMax = 128;
     // persistent 2D step: a run of steps in one cooperative launch (spectral_persist.h); its
-    // plan (the passes' arguments) is rebuilt whenever the graph is dropped
+    // plan (the passes' arguments) is rebuilt whenever the graph is dropped.  Modes: -1 where
+    // measured faster, 0 never, 1 wherever admissible, 2 as 1 with the agent-fenced barrier
+    // (the layout taken when the device does not place the participants on one XCD)
     bool persist_eligible();
     void build_persist();
     long run_persist(long n);  // returns the steps left to the per-pass launches
@@ -1710,7 +1712,8 @@ bool Engine<T>::persist_eligible() {
 
 template <typename T>
 void Engine<T>::set_persistent(int mode, int threads) {
-    if (mode < -1 || mode > 1) throw Error(GPF_EINVAL, "persistent mode must be -1 (auto), 0 (off) or 1 (on)");
+    if (mode < -1 || mode > 2)
+        throw Error(GPF_EINVAL, "persistent mode must be -1 (auto), 0 (off), 1 (on) or 2 (on, agent-fenced barrier)");
     if (threads != 0 && threads != 64 && threads != 128 && threads != 256 && threads != 512)
         throw Error(GPF_EINVAL, "threads per block of the persistent step: 0 (64), 64, 128, 256 or 512");
     if (threads > 0 && g_.ndim == 2 && persist_side(g_.nx) && g_.nx == g_.ny) {
@@ -1800,7 +1803,7 @@ void Engine<T>::build_persist() {
     pl_.blocks = (int)(P.rows_pass ? std::max(nr, nu) : nu);
     pl_.bar = pbar_.get();
     pl_.spin_limit = 1u << 20;
-    pl_.local = local_ != 0 ? 1 : 0;
+    pl_.local = (local_ != 0 && persist_mode_ != 2) ? 1 : 0;
     pl_.host_fail = pfail_dev_;
     pnr_ = nr;
     pnu_ = nu;

@@ -222,7 +222,8 @@ int gpf_solver_set_tuning(gpf_solver* s, const int* tws, int n);
  * of steps is one cooperative launch whose workgroups meet at device-wide barriers between
  * the passes.  mode -1 (default): taken for sides up to 128 when the device places the
  * launch's workgroups on one XCD; 0: never (per-pass launches and replayed graphs); 1:
- * wherever admissible.  threads per block: 0 (the default, 64) or 64,
+ * wherever admissible; 2: as 1 with agent-scope fences at every barrier (the layout used
+ * when the workgroups are not on one XCD).  threads per block: 0 (the default, 64) or 64,
  * 128, 256, 512; GPF_EINVAL when not admissible for the grid.  A barrier wait that exceeds
  * its limit ends the launch and the next synchronising call fails with GPF_EDEVICE. */
 int gpf_solver_set_persistent(gpf_solver* s, int mode, int threads);

@@ -66,6 +66,19 @@ def test_persistent_matches_per_pass(side, threads, precision):
     assert got[4] == ref[4]
 
 
+@pytest.mark.parametrize("side,precision", [(128, "fp64"), (64, "fp32")])
+def test_agent_fenced_barrier_matches_one_xcd_barrier(side, precision):
+    """The barrier layout taken when the workgroups are not on one XCD (agent-scope fences,
+    every block of the grid a participant) gives the one-XCD layout's results bit for bit."""
+    dims = [side, side]
+    c0 = ch_init(side * side, 16)
+    a = run(c0, dims, precision, 2, 0, [17, 2])
+    b = run(c0, dims, precision, 1, 0, [17, 2])
+    assert a[3] == b[3] == 24
+    assert np.array_equal(a[0], b[0])
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+
+
 def test_persistent_runs_split_across_calls_equal_one_run():
     """Launch boundaries move nothing: 7 + 13 + 3 steps equal 23 steps bit for bit, series
     included (the slot of each launch's last step is written after it)."""
@@ -105,7 +118,7 @@ def test_persistent_settings_are_checked():
     c0 = ch_init(64 * 64, 14)
     s = G.Solver(ch_model(c0), [64, 64], 0.05)
     with pytest.raises(G.GpfError):
-        s.SetPersistent(2)
+        s.SetPersistent(3)
     with pytest.raises(G.GpfError):
         s.SetPersistent(1, 96)
     # a grid without the persistent step (not square) runs the per-pass launches
