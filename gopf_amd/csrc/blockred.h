@@ -8,13 +8,16 @@ namespace gpf {
 // Block-wide FP64 sum, returned to every thread; red must hold blockDim.x doubles.  Blocks of
 // whole wavefronts sum each wavefront with cross-lane exchanges and then one value per
 // wavefront through LDS (two barriers); others use a tree in LDS.  Every thread adds the
-// same values in the same order, so the result is deterministic.
+// same values in the same order, so the result is deterministic.  A block of one wavefront
+// needs no LDS round: after the exchanges every lane holds the same sum (each pair of lanes
+// adds the same two operands at every stage).
 __device__ __forceinline__ double block_sum(double v, double* red) {
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
     if ((nt & 63) == 0) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (nt == 64) return v;
         if ((tid & 63) == 0) red[tid >> 6] = v;
         __syncthreads();
         double r = 0.0;
