@@ -587,8 +587,8 @@ def main() -> int:
                          "sweep; c4 192^3 elastic RK4; c5 512^2 adjoint; ie 128^2 CH ImplicitEuler; io snapshots")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: the multi-GPU slab decomposition is "
-                                   "design-only (DESIGN.md); run with --gpus 1"}))
+        print(json.dumps({"error": "this library runs on one GPU: the multi-GPU slab decomposition along z "
+                                   "is not implemented; run with --gpus 1"}))
         return 2
     import gopf_amd as G
     from gopf_amd import workloads as W
