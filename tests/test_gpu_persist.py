@@ -114,6 +114,48 @@ def test_persistent_fp64_matches_oracle(oracle_lib, side, steps):
     assert np.abs(s.Field(0).real - ref).max() <= 1e-10 * np.abs(ref).max()
 
 
+def plain_run(eq, scalars, c0, dims, precision, mode, steps):
+    m = G.Model()
+    m.AddField("conc", c0)
+    for k, v in scalars.items():
+        m.AddScalar(k, v)
+    m.AddEquation(eq)
+    m.Init()
+    s = G.Solver(m, dims, 0.05, precision=precision)
+    s.SetPersistent(mode)
+    s.StepperPropagate(steps)
+    return s, s.Field(0).real
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_persistent_without_source_matches_per_pass(precision):
+    """Pure diffusion has no explicit source and no monitor: the launch runs update passes
+    only, one barrier per step.  Mass is conserved."""
+    dims = [64, 64]
+    c0 = ch_init(64 * 64, 17)
+    s1, a = plain_run("dconc/dt = LAP conc", {}, c0, dims, precision, 1, 30)
+    _, b = plain_run("dconc/dt = LAP conc", {}, c0, dims, precision, 0, 30)
+    assert s1.fused and s1.PersistentSteps() == 30
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    assert np.abs(a - b).max() <= tol * np.abs(b).max()
+    assert abs(a.sum() - c0.sum()) <= 1e-9 * c0.sum()
+
+
+@pytest.mark.parametrize("side", [64, 128])
+def test_persistent_moving_mean_matches_per_pass(side):
+    """A non-conserved model (an implicit linear term and an explicit cubic source without
+    a Laplacian): the k = 0 mode moves every step through the tabulated m1(0), m2(0)."""
+    eq = "dconc/dt = a*conc - b*conc^3 + LAP conc"
+    sc = {"a": 0.5, "b": 1.0}
+    dims = [side, side]
+    c0 = ch_init(side * side, 18)
+    s1, a = plain_run(eq, sc, c0, dims, "fp64", 1, 40)
+    _, b = plain_run(eq, sc, c0, dims, "fp64", 0, 40)
+    assert s1.fused and s1.PersistentSteps() == 40
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+    assert abs(a.mean() - c0.mean()) > 1e-3
+
+
 def test_persistent_settings_are_checked():
     c0 = ch_init(64 * 64, 14)
     s = G.Solver(ch_model(c0), [64, 64], 0.05)
