@@ -263,6 +263,15 @@ static int choose_tw(int n, int narr, size_t elem, int min_tw, int variant = 0, 
     return tw;
 }
 
+// lines per block of a general-path pass over `lines` lines: a small grid's pass is
+// latency-bound, so fewer lines per block spread it over at least kSpreadBlocks workgroups
+// (at 128^2 the row transform ran on 4 workgroups); large grids keep tw
+constexpr long kSpreadBlocks = 256;
+static int spread_tw(int tw, long lines) {
+    while (tw > 1 && (lines + tw - 1) / tw < kSpreadBlocks) tw /= 2;
+    return tw;
+}
+
 static Seg seg_y_main(const Geometry& g) {
     Seg s;
     s.na = g.nz; s.nb = g.nxh; s.sa = (long)g.ny * g.nxh; s.sb = 1; s.es = g.nxh;
@@ -965,7 +974,8 @@ void Engine<T>::pencils(const std::vector<cpx<T>*>& dst, const std::vector<const
         a.nseg = 2;
         // about 128 threads per block: at 192^3 (C4) 8 lines per block beat 16 by 1 % (FP32)
         // and 3 % (FP64) per step
-        a.TW = choose_tw(n, a.narr, sizeof(cpx<T>), 8, 0, 128);
+        a.TW = spread_tw(choose_tw(n, a.narr, sizeof(cpx<T>), 8, 0, 128),
+                         (long)a.seg[0].na * a.seg[0].nb + (long)a.seg[1].na * a.seg[1].nb);
         for (int q = 0; q < a.narr; ++q) {
             a.arr[q][0] = dst[q0 + q];
             a.arr[q][1] = dst[q0 + q] + g_.nmain;
@@ -1008,7 +1018,7 @@ void Engine<T>::r2c(const std::vector<const T*>& in, const std::vector<cpx<T>*>&
         }
         a.rows = g_.rows;
         a.nx = g_.nx;
-        a.TW = choose_tw(g_.nx, (a.narr + 1) / 2, sizeof(cpx<T>), 1);
+        a.TW = spread_tw(choose_tw(g_.nx, (a.narr + 1) / 2, sizeof(cpx<T>), 1), g_.rows);
         a.in_scale = in_scale;
         HIPCHK(launch_rows_r2c<T>(a, st_));
     }
@@ -1044,7 +1054,7 @@ void Engine<T>::c2r(const std::vector<const cpx<T>*>& in, const std::vector<T*>&
         }
         a.rows = g_.rows;
         a.nx = g_.nx;
-        a.TW = choose_tw(g_.nx, (na + 1) / 2, sizeof(cpx<T>), 1);
+        a.TW = spread_tw(choose_tw(g_.nx, (na + 1) / 2, sizeof(cpx<T>), 1), g_.rows);
         a.scale = scale;
         a.add_mean = (q0 == 0) ? add_mean : nullptr;
         HIPCHK(launch_rows_c2r<T>(a, st_));
@@ -1981,7 +1991,7 @@ void Engine<T>::eval_elastic() {
         r.dnyq[0] = Tel_[e].get() + g_.nmain;
         r.rows = g_.rows;
         r.nx = g_.nx;
-        r.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
+        r.TW = spread_tw(choose_tw(g_.nx, 1, sizeof(cpx<T>), 1), g_.rows);
         HIPCHK(launch_rows_sync<T>(r, st_));
         pencils({Tel_[e].get()}, {}, 1, -1);
         if (g_.ndim == 3) pencils({Tel_[e].get()}, {}, 2, -1);
@@ -2110,7 +2120,7 @@ void Engine<T>::sync_from_spectra(const PencilStageArgs<T>* stage) {
     }
     a.rows = g_.rows;
     a.nx = g_.nx;
-    a.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
+    a.TW = spread_tw(choose_tw(g_.nx, 1, sizeof(cpx<T>), 1), g_.rows);
     HIPCHK(launch_rows_sync<T>(a, st_));
     pencils(out, {}, 1, -1);
     if (g_.ndim == 3) pencils(out, {}, 2, -1);
@@ -2949,7 +2959,7 @@ void Engine<T>::adjoint(long nsteps, long every, const double* target, double* l
                 r.dnyq[0] = Lk.get() + g_.nmain;
                 r.rows = g_.rows;
                 r.nx = g_.nx;
-                r.TW = choose_tw(g_.nx, 1, sizeof(cpx<T>), 1);
+                r.TW = spread_tw(choose_tw(g_.nx, 1, sizeof(cpx<T>), 1), g_.rows);
                 HIPCHK(launch_rows_sync<T>(r, st_));
                 pencils({Lk.get()}, {}, 1, -1);
                 if (g_.ndim == 3) pencils({Lk.get()}, {}, 2, -1);
