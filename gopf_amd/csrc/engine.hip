@@ -267,8 +267,8 @@ static int choose_tw(int n, int narr, size_t elem, int min_tw, int variant = 0, 
 // latency-bound, so fewer lines per block spread it over at least kSpreadBlocks workgroups
 // (at 128^2 the row transform ran on 4 workgroups); large grids keep tw
 constexpr long kSpreadBlocks = 256;
-static int spread_tw(int tw, long lines) {
-    while (tw > 1 && (lines + tw - 1) / tw < kSpreadBlocks) tw /= 2;
+static int spread_tw(int tw, long lines, int min_tw = 1) {
+    while (tw > min_tw && (lines + tw - 1) / tw < kSpreadBlocks) tw /= 2;
     return tw;
 }
 
@@ -1282,7 +1282,9 @@ RowFusedArgs<T> Engine<T>::rows_args(cpx<T>* W, double* mean, int tw, int var) {
     ra.g_nyq = W + g_.nmain;
     ra.rows = g_.rows;
     ra.nx = g_.nx;
-    ra.TW = tw > 0 ? tw : choose_tw(g_.nx, 1, sizeof(cpx<T>), 1, var);
+    // untuned: spread over more workgroups on small grids, down to the autotuner's smallest
+    // candidate (4 lines per block)
+    ra.TW = tw > 0 ? tw : spread_tw(choose_tw(g_.nx, 1, sizeof(cpx<T>), 1, var), (g_.rows + 1) / 2, 4);
     ra.variant = var;
     ra.c_out = c_out_;
     ra.npoly = (int)fp_.gpoly.size();
@@ -1305,7 +1307,9 @@ PencilUpdateArgs<T> Engine<T>::update_args(cpx<T>* S, cpx<T>* W, double* mean, i
     u.p.seg[0] = (axis == 2) ? seg_z_main(g_) : seg_y_main(g_);
     u.p.seg[1] = (axis == 2) ? seg_z_nyq(g_) : seg_y_nyq(g_);
     u.p.nseg = 2;
-    u.p.TW = tw > 0 ? tw : choose_tw(nu, u.p.narr, sizeof(cpx<T>), 8, 1);
+    u.p.TW = tw > 0 ? tw
+                    : spread_tw(choose_tw(nu, u.p.narr, sizeof(cpx<T>), 8, 1),
+                                (long)u.p.seg[0].na * u.p.seg[0].nb + (long)u.p.seg[1].na * u.p.seg[1].nb, 4);
     u.g = g_.kg;
     u.dt = dt_;
     u.scale = 1.0 / (double)g_.n;
@@ -1334,7 +1338,9 @@ void Engine<T>::pencils_tw(const std::vector<cpx<T>*>& dst, int axis, int dir, i
     a.seg[0] = (axis == 1) ? seg_y_main(g_) : seg_z_main(g_);
     a.seg[1] = (axis == 1) ? seg_y_nyq(g_) : seg_z_nyq(g_);
     a.nseg = 2;
-    a.TW = tw > 0 ? tw : choose_tw(n, a.narr, sizeof(cpx<T>), 8, var);
+    a.TW = tw > 0 ? tw
+                  : spread_tw(choose_tw(n, a.narr, sizeof(cpx<T>), 8, var),
+                              (long)a.seg[0].na * a.seg[0].nb + (long)a.seg[1].na * a.seg[1].nb, 4);
     a.lpt = lpt;
     a.variant = var;
     for (int q = 0; q < a.narr; ++q) {
