@@ -532,8 +532,8 @@ private:
     static constexpr int kGraphSteps = 10;
     // largest side taking the persistent step by default: faster than the replayed per-pass
     // graphs at 32^2-128^2 (FP64 128^2 with the energy series 12.9 against 17.0 us per step,
-    // 32^2 10.7 against 14.6), not reliably at 256^2 (FP64 17.0 against 18.1 us with 128
-    // threads per block, FP32 19.7 against 17.4)
+    // 32^2 10.7 against 14.6), slower at 256^2 (FP64 17.0 us with 128 threads per block
+    // against 13.7-14.3 us for the tuned graphs)
     static constexpr int kPersistAutoMax = 128;
     // persistent 2D step: a run of steps in one cooperative launch (spectral_persist.h); its
     // plan (the passes' arguments) is rebuilt whenever the graph is dropped.  Modes: -1 where
