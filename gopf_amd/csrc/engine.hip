@@ -681,7 +681,7 @@ private:
 
 template <typename T>
 Engine<T>::Engine(const Model& m, const int* dims, int ndim, double dt, const gpf_options& o)
-    : dt_(dt), stepper_(o.stepper), mcopy_(m), autotune_(o.autotune != 0) {
+    : autotune_(o.autotune != 0), dt_(dt), stepper_(o.stepper), mcopy_(m) {
     g_.init(dims, ndim);
     HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
     {
