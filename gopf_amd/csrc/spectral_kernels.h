@@ -1157,10 +1157,12 @@ __device__ __forceinline__ void rows_pointwise(cpx<T>* v, const T* gp, T* c_out,
             for (int h = 0; h < 2; ++h) {
                 if (h == 0 ? !v1 : !v2) continue;
                 const double cd = (double)(h == 0 ? d1 : d2) + mu;
+                // Horner over all kMaxPoly coefficients (zero from nbulk on): the leading
+                // zero terms leave f = 0 * cd + 0 until the first nonzero one, exactly as a
+                // start at q = nbulk - 1 would, with no branch per coefficient
                 double f = 0.0;
 #pragma unroll
-                for (int q = kMaxPoly - 1; q >= 0; --q)
-                    if (q < nbulk) f = f * cd + fb[q];
+                for (int q = kMaxPoly - 1; q >= 0; --q) f = f * cd + fb[q];
                 sb += f;
                 sm += cd;
             }
